@@ -1,0 +1,173 @@
+/*
+ * partmc_coag.h -- C ABI of the MI355X (gfx950) particle-resolved Monte
+ * Carlo coagulation step.
+ *
+ * The library replaces PartMC's mc_coag (src/coagulation.F90:43-96), which
+ * run_part_timestep calls once per timestep (src/run_part.F90:919-920).
+ * The calling convention follows the reference's only FFI precedent, the
+ * ISO_C_BINDING interfaces to its GSL wrapper (src/rand.F90:82-86 and
+ * src/rand_gsl.c:23-34): plain C types, integer status return, 0 = success.
+ * A positive return is either one of the reference's own error codes
+ * (e.g. 765620746 "particle outside of bin_grid", src/coagulation.F90:561)
+ * or one of the PMC_ERR_* codes below; pmc_gpu_last_error() gives the text.
+ *
+ * Ownership: the caller owns the particle state between calls (Fortran's
+ * aero_state); the device holds a mirror per box.  Upload before and
+ * download after mc_coag when other processes touch the particles.
+ *
+ * Array conventions follow Fortran: species volumes are vol(n_spec, n_part)
+ * column-major (vol(:, i) is particle i), weights are
+ * weight(n_group, n_class) column-major, groups/classes/bins are 1-based.
+ *
+ * Threading: one handle per host thread; a handle owns one HIP stream.
+ */
+#ifndef PARTMC_COAG_H
+#define PARTMC_COAG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pmc_gpu pmc_gpu;
+
+/* library error codes (reference error codes pass through unchanged) */
+#define PMC_ERR_ARG 1          /* invalid argument */
+#define PMC_ERR_HIP 2          /* HIP runtime failure */
+#define PMC_ERR_CAPACITY 3     /* particle or list capacity exceeded */
+#define PMC_ERR_UNSUPPORTED 4  /* kernel type or option not supported */
+#define PMC_ERR_STATE 5        /* call out of order (e.g. no particles) */
+
+/* execution modes */
+#define PMC_MODE_EXACT 0     /* sequential-order replay of mc_coag, one
+                                workgroup per box; bit-identical to the
+                                sequential algorithm for the same streams */
+#define PMC_MODE_PARALLEL 1  /* all bin pairs and test pairs in parallel,
+                                deterministic claim protocol for conflicts */
+
+/* precision of the test-pair kernel evaluation (parallel mode) */
+#define PMC_PREC_FP64 0          /* reference-style FP64 from raw volumes */
+#define PMC_PREC_FP32_RECHECK 1  /* FP32 pair combine on FP64 per-particle
+                                    terms; FP64 re-evaluation whenever
+                                    |u - k/k_max| < band */
+
+/* Kernel type codes are PartMC's (src/coag_kernel.F90:29-45):
+ * 1 sedi, 2 additive, 3 constant, 4 brown, 5 zero.  6 and 7 (fractal
+ * brown_free / brown_cont) return PMC_ERR_UNSUPPORTED. */
+
+/* Create a handle on `device` for `n_box` independent boxes sharing one
+ * aero_data (n_spec species of the given densities, kg/m^3).  `max_bin`
+ * bounds the size of any bin grid (0 = 256).  Replaces the setup that
+ * aero_data_t / aero_state_t carry (src/aero_data.F90:55-97,
+ * src/aero_state.F90:69-86). */
+int pmc_gpu_create(pmc_gpu **out, int device, int n_box, int n_spec,
+                   const double *density, int max_bin);
+int pmc_gpu_destroy(pmc_gpu *h);
+const char *pmc_gpu_last_error(void);
+
+/* env_state_t fields read by the kernels: temp (K), pressure (Pa) and
+ * additive_kernel_coefficient (s^-1) (src/env_state.F90:29-78). */
+int pmc_gpu_set_env(pmc_gpu *h, int box, double temp, double pressure,
+                    double additive_coeff);
+
+/* aero_weight_array_t (src/aero_weight_array.F90:35-39): arrays of
+ * n_group*n_class in Fortran order; type codes of src/aero_weight.F90:25-29
+ * (1 none, 2 power, 3 mfa). */
+int pmc_gpu_set_weights(pmc_gpu *h, int box, int n_group, int n_class,
+                        const int *type, const double *magnitude,
+                        const double *exponent);
+
+/* Seed of the counter-based streams and the index of the next step.
+ * pmc_srand (src/rand.F90:66-113) seeds the reference's sequential
+ * generator; here every call site of a step has its own Philox stream. */
+int pmc_gpu_set_rng(pmc_gpu *h, uint64_t seed, uint32_t step);
+
+/* next_id of aero_particle_new_id (src/aero_particle.F90:72,157-165) for
+ * one box; new ids are (next_id - 1) * id_stride + id_offset + 1. */
+int pmc_gpu_set_next_id(pmc_gpu *h, int box, int64_t next_id,
+                        int64_t id_stride, int64_t id_offset);
+int pmc_gpu_get_next_id(pmc_gpu *h, int box, int64_t *next_id);
+
+/* Replace one box's particles (aero_state%apa, src/aero_particle_array.F90).
+ * capacity >= n_part reserves room for particles created during steps.
+ * n_comp/comp_source/comp_time may be NULL (one component of source 0 at
+ * time 0); comp arrays are (20, n_part) column-major.  least/greatest
+ * create times may be NULL (0).  Invalidates the box's sort. */
+int pmc_gpu_upload(pmc_gpu *h, int box, int n_part, int capacity,
+                   const double *vol, const int64_t *id, const int *group,
+                   const int *klass, const int *n_primary, const int *n_comp,
+                   const int *comp_source, const double *comp_time,
+                   const double *least_create_time,
+                   const double *greatest_create_time);
+
+/* aero_state_sort with an explicit log bin grid of n_bin bins over radii
+ * [r_min, r_max] (src/aero_state.F90:3291-3306, src/aero_sorted.F90:254-262):
+ * particles outside the grid are discarded, the rest sorted. */
+int pmc_gpu_set_bin_grid(pmc_gpu *h, int box, int n_bin, double r_min,
+                         double r_max);
+
+int pmc_gpu_get_n_part(pmc_gpu *h, int box, int *n_part);
+
+/* Copy one box's particles back (any pointer may be NULL). */
+int pmc_gpu_download(pmc_gpu *h, int box, int max_part, int *n_part,
+                     double *vol, int64_t *id, int *group, int *klass,
+                     int *n_primary, int *n_comp, int *comp_source,
+                     double *comp_time, double *least_create_time,
+                     double *greatest_create_time);
+
+/* Execution mode and precision (defaults: PMC_MODE_EXACT, PMC_PREC_FP64,
+ * band 1e-5). */
+int pmc_gpu_set_mode(pmc_gpu *h, int mode, int precision, double recheck_band);
+
+/* mc_coag (src/coagulation.F90:43-96) for every box: one timestep of
+ * length del_t.  tot_n_samp / tot_n_coag (arrays of n_box, may be NULL)
+ * receive the reference's counters per box (src/coagulation.F90:57-59).
+ * Blocks until the step is complete. */
+int pmc_gpu_mc_coag(pmc_gpu *h, int kernel_type, double del_t,
+                    int64_t *tot_n_samp, int64_t *tot_n_coag);
+
+/* The same step enqueued on the handle's stream without waiting; counters
+ * accumulate on the device (read them with pmc_gpu_get_counters). */
+int pmc_gpu_mc_coag_async(pmc_gpu *h, int kernel_type, double del_t);
+int pmc_gpu_sync(pmc_gpu *h);
+/* Per-box counters accumulated since the last reset: test pairs, accepted
+ * coagulations, kmax warnings (src/coagulation.F90:749-751), FP64 rechecks,
+ * decision flips and max relative FP32 kernel error (validation mode). */
+int pmc_gpu_get_counters(pmc_gpu *h, int64_t *n_samp, int64_t *n_coag,
+                         int64_t *n_warn, int64_t *n_recheck,
+                         int64_t *n_flip, double *max_rel_err);
+int pmc_gpu_reset_counters(pmc_gpu *h);
+/* validation on: every FP32 test is also evaluated in FP64 to count decision
+ * flips and the max relative kernel error */
+int pmc_gpu_set_validation(pmc_gpu *h, int on);
+
+/* Inspection (tests): the box's bin grid, (bin, class) list counts and
+ * entries (exact mode; 0-based particle indices, class-major then bin),
+ * and the cached k_min/k_max tables (n_bin x n_bin, column-major). */
+int pmc_gpu_get_grid(pmc_gpu *h, int box, int *n_bin, double *edges,
+                     double *centers, double *widths);
+int pmc_gpu_get_lists(pmc_gpu *h, int box, int *counts, int *entries);
+int pmc_gpu_get_kbounds(pmc_gpu *h, int box, double *kmin, double *kmax);
+/* Removal records (aero_info_array, src/aero_info.F90:48-56) of one box
+ * since the last upload; returns the count in *n. */
+int pmc_gpu_get_removed(pmc_gpu *h, int box, int max_n, int *n, int64_t *id,
+                        int *action, int64_t *other_id);
+
+/* Kernel evaluation on the device for n particle pairs (tests and
+ * golden vectors): kernel(type, p1, p2) of src/coag_kernel.F90:83-125 in
+ * FP64 and, when k32 is not NULL, the FP32 pair-combine value. */
+int pmc_gpu_eval_kernel(pmc_gpu *h, int kernel_type, int n,
+                        const double *vol1, const double *vol2, double temp,
+                        double pressure, double beta, double *k64, float *k32);
+/* Uniforms / Poisson / binomial variates of one call-site stream, for
+ * testing the device generators against the oracle's. which: 0 uniform,
+ * 1 poisson(param1), 2 binomial(int(param1), param2), 3 rand_int(param1). */
+int pmc_gpu_rng_draw(pmc_gpu *h, uint64_t seed, uint32_t box, uint32_t step,
+                     uint32_t site, uint32_t ordinal, uint32_t kind, int which,
+                     double param1, double param2, int n, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

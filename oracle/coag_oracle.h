@@ -1,0 +1,183 @@
+/*
+ * coag_oracle.h -- CPU oracle for the particle-resolved Monte Carlo
+ * coagulation step (PartMC mc_coag).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a sequential FP64 restatement of the
+ * reference algorithm, used as the checker for the HIP implementation in
+ * partmc_amd/.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product path never calls it.
+ *
+ * Every function cites the reference file:line it restates (paths are
+ * relative to the PartMC source tree).
+ *
+ * Random numbers: the reference draws from one sequential stream
+ * (src/rand.F90:138-207).  The oracle keeps the reference's *order of draws
+ * within each call site* but addresses each call site with its own
+ * counter-based Philox4x32-10 stream (key = seed and box, counter = step,
+ * site, ordinal, kind).  The same addressing is used by the GPU code, which
+ * is what makes a step replayable bit-for-bit on the device.
+ */
+#ifndef PMC_COAG_ORACLE_H
+#define PMC_COAG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_SPEC 32
+#define ORC_MAX_COMP 20          /* MAX_AERO_COMPONENT_SIZE, src/aero_particle.F90:22 */
+#define ORC_MAX_GROUP 8
+#define ORC_MAX_CLASS 8
+
+/* kernel type codes, src/coag_kernel.F90:29-45 */
+#define ORC_KERNEL_SEDI 1
+#define ORC_KERNEL_ADDITIVE 2
+#define ORC_KERNEL_CONSTANT 3
+#define ORC_KERNEL_BROWN 4
+#define ORC_KERNEL_ZERO 5
+
+/* weight type codes, src/aero_weight.F90:25-29 */
+#define ORC_WEIGHT_NONE 1
+#define ORC_WEIGHT_POWER 2
+#define ORC_WEIGHT_MFA 3
+
+/* removal actions, src/aero_info.F90:19-29 */
+#define ORC_INFO_COAG 2
+#define ORC_INFO_WEIGHT 4
+
+/* RNG stream kinds (the call sites of mc_coag) */
+#define ORC_RNG_POISSON 1      /* compute_n_samp: rand_poisson            */
+#define ORC_RNG_TEST 2         /* maybe_coag_pair: 2 rand_int + 1 random   */
+#define ORC_RNG_MERGE 3        /* coagulate: weighting + component draws   */
+#define ORC_RNG_ACC_POISSON 4  /* sample_source_particle: 2 rand_poisson   */
+#define ORC_RNG_ACC_SAMPLE 5   /* sample_source_particle: per-sample draws */
+#define ORC_RNG_ACC_FINISH 6   /* binomial, prob_round, resampling         */
+#define ORC_RNG_ACC_TARGET 7   /* coag_target_with_source: merge, groups, dups */
+
+/* ---- Philox4x32-10 and the per-site uniform stream ---- */
+typedef struct {
+    uint32_t key[2];
+    uint32_t ctr[4];       /* ctr[3] low 24 bits = block number */
+    uint32_t buf[4];
+    int have;              /* uniforms left in buf (0..2) */
+    uint32_t block;
+} orc_stream;
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void orc_stream_open(orc_stream *s, uint64_t seed, uint32_t box, uint32_t step,
+                     uint32_t site, uint32_t ordinal, uint32_t kind);
+double orc_stream_uniform(orc_stream *s);
+int orc_rand_int(orc_stream *s, int n);
+int orc_rand_poisson(orc_stream *s, double mean);
+int orc_rand_binomial(orc_stream *s, int n, double p);
+double orc_rand_normal(orc_stream *s, double mean, double stddev);
+int orc_prob_round(orc_stream *s, double val);
+
+/* ---- pure numerics (golden-vector entry points) ---- */
+double orc_sphere_vol2rad(double v);
+double orc_sphere_rad2vol(double r);
+double orc_powi(double x, int n);
+double orc_fast_pow(double base, double exponent);
+double orc_interp_linear_disc(double x1, double xn, int n, int i);
+int orc_linspace_find(double min_x, double max_x, int n, double x);
+int orc_logspace_find(double min_x, double max_x, int n, double x);
+void orc_logspace(double min_x, double max_x, int n, double *out);
+void orc_bin_grid_make_log(int n_bin, double rmin, double rmax,
+                           double *edges, double *centers, double *widths);
+int orc_bin_grid_find(int n_bin, const double *edges, double r);
+double orc_student_t_95_coeff(int n_sample);
+void orc_fall_g(double r, double *w_inf);
+int orc_effic(double r1, double r2, double *ec);
+double orc_kernel_brown_helper(double vol_i, double den_i, double vol_j,
+                               double den_j, double temp, double pressure);
+int orc_kernel_sedi_helper(double v1, double v2, double *k);
+/* kernel between two particles given species volumes (src/coag_kernel.F90:83-125) */
+int orc_kernel(int type, int n_spec, const double *density, const double *vol1,
+               const double *vol2, double temp, double pressure, double beta,
+               double *k);
+int orc_kernel_minmax(int type, int n_spec, const double *density, double v1,
+                      double v2, double temp, double pressure, double beta,
+                      double *kmin, double *kmax);
+int orc_est_k_minmax_binned(int type, int n_spec, const double *density,
+                            int n_bin, const double *edges, double temp,
+                            double pressure, double beta, double *kmin,
+                            double *kmax);
+/* weights: arrays of n_group*n_class, index g + n_group*c (Fortran order) */
+double orc_num_conc_at_radius(int n_group, int n_class, const int *wtype,
+                              const double *wmag, const double *wexp,
+                              int i_class, double r);
+double orc_coag_num_conc_factor(int n_group, int n_class, const int *wtype,
+                                const double *wmag, const double *wexp,
+                                double i_r, double j_r, int i_class,
+                                int j_class, int ij_class);
+int orc_coag_dest_class(int n_group, int n_class, const int *wtype,
+                        const double *wmag, const double *wexp,
+                        const double *centers, int i_bin, int j_bin,
+                        int i_class, int j_class);
+double orc_max_coag_num_conc_factor(int n_group, int n_class, const int *wtype,
+                                    const double *wmag, const double *wexp,
+                                    const double *edges, int i_bin, int j_bin,
+                                    int i_class, int j_class, int ij_class);
+double orc_bessi1(double x);
+void orc_soln_additive_exp(int n_bin, const double *centers, double time,
+                           double num_conc, double radius_at_mean_vol,
+                           double beta, double *num_conc_out,
+                           double *vol_conc_out);
+void orc_soln_constant_exp(int n_bin, const double *centers, double time,
+                           double num_conc, double radius_at_mean_vol,
+                           double *num_conc_out, double *vol_conc_out);
+void orc_histogram_1d(int n_bin, const double *edges, const double *widths,
+                      int n, const double *x, const double *w, double *out);
+
+/* ---- the particle box (aero_state restatement) ---- */
+typedef struct orc_box orc_box;
+
+orc_box *orc_box_create(int n_spec, const double *density, int n_group,
+                        int n_class, const int *wtype, const double *wmag,
+                        const double *wexp);
+void orc_box_destroy(orc_box *b);
+void orc_box_set_env(orc_box *b, double temp, double pressure, double beta);
+void orc_box_set_rng(orc_box *b, uint64_t seed, uint32_t box_index);
+void orc_box_set_next_id(orc_box *b, int64_t next_id);
+int64_t orc_box_get_next_id(const orc_box *b);
+/* aero_state_sort with an explicit bin grid (src/aero_state.F90:3291-3306
+   with bin_grid present, src/aero_sorted.F90:254-262) */
+int orc_box_sort_with_grid(orc_box *b, int n_bin, double rmin, double rmax);
+/* append particles; vol is particle-major [n][n_spec]; comp arrays are
+   [n][ORC_MAX_COMP] (may be NULL: one component of source 0, time 0) */
+int orc_box_add_particles(orc_box *b, int n, const double *vol,
+                          const int64_t *id, const int *group,
+                          const int *klass, const int *n_primary,
+                          const int *n_comp, const int *comp_source,
+                          const double *comp_time);
+int orc_box_n_part(const orc_box *b);
+void orc_box_get_particles(const orc_box *b, double *vol, int64_t *id,
+                           int *group, int *klass, int *n_primary,
+                           int *n_comp, int *comp_source, double *comp_time);
+int orc_box_n_bin(const orc_box *b);
+void orc_box_get_grid(const orc_box *b, double *edges, double *centers,
+                      double *widths);
+/* counts[b + n_bin*c]; entries[...] concatenated in (bin-major within class
+   order c-major) -- entries are 0-based particle indices */
+void orc_box_get_lists(const orc_box *b, int *counts, int *entries);
+void orc_box_get_kmax(const orc_box *b, double *kmin, double *kmax);
+/* removal log (aero_info_array) */
+int orc_box_n_info(const orc_box *b);
+void orc_box_get_info(const orc_box *b, int64_t *id, int *action,
+                      int64_t *other_id);
+void orc_box_clear_info(orc_box *b);
+/* counters of "kernel upper bound estimation is too tight" warnings */
+int64_t orc_box_n_warn_kmax(const orc_box *b);
+/* whether the last mc_coag call made a new grid */
+int orc_box_regridded(const orc_box *b);
+
+/* mc_coag, src/coagulation.F90:43-96.  Returns 0 or a reference error code. */
+int orc_mc_coag(orc_box *b, int kernel_type, double del_t, uint32_t step,
+                int64_t *tot_n_samp, int64_t *tot_n_coag);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,810 @@
+// pmc_coag.hip -- C ABI and host orchestration of the gfx950 coagulation
+// step (PartMC mc_coag, src/coagulation.F90:43-96).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/partmc_coag.h"
+#include "pmc_device.h"
+#include "pmc_state.h"
+
+namespace pmc {
+// Hall (1980) collision efficiencies as tabulated in the reference's
+// effic (src/coag_kernel_sedi.F90:190-245), in its DATA (column-major) order.
+__constant__ double HALL_R0[15] = {6.0,  8.0,  10.0, 15.0,  20.0,  25.0,  30.0, 40.0,
+                                   50.0, 60.0, 70.0, 100.0, 150.0, 200.0, 300.0};
+__constant__ double HALL_RAT[21] = {0.0,  0.05, 0.1,  0.15, 0.2,  0.25, 0.3,
+                                    0.35, 0.4,  0.45, 0.5,  0.55, 0.6,  0.65,
+                                    0.7,  0.75, 0.8,  0.85, 0.9,  0.95, 1.0};
+__constant__ double HALL_ECOLL[315] = {
+    0.001, 0.001, 0.001, 0.001, 0.001, 0.001, 0.001, 0.001, 0.001, 0.001, 0.001, 0.001,
+    0.001, 0.001, 0.001, 0.003, 0.003, 0.003, 0.004, 0.005, 0.005, 0.005, 0.010, 0.100,
+    0.050, 0.200, 0.500, 0.770, 0.870, 0.970, 0.007, 0.007, 0.007, 0.008, 0.009, 0.010,
+    0.010, 0.070, 0.400, 0.430, 0.580, 0.790, 0.930, 0.960, 1.000, 0.009, 0.009, 0.009,
+    0.012, 0.015, 0.010, 0.020, 0.280, 0.600, 0.640, 0.750, 0.910, 0.970, 0.980, 1.000,
+    0.014, 0.014, 0.014, 0.015, 0.016, 0.030, 0.060, 0.500, 0.700, 0.770, 0.840, 0.950,
+    0.970, 1.000, 1.000, 0.017, 0.017, 0.017, 0.020, 0.022, 0.060, 0.100, 0.620, 0.780,
+    0.840, 0.880, 0.950, 1.000, 1.000, 1.000, 0.030, 0.030, 0.024, 0.022, 0.032, 0.062,
+    0.200, 0.680, 0.830, 0.870, 0.900, 0.950, 1.000, 1.000, 1.000, 0.025, 0.025, 0.025,
+    0.036, 0.043, 0.130, 0.270, 0.740, 0.860, 0.890, 0.920, 1.000, 1.000, 1.000, 1.000,
+    0.027, 0.027, 0.027, 0.040, 0.052, 0.200, 0.400, 0.780, 0.880, 0.900, 0.940, 1.000,
+    1.000, 1.000, 1.000, 0.030, 0.030, 0.030, 0.047, 0.064, 0.250, 0.500, 0.800, 0.900,
+    0.910, 0.950, 1.000, 1.000, 1.000, 1.000, 0.040, 0.040, 0.033, 0.037, 0.068, 0.240,
+    0.550, 0.800, 0.900, 0.910, 0.950, 1.000, 1.000, 1.000, 1.000, 0.035, 0.035, 0.035,
+    0.055, 0.079, 0.290, 0.580, 0.800, 0.900, 0.910, 0.950, 1.000, 1.000, 1.000, 1.000,
+    0.037, 0.037, 0.037, 0.062, 0.082, 0.290, 0.590, 0.780, 0.900, 0.910, 0.950, 1.000,
+    1.000, 1.000, 1.000, 0.037, 0.037, 0.037, 0.060, 0.080, 0.290, 0.580, 0.770, 0.890,
+    0.910, 0.950, 1.000, 1.000, 1.000, 1.000, 0.037, 0.037, 0.037, 0.041, 0.075, 0.250,
+    0.540, 0.760, 0.880, 0.920, 0.950, 1.000, 1.000, 1.000, 1.000, 0.037, 0.037, 0.037,
+    0.052, 0.067, 0.250, 0.510, 0.770, 0.880, 0.930, 0.970, 1.000, 1.000, 1.000, 1.000,
+    0.037, 0.037, 0.037, 0.047, 0.057, 0.250, 0.490, 0.770, 0.890, 0.950, 1.000, 1.000,
+    1.000, 1.000, 1.000, 0.036, 0.036, 0.036, 0.042, 0.048, 0.230, 0.470, 0.780, 0.920,
+    1.000, 1.020, 1.020, 1.020, 1.020, 1.020, 0.040, 0.040, 0.035, 0.033, 0.040, 0.112,
+    0.450, 0.790, 1.010, 1.030, 1.040, 1.040, 1.040, 1.040, 1.040, 0.033, 0.033, 0.033,
+    0.033, 0.033, 0.119, 0.470, 0.950, 1.300, 1.700, 2.300, 2.300, 2.300, 2.300, 2.300,
+    0.027, 0.027, 0.027, 0.027, 0.027, 0.125, 0.520, 1.400, 2.300, 3.000, 4.000, 4.000,
+    4.000, 4.000, 4.000};
+}  // namespace pmc
+
+#include "pmc_exact.h"
+#include "pmc_parallel.h"
+
+using namespace pmc;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_last_error;
+
+static int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIPCHECK(expr)                                                          \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess)                                                       \
+      return fail(ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));      \
+  } while (0)
+
+// ------------------------------------------------------------------ handle
+struct pmc_gpu {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n_box = 0, n_spec = 0, max_bin = 256, tab_classes = 1;
+  std::vector<double> density;
+  std::vector<int64_t> cap;     // per-box particle capacity
+  std::vector<int64_t> info_cap;
+  BoxDesc *d_box = nullptr;
+  Arrays A{};
+  int mode = PMC_MODE_EXACT;
+  int precision = PMC_PREC_FP64;
+  double band = 1e-5;
+  int validation = 0;
+  uint64_t seed = 0;
+  uint32_t step = 0;
+  ParallelWork work{};
+};
+
+static void free_particles(pmc_gpu *h) {
+  Arrays &A = h->A;
+  void *ptrs[] = {A.vol,   A.vtot,    A.id,      A.group,   A.klass,    A.nprim,
+                  A.ncomp, A.csrc,    A.ctime,   A.least,   A.greatest, A.fwd_bin,
+                  A.fwd_class, A.fwd_idx, A.pool, A.info_id, A.info_other, A.info_action};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  A.vol = A.vtot = A.ctime = A.least = A.greatest = nullptr;
+  A.id = nullptr;
+  A.group = A.klass = A.nprim = A.ncomp = A.csrc = nullptr;
+  A.fwd_bin = A.fwd_class = A.fwd_idx = A.pool = nullptr;
+  A.info_id = A.info_other = nullptr;
+  A.info_action = nullptr;
+  A.total_cap = 0;
+}
+
+template <class T>
+static hipError_t dmalloc(T **p, size_t n) {
+  return hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+static int alloc_tables(pmc_gpu *h, int classes) {
+  Arrays &A = h->A;
+  if (A.kw) (void)hipFree(A.kw);
+  if (A.cc) (void)hipFree(A.cc);
+  size_t n = (size_t)h->n_box * MAX_CLASS * MAX_CLASS * h->max_bin * h->max_bin;
+  (void)classes;
+  HIPCHECK(dmalloc(&A.kw, n));
+  HIPCHECK(dmalloc(&A.cc, n));
+  h->tab_classes = classes;
+  return 0;
+}
+
+// Lay out particle storage for the given per-box capacities, keeping the
+// particles already uploaded.
+static int layout(pmc_gpu *h, const std::vector<int64_t> &new_cap) {
+  std::vector<BoxDesc> hb(h->n_box);
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
+  Arrays old = h->A;
+  int64_t total = 0, pool_total = 0, info_total = 0;
+  std::vector<int64_t> off(h->n_box), poff(h->n_box), ioff(h->n_box), phalf(h->n_box);
+  for (int b = 0; b < h->n_box; ++b) {
+    off[b] = total;
+    total += new_cap[b];
+    phalf[b] = 2 * new_cap[b] + 16 * (int64_t)h->A.max_keys;
+    poff[b] = pool_total;
+    pool_total += 2 * phalf[b];
+    ioff[b] = info_total;
+    info_total += new_cap[b] + 64;
+  }
+  Arrays A = h->A;
+  A.total_cap = total;
+  HIPCHECK(dmalloc(&A.vol, (size_t)total * h->n_spec));
+  HIPCHECK(dmalloc(&A.vtot, total));
+  HIPCHECK(dmalloc(&A.id, total));
+  HIPCHECK(dmalloc(&A.group, total));
+  HIPCHECK(dmalloc(&A.klass, total));
+  HIPCHECK(dmalloc(&A.nprim, total));
+  HIPCHECK(dmalloc(&A.ncomp, total));
+  HIPCHECK(dmalloc(&A.csrc, (size_t)total * MAX_COMP));
+  HIPCHECK(dmalloc(&A.ctime, (size_t)total * MAX_COMP));
+  HIPCHECK(dmalloc(&A.least, total));
+  HIPCHECK(dmalloc(&A.greatest, total));
+  HIPCHECK(dmalloc(&A.fwd_bin, total));
+  HIPCHECK(dmalloc(&A.fwd_class, total));
+  HIPCHECK(dmalloc(&A.fwd_idx, total));
+  HIPCHECK(dmalloc(&A.pool, pool_total));
+  HIPCHECK(dmalloc(&A.info_id, info_total));
+  HIPCHECK(dmalloc(&A.info_other, info_total));
+  HIPCHECK(dmalloc(&A.info_action, info_total));
+  // move existing particles
+  for (int b = 0; b < h->n_box; ++b) {
+    int64_t n = hb[b].n_part;
+    if (old.total_cap > 0 && n > 0) {
+      int64_t o = hb[b].off, d = off[b];
+      for (int s = 0; s < h->n_spec; ++s)
+        HIPCHECK(hipMemcpy(A.vol + s * total + d, old.vol + s * old.total_cap + o,
+                           n * sizeof(double), hipMemcpyDeviceToDevice));
+      for (int c = 0; c < MAX_COMP; ++c) {
+        HIPCHECK(hipMemcpy(A.csrc + c * total + d, old.csrc + c * old.total_cap + o,
+                           n * sizeof(int), hipMemcpyDeviceToDevice));
+        HIPCHECK(hipMemcpy(A.ctime + c * total + d, old.ctime + c * old.total_cap + o,
+                           n * sizeof(double), hipMemcpyDeviceToDevice));
+      }
+#define MOVE(F, T)                                                               \
+  HIPCHECK(hipMemcpy(A.F + d, old.F + o, n * sizeof(T), hipMemcpyDeviceToDevice))
+      MOVE(vtot, double);
+      MOVE(id, int64_t);
+      MOVE(group, int);
+      MOVE(klass, int);
+      MOVE(nprim, int);
+      MOVE(ncomp, int);
+      MOVE(least, double);
+      MOVE(greatest, double);
+      MOVE(fwd_bin, int);
+      MOVE(fwd_class, int);
+      MOVE(fwd_idx, int);
+#undef MOVE
+    }
+    hb[b].off = off[b];
+    hb[b].cap = new_cap[b];
+    hb[b].pool_off = poff[b];
+    hb[b].pool_half = phalf[b];
+    hb[b].info_off = ioff[b];
+    hb[b].info_cap = new_cap[b] + 64;
+    hb[b].valid_sort = 0;  // lists are rebuilt after a re-layout
+  }
+  h->A = old;
+  free_particles(h);
+  h->A = A;
+  h->cap = new_cap;
+  HIPCHECK(hipMemcpy(h->d_box, hb.data(), sizeof(BoxDesc) * h->n_box, hipMemcpyHostToDevice));
+  return parallel_layout(h->work, h->A, h->n_box, new_cap, h->max_bin);
+}
+
+template <class T>
+static int set_field(pmc_gpu *h, int box, size_t offset, const T &v) {
+  HIPCHECK(hipMemcpy((char *)(h->d_box + box) + offset, &v, sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+#define SET_FIELD(h, box, field, v) set_field(h, box, offsetof(BoxDesc, field), v)
+
+static int check_box(pmc_gpu *h, int box) {
+  if (!h) return fail(ERR_ARG, "null handle");
+  if (box < 0 || box >= h->n_box) return fail(ERR_ARG, "box %d out of range [0, %d)", box, h->n_box);
+  return 0;
+}
+
+extern "C" {
+
+const char *pmc_gpu_last_error(void) { return g_last_error.c_str(); }
+
+int pmc_gpu_create(pmc_gpu **out, int device, int n_box, int n_spec, const double *density,
+                   int max_bin) {
+  if (!out || n_box < 1 || n_spec < 1 || n_spec > MAX_SPEC || !density)
+    return fail(ERR_ARG, "pmc_gpu_create: bad arguments (n_box %d, n_spec %d)", n_box, n_spec);
+  if (max_bin <= 0) max_bin = 256;
+  pmc_gpu *h = new pmc_gpu();
+  h->device = device;
+  h->n_box = n_box;
+  h->n_spec = n_spec;
+  h->max_bin = max_bin;
+  h->density.assign(density, density + n_spec);
+  HIPCHECK(hipSetDevice(device));
+  HIPCHECK(hipStreamCreate(&h->stream));
+  Arrays &A = h->A;
+  A.n_spec = n_spec;
+  A.max_bin = max_bin;
+  A.max_keys = max_bin * MAX_CLASS;
+  A.d_min = *std::min_element(density, density + n_spec);
+  A.d_max = *std::max_element(density, density + n_spec);
+  HIPCHECK(dmalloc(&A.density, n_spec));
+  HIPCHECK(hipMemcpy(A.density, density, n_spec * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHECK(dmalloc(&A.edges, (size_t)n_box * (max_bin + 1)));
+  HIPCHECK(dmalloc(&A.centers, (size_t)n_box * max_bin));
+  HIPCHECK(dmalloc(&A.widths, (size_t)n_box * max_bin));
+  HIPCHECK(dmalloc(&A.kmin, (size_t)n_box * max_bin * max_bin));
+  HIPCHECK(dmalloc(&A.kmax, (size_t)n_box * max_bin * max_bin));
+  HIPCHECK(dmalloc(&A.lstart, (size_t)n_box * A.max_keys));
+  HIPCHECK(dmalloc(&A.llen, (size_t)n_box * A.max_keys));
+  HIPCHECK(dmalloc(&A.lcap, (size_t)n_box * A.max_keys));
+  HIPCHECK(hipMemset(A.llen, 0, sizeof(int) * (size_t)n_box * A.max_keys));
+  HIPCHECK(dmalloc(&A.sort_scratch, (size_t)n_box * SORT_THREADS * A.max_keys));
+  int err = alloc_tables(h, 1);
+  if (err) return err;
+  std::vector<BoxDesc> hb(n_box);
+  for (int b = 0; b < n_box; ++b) {
+    BoxDesc &B = hb[b];
+    memset(&B, 0, sizeof(B));
+    B.next_id = 1;
+    B.id_stride = 1;
+    B.grid_req_n_bin = -1;
+    B.w.n_group = 1;
+    B.w.n_class = 1;
+    B.w.type[0] = WEIGHT_NONE;
+    B.w.mag[0] = 1.0;
+    B.w.expo[0] = 0.0;
+    B.env.temp = 288.0;
+    B.env.pressure = 1e5;
+    B.env.beta = 1000.0;
+  }
+  HIPCHECK(dmalloc(&h->d_box, n_box));
+  HIPCHECK(hipMemcpy(h->d_box, hb.data(), sizeof(BoxDesc) * n_box, hipMemcpyHostToDevice));
+  h->cap.assign(n_box, 0);
+  err = parallel_init(h->work, n_box, max_bin);
+  if (err) return fail(err, "parallel work allocation failed");
+  *out = h;
+  return 0;
+}
+
+int pmc_gpu_destroy(pmc_gpu *h) {
+  if (!h) return 0;
+  (void)hipSetDevice(h->device);
+  (void)hipStreamSynchronize(h->stream);
+  free_particles(h);
+  Arrays &A = h->A;
+  void *ptrs[] = {A.density, A.edges, A.centers, A.widths, A.kmin, A.kmax, A.kw, A.cc,
+                  A.lstart,  A.llen,  A.lcap,    A.sort_scratch, h->d_box};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  parallel_free(h->work);
+  (void)hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+int pmc_gpu_set_env(pmc_gpu *h, int box, double temp, double pressure, double additive_coeff) {
+  int e = check_box(h, box);
+  if (e) return e;
+  Env env{temp, pressure, additive_coeff};
+  return SET_FIELD(h, box, env, env);
+}
+
+int pmc_gpu_set_weights(pmc_gpu *h, int box, int n_group, int n_class, const int *type,
+                        const double *magnitude, const double *exponent) {
+  int e = check_box(h, box);
+  if (e) return e;
+  if (n_group < 1 || n_group > MAX_GROUP || n_class < 1 || n_class > MAX_CLASS || !type ||
+      !magnitude || !exponent)
+    return fail(ERR_ARG, "set_weights: n_group %d / n_class %d outside 1..%d/1..%d", n_group,
+                n_class, MAX_GROUP, MAX_CLASS);
+  Weights w{};
+  w.n_group = n_group;
+  w.n_class = n_class;
+  for (int i = 0; i < n_group * n_class; ++i) {
+    if (type[i] < WEIGHT_NONE || type[i] > WEIGHT_MFA)
+      return fail(700421478, "unknown aero_weight type: %d", type[i]);
+    w.type[i] = type[i];
+    w.mag[i] = magnitude[i];
+    w.expo[i] = exponent[i];
+  }
+  HIPCHECK(hipSetDevice(h->device));
+  e = SET_FIELD(h, box, w, w);
+  if (e) return e;
+  int zero = 0;
+  e = SET_FIELD(h, box, tables_valid, zero);
+  if (e) return e;
+  // class count changes the key space of the lists
+  return SET_FIELD(h, box, valid_sort, zero);
+}
+
+int pmc_gpu_set_rng(pmc_gpu *h, uint64_t seed, uint32_t step) {
+  if (!h) return fail(ERR_ARG, "null handle");
+  h->seed = seed;
+  h->step = step;
+  return 0;
+}
+
+int pmc_gpu_set_next_id(pmc_gpu *h, int box, int64_t next_id, int64_t id_stride,
+                        int64_t id_offset) {
+  int e = check_box(h, box);
+  if (e) return e;
+  if (id_stride < 1) return fail(ERR_ARG, "id_stride must be >= 1");
+  HIPCHECK(hipSetDevice(h->device));
+  if ((e = SET_FIELD(h, box, next_id, next_id))) return e;
+  if ((e = SET_FIELD(h, box, id_stride, id_stride))) return e;
+  return SET_FIELD(h, box, id_offset, id_offset);
+}
+
+int pmc_gpu_get_next_id(pmc_gpu *h, int box, int64_t *next_id) {
+  int e = check_box(h, box);
+  if (e) return e;
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipMemcpy(next_id, (char *)(h->d_box + box) + offsetof(BoxDesc, next_id),
+                     sizeof(int64_t), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+__global__ void k_default_components(Arrays A, int64_t base, int n, int set_comp, int set_ct) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t p = base + i;
+  if (set_comp) {
+    A.ncomp[p] = 1;
+    for (int c = 0; c < MAX_COMP; ++c) {
+      A.csrc[(int64_t)c * A.total_cap + p] = 0;
+      A.ctime[(int64_t)c * A.total_cap + p] = 0.0;
+    }
+  }
+  if (set_ct) {
+    A.least[p] = 0.0;
+    A.greatest[p] = 0.0;
+  }
+}
+
+int pmc_gpu_upload(pmc_gpu *h, int box, int n_part, int capacity, const double *vol,
+                   const int64_t *id, const int *group, const int *klass, const int *n_primary,
+                   const int *n_comp, const int *comp_source, const double *comp_time,
+                   const double *least, const double *greatest) {
+  int e = check_box(h, box);
+  if (e) return e;
+  if (n_part < 0 || (n_part > 0 && (!vol || !id || !group || !klass)))
+    return fail(ERR_ARG, "upload: bad arguments");
+  if (n_comp && (!comp_source || !comp_time))
+    return fail(ERR_ARG, "upload: n_comp given without comp_source/comp_time");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  int64_t want = std::max<int64_t>(capacity, n_part);
+  if (want < 16) want = 16;
+  if (want > h->cap[box]) {
+    std::vector<int64_t> nc = h->cap;
+    nc[box] = want;
+    if ((e = layout(h, nc))) return e;
+  }
+  BoxDesc hb;
+  HIPCHECK(hipMemcpy(&hb, h->d_box + box, sizeof(BoxDesc), hipMemcpyDeviceToHost));
+  Arrays &A = h->A;
+  const int64_t base = hb.off, tc = A.total_cap;
+  const int ns = h->n_spec;
+  if (n_part > 0) {
+    std::vector<double> row(n_part);
+    for (int s = 0; s < ns; ++s) {
+      for (int i = 0; i < n_part; ++i) row[i] = vol[(size_t)i * ns + s];
+      HIPCHECK(hipMemcpy(A.vol + s * tc + base, row.data(), n_part * sizeof(double),
+                         hipMemcpyHostToDevice));
+    }
+    for (int i = 0; i < n_part; ++i) {
+      double v = 0.0;
+      for (int s = 0; s < ns; ++s) v += vol[(size_t)i * ns + s];
+      row[i] = v;
+    }
+    HIPCHECK(hipMemcpy(A.vtot + base, row.data(), n_part * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(A.id + base, id, n_part * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(A.group + base, group, n_part * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(A.klass + base, klass, n_part * sizeof(int), hipMemcpyHostToDevice));
+    if (n_primary) {
+      HIPCHECK(hipMemcpy(A.nprim + base, n_primary, n_part * sizeof(int), hipMemcpyHostToDevice));
+    } else {
+      std::vector<int> ones(n_part, 1);
+      HIPCHECK(hipMemcpy(A.nprim + base, ones.data(), n_part * sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (n_comp) {
+      HIPCHECK(hipMemcpy(A.ncomp + base, n_comp, n_part * sizeof(int), hipMemcpyHostToDevice));
+      std::vector<int> ci(n_part);
+      std::vector<double> cd(n_part);
+      for (int c = 0; c < MAX_COMP; ++c) {
+        for (int i = 0; i < n_part; ++i) {
+          ci[i] = comp_source[(size_t)i * MAX_COMP + c];
+          cd[i] = comp_time[(size_t)i * MAX_COMP + c];
+        }
+        HIPCHECK(hipMemcpy(A.csrc + c * tc + base, ci.data(), n_part * sizeof(int),
+                           hipMemcpyHostToDevice));
+        HIPCHECK(hipMemcpy(A.ctime + c * tc + base, cd.data(), n_part * sizeof(double),
+                           hipMemcpyHostToDevice));
+      }
+    }
+    if (least && greatest) {
+      HIPCHECK(hipMemcpy(A.least + base, least, n_part * sizeof(double), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(A.greatest + base, greatest, n_part * sizeof(double),
+                         hipMemcpyHostToDevice));
+    }
+    if (!n_comp || !(least && greatest)) {
+      hipLaunchKernelGGL(k_default_components, dim3((n_part + 255) / 256), dim3(256), 0, h->stream,
+                         A, base, n_part, n_comp ? 0 : 1, (least && greatest) ? 0 : 1);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipStreamSynchronize(h->stream));
+    }
+  }
+  hb.n_part = n_part;
+  hb.valid_sort = 0;
+  hb.n_info = 0;
+  hb.error = 0;
+  HIPCHECK(hipMemcpy(h->d_box + box, &hb, sizeof(BoxDesc), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int check_box_errors(pmc_gpu *h, std::vector<BoxDesc> &hb) {
+  for (int b = 0; b < h->n_box; ++b)
+    if (hb[b].error) {
+      int code = hb[b].error;
+      const char *what = "";
+      switch (code) {
+        case 765620746: case 134572570: what = "particle outside of bin_grid: try reducing the timestep del_t"; break;
+        case 99: what = "sedimentation collision efficiency below 1e-20 (pmc_stop(99))"; break;
+        case ERR_CAPACITY: what = "particle, list or grid capacity exceeded"; break;
+        case 653606684: what = "prob_remove_source_max > 1"; break;
+        case 538534122: what = "bin_grid requires a non-negative n_bin"; break;
+        case 708651144: what = "particle outside the sort grid"; break;
+        default: what = "device error";
+      }
+      return fail(code, "box %d: %s (code %d)", b, what, code);
+    }
+  return 0;
+}
+
+int pmc_gpu_set_bin_grid(pmc_gpu *h, int box, int n_bin, double r_min, double r_max) {
+  int e = check_box(h, box);
+  if (e) return e;
+  if (n_bin < 0 || n_bin > h->max_bin)
+    return fail(ERR_ARG, "set_bin_grid: n_bin %d outside 0..%d", n_bin, h->max_bin);
+  if (n_bin > 0 && !(r_min > 0.0 && r_min < r_max))
+    return fail(711537859, "bin_grid requires 0 < min < max");
+  if (h->cap[box] == 0) return fail(ERR_STATE, "set_bin_grid: upload particles first");
+  HIPCHECK(hipSetDevice(h->device));
+  if ((e = SET_FIELD(h, box, grid_req_n_bin, n_bin))) return e;
+  if ((e = SET_FIELD(h, box, grid_req_min, r_min))) return e;
+  if ((e = SET_FIELD(h, box, grid_req_max, r_max))) return e;
+  hipLaunchKernelGGL(k_prepare, dim3(1), dim3(SORT_THREADS), 0, h->stream, h->d_box, h->A,
+                     box);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  std::vector<BoxDesc> hb(h->n_box);
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
+  return check_box_errors(h, hb);
+}
+
+int pmc_gpu_get_n_part(pmc_gpu *h, int box, int *n_part) {
+  int e = check_box(h, box);
+  if (e) return e;
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  HIPCHECK(hipMemcpy(n_part, (char *)(h->d_box + box) + offsetof(BoxDesc, n_part), sizeof(int),
+                     hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pmc_gpu_download(pmc_gpu *h, int box, int max_part, int *n_part, double *vol, int64_t *id,
+                     int *group, int *klass, int *n_primary, int *n_comp, int *comp_source,
+                     double *comp_time, double *least, double *greatest) {
+  int e = check_box(h, box);
+  if (e) return e;
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  BoxDesc hb;
+  HIPCHECK(hipMemcpy(&hb, h->d_box + box, sizeof(BoxDesc), hipMemcpyDeviceToHost));
+  if (n_part) *n_part = hb.n_part;
+  int n = hb.n_part;
+  if (n > max_part) return fail(ERR_CAPACITY, "download: box holds %d particles, room for %d", n, max_part);
+  if (n == 0) return 0;
+  const Arrays &A = h->A;
+  const int64_t base = hb.off, tc = A.total_cap;
+  const int ns = h->n_spec;
+  if (vol) {
+    std::vector<double> row(n);
+    for (int s = 0; s < ns; ++s) {
+      HIPCHECK(hipMemcpy(row.data(), A.vol + s * tc + base, n * sizeof(double), hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i) vol[(size_t)i * ns + s] = row[i];
+    }
+  }
+  if (id) HIPCHECK(hipMemcpy(id, A.id + base, n * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (group) HIPCHECK(hipMemcpy(group, A.group + base, n * sizeof(int), hipMemcpyDeviceToHost));
+  if (klass) HIPCHECK(hipMemcpy(klass, A.klass + base, n * sizeof(int), hipMemcpyDeviceToHost));
+  if (n_primary) HIPCHECK(hipMemcpy(n_primary, A.nprim + base, n * sizeof(int), hipMemcpyDeviceToHost));
+  if (n_comp) HIPCHECK(hipMemcpy(n_comp, A.ncomp + base, n * sizeof(int), hipMemcpyDeviceToHost));
+  if (comp_source || comp_time) {
+    std::vector<int> ci(n);
+    std::vector<double> cd(n);
+    for (int c = 0; c < MAX_COMP; ++c) {
+      if (comp_source) {
+        HIPCHECK(hipMemcpy(ci.data(), A.csrc + c * tc + base, n * sizeof(int), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) comp_source[(size_t)i * MAX_COMP + c] = ci[i];
+      }
+      if (comp_time) {
+        HIPCHECK(hipMemcpy(cd.data(), A.ctime + c * tc + base, n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) comp_time[(size_t)i * MAX_COMP + c] = cd[i];
+      }
+    }
+  }
+  if (least) HIPCHECK(hipMemcpy(least, A.least + base, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (greatest) HIPCHECK(hipMemcpy(greatest, A.greatest + base, n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pmc_gpu_set_mode(pmc_gpu *h, int mode, int precision, double band) {
+  if (!h) return fail(ERR_ARG, "null handle");
+  if (mode != PMC_MODE_EXACT && mode != PMC_MODE_PARALLEL) return fail(ERR_ARG, "unknown mode %d", mode);
+  if (precision != PMC_PREC_FP64 && precision != PMC_PREC_FP32_RECHECK)
+    return fail(ERR_ARG, "unknown precision %d", precision);
+  if (mode == PMC_MODE_EXACT && precision != PMC_PREC_FP64)
+    return fail(ERR_UNSUPPORTED, "exact mode evaluates the kernel in FP64 only");
+  if (!(band >= 0.0)) return fail(ERR_ARG, "recheck band must be >= 0");
+  h->mode = mode;
+  h->precision = precision;
+  h->band = band;
+  return 0;
+}
+
+int pmc_gpu_set_validation(pmc_gpu *h, int on) {
+  if (!h) return fail(ERR_ARG, "null handle");
+  h->validation = on ? 1 : 0;
+  return 0;
+}
+
+static int check_kernel_type(int t) {
+  if (t == 6 || t == 7) return fail(ERR_UNSUPPORTED, "fractal kernel type %d is not supported", t);
+  if (t < 1 || t > 5) return fail(200724934, "Unknown kernel type: %d", t);
+  return 0;
+}
+
+static int enqueue_tables(pmc_gpu *h, int kernel_type) {
+  const int mb = h->max_bin;
+  dim3 gb((mb * mb + 255) / 256, h->n_box);
+  hipLaunchKernelGGL(k_kbounds, gb, dim3(256), 0, h->stream, h->d_box, h->A, kernel_type);
+  hipLaunchKernelGGL(k_mark_bounds, dim3((h->n_box + 63) / 64), dim3(64), 0, h->stream, h->d_box,
+                     h->n_box, kernel_type);
+  dim3 gt((MAX_CLASS * MAX_CLASS * mb * mb + 255) / 256, h->n_box);
+  hipLaunchKernelGGL(k_pair_tables, gt, dim3(256), 0, h->stream, h->d_box, h->A);
+  hipLaunchKernelGGL(k_mark_tables, dim3((h->n_box + 63) / 64), dim3(64), 0, h->stream, h->d_box,
+                     h->n_box);
+  HIPCHECK(hipGetLastError());
+  return 0;
+}
+
+int pmc_gpu_mc_coag_async(pmc_gpu *h, int kernel_type, double del_t) {
+  if (!h) return fail(ERR_ARG, "null handle");
+  int e = check_kernel_type(kernel_type);
+  if (e) return e;
+  if (!(del_t >= 0.0)) return fail(ERR_ARG, "del_t must be >= 0");
+  for (int b = 0; b < h->n_box; ++b)
+    if (h->cap[b] == 0) return fail(ERR_STATE, "box %d has no particle storage: upload first", b);
+  HIPCHECK(hipSetDevice(h->device));
+  if (h->mode == PMC_MODE_EXACT) {
+    hipLaunchKernelGGL(k_prepare, dim3(h->n_box), dim3(SORT_THREADS), 0, h->stream, h->d_box, h->A,
+                       -1);
+    if ((e = enqueue_tables(h, kernel_type))) return e;
+    hipLaunchKernelGGL(k_exact_sweep, dim3(h->n_box), dim3(64), 0, h->stream, h->d_box, h->A,
+                       kernel_type, del_t, h->seed, h->step);
+    HIPCHECK(hipGetLastError());
+  } else {
+    e = parallel_step(h->work, h->A, h->d_box, h->n_box, h->cap, h->max_bin, h->stream,
+                      kernel_type, del_t, h->seed, h->step, h->precision, h->band, h->validation);
+    if (e) return fail(e, "parallel step enqueue failed: %s", parallel_last_error());
+  }
+  h->step++;
+  return 0;
+}
+
+int pmc_gpu_sync(pmc_gpu *h) {
+  if (!h) return fail(ERR_ARG, "null handle");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  std::vector<BoxDesc> hb(h->n_box);
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
+  return check_box_errors(h, hb);
+}
+
+int pmc_gpu_mc_coag(pmc_gpu *h, int kernel_type, double del_t, int64_t *tot_n_samp,
+                    int64_t *tot_n_coag) {
+  int e = pmc_gpu_mc_coag_async(h, kernel_type, del_t);
+  if (e) return e;
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  std::vector<BoxDesc> hb(h->n_box);
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
+  for (int b = 0; b < h->n_box; ++b) {
+    if (tot_n_samp) tot_n_samp[b] = hb[b].step_samp;
+    if (tot_n_coag) tot_n_coag[b] = hb[b].step_coag;
+  }
+  return check_box_errors(h, hb);
+}
+
+int pmc_gpu_get_counters(pmc_gpu *h, int64_t *n_samp, int64_t *n_coag, int64_t *n_warn,
+                         int64_t *n_recheck, int64_t *n_flip, double *max_rel_err) {
+  if (!h) return fail(ERR_ARG, "null handle");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  std::vector<BoxDesc> hb(h->n_box);
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
+  for (int b = 0; b < h->n_box; ++b) {
+    if (n_samp) n_samp[b] = hb[b].n_samp;
+    if (n_coag) n_coag[b] = hb[b].n_coag;
+    if (n_warn) n_warn[b] = hb[b].n_warn;
+    if (n_recheck) n_recheck[b] = hb[b].n_recheck;
+    if (n_flip) n_flip[b] = hb[b].n_flip;
+    if (max_rel_err) max_rel_err[b] = hb[b].max_rel_err;
+  }
+  return 0;
+}
+
+int pmc_gpu_reset_counters(pmc_gpu *h) {
+  if (!h) return fail(ERR_ARG, "null handle");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  std::vector<BoxDesc> hb(h->n_box);
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
+  for (auto &B : hb) {
+    B.n_samp = B.n_coag = B.n_warn = B.n_recheck = B.n_flip = 0;
+    B.max_rel_err = 0.0;
+    B.n_rounds = B.n_dropped = 0;
+  }
+  HIPCHECK(hipMemcpy(h->d_box, hb.data(), sizeof(BoxDesc) * h->n_box, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int pmc_gpu_get_grid(pmc_gpu *h, int box, int *n_bin, double *edges, double *centers,
+                     double *widths) {
+  int e = check_box(h, box);
+  if (e) return e;
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  BoxDesc hb;
+  HIPCHECK(hipMemcpy(&hb, h->d_box + box, sizeof(BoxDesc), hipMemcpyDeviceToHost));
+  if (n_bin) *n_bin = hb.n_bin;
+  int nb = hb.n_bin;
+  if (nb <= 0) return 0;
+  if (edges) HIPCHECK(hipMemcpy(edges, h->A.edges + (size_t)box * (h->max_bin + 1), (nb + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  if (centers) HIPCHECK(hipMemcpy(centers, h->A.centers + (size_t)box * h->max_bin, nb * sizeof(double), hipMemcpyDeviceToHost));
+  if (widths) HIPCHECK(hipMemcpy(widths, h->A.widths + (size_t)box * h->max_bin, nb * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pmc_gpu_get_lists(pmc_gpu *h, int box, int *counts, int *entries) {
+  int e = check_box(h, box);
+  if (e) return e;
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  BoxDesc hb;
+  HIPCHECK(hipMemcpy(&hb, h->d_box + box, sizeof(BoxDesc), hipMemcpyDeviceToHost));
+  int K = hb.n_bin * hb.w.n_class;
+  if (K <= 0) return 0;
+  std::vector<int> st(K), ln(K);
+  HIPCHECK(hipMemcpy(st.data(), h->A.lstart + (size_t)box * h->A.max_keys, K * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(ln.data(), h->A.llen + (size_t)box * h->A.max_keys, K * sizeof(int), hipMemcpyDeviceToHost));
+  int off = 0;
+  for (int k = 0; k < K; ++k) {
+    if (counts) counts[k] = ln[k];
+    if (entries && ln[k] > 0) {
+      HIPCHECK(hipMemcpy(entries + off, h->A.pool + st[k], ln[k] * sizeof(int), hipMemcpyDeviceToHost));
+      for (int i = 0; i < ln[k]; ++i) entries[off + i] -= 1;
+    }
+    off += ln[k];
+  }
+  return 0;
+}
+
+int pmc_gpu_get_kbounds(pmc_gpu *h, int box, double *kmin, double *kmax) {
+  int e = check_box(h, box);
+  if (e) return e;
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  BoxDesc hb;
+  HIPCHECK(hipMemcpy(&hb, h->d_box + box, sizeof(BoxDesc), hipMemcpyDeviceToHost));
+  size_t nn = (size_t)hb.n_bin * hb.n_bin;
+  size_t o = (size_t)box * h->max_bin * h->max_bin;
+  if (kmin) HIPCHECK(hipMemcpy(kmin, h->A.kmin + o, nn * sizeof(double), hipMemcpyDeviceToHost));
+  if (kmax) HIPCHECK(hipMemcpy(kmax, h->A.kmax + o, nn * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pmc_gpu_get_removed(pmc_gpu *h, int box, int max_n, int *n, int64_t *id, int *action,
+                        int64_t *other_id) {
+  int e = check_box(h, box);
+  if (e) return e;
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  BoxDesc hb;
+  HIPCHECK(hipMemcpy(&hb, h->d_box + box, sizeof(BoxDesc), hipMemcpyDeviceToHost));
+  int64_t cnt = std::min<int64_t>(hb.n_info, hb.info_cap);
+  if (n) *n = (int)hb.n_info;
+  if (hb.n_info > hb.info_cap) return fail(ERR_CAPACITY, "removal log overflow (%lld records)", (long long)hb.n_info);
+  if (cnt > max_n) return fail(ERR_CAPACITY, "removal log holds %lld records, room for %d", (long long)cnt, max_n);
+  if (cnt == 0) return 0;
+  if (id) HIPCHECK(hipMemcpy(id, h->A.info_id + hb.info_off, cnt * sizeof(int64_t), hipMemcpyDeviceToHost));
+  if (action) HIPCHECK(hipMemcpy(action, h->A.info_action + hb.info_off, cnt * sizeof(int), hipMemcpyDeviceToHost));
+  if (other_id) HIPCHECK(hipMemcpy(other_id, h->A.info_other + hb.info_off, cnt * sizeof(int64_t), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pmc_gpu_eval_kernel(pmc_gpu *h, int kernel_type, int n, const double *vol1, const double *vol2,
+                        double temp, double pressure, double beta, double *k64, float *k32) {
+  if (!h || n < 0 || !vol1 || !vol2 || !k64) return fail(ERR_ARG, "eval_kernel: bad arguments");
+  int e = check_kernel_type(kernel_type);
+  if (e) return e;
+  if (n == 0) return 0;
+  HIPCHECK(hipSetDevice(h->device));
+  const int ns = h->n_spec;
+  double *d1, *d2, *dk;
+  float *dk32;
+  int *derr;
+  HIPCHECK(dmalloc(&d1, (size_t)n * ns));
+  HIPCHECK(dmalloc(&d2, (size_t)n * ns));
+  HIPCHECK(dmalloc(&dk, n));
+  HIPCHECK(dmalloc(&dk32, n));
+  HIPCHECK(dmalloc(&derr, n));
+  HIPCHECK(hipMemcpy(d1, vol1, (size_t)n * ns * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(d2, vol2, (size_t)n * ns * sizeof(double), hipMemcpyHostToDevice));
+  Env env{temp, pressure, beta};
+  hipLaunchKernelGGL(k_eval_kernel, dim3((n + 127) / 128), dim3(128), 0, h->stream, h->A, kernel_type,
+                     n, d1, d2, env, dk, dk32, derr);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  std::vector<int> herr(n);
+  HIPCHECK(hipMemcpy(k64, dk, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (k32) HIPCHECK(hipMemcpy(k32, dk32, n * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(herr.data(), derr, n * sizeof(int), hipMemcpyDeviceToHost));
+  (void)hipFree(d1);
+  (void)hipFree(d2);
+  (void)hipFree(dk);
+  (void)hipFree(dk32);
+  (void)hipFree(derr);
+  for (int i = 0; i < n; ++i)
+    if (herr[i]) return fail(herr[i], "kernel evaluation error %d at pair %d", herr[i], i);
+  return 0;
+}
+
+int pmc_gpu_rng_draw(pmc_gpu *h, uint64_t seed, uint32_t box, uint32_t step, uint32_t site,
+                     uint32_t ordinal, uint32_t kind, int which, double p1, double p2, int n,
+                     double *out) {
+  if (!h || n < 0 || !out || which < 0 || which > 3) return fail(ERR_ARG, "rng_draw: bad arguments");
+  if (n == 0) return 0;
+  HIPCHECK(hipSetDevice(h->device));
+  double *d;
+  HIPCHECK(dmalloc(&d, n));
+  hipLaunchKernelGGL(k_rng_draw, dim3((n + 255) / 256), dim3(256), 0, h->stream, seed, box, step,
+                     site, ordinal, kind, which, p1, p2, n, d);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  HIPCHECK(hipMemcpy(out, d, n * sizeof(double), hipMemcpyDeviceToHost));
+  (void)hipFree(d);
+  return 0;
+}
+
+}  // extern "C"
