@@ -89,6 +89,10 @@ int pmc_gpu_set_next_id(pmc_gpu *h, int box, int64_t next_id,
                         int64_t id_stride, int64_t id_offset);
 int pmc_gpu_get_next_id(pmc_gpu *h, int box, int64_t *next_id);
 
+/* Reserve particle capacity for every box at once (capacity[n_box]),
+ * avoiding a re-layout per upload when many boxes are loaded. */
+int pmc_gpu_reserve(pmc_gpu *h, const int64_t *capacity);
+
 /* Replace one box's particles (aero_state%apa, src/aero_particle_array.F90).
  * capacity >= n_part reserves room for particles created during steps.
  * n_comp/comp_source/comp_time may be NULL (one component of source 0 at
@@ -138,6 +142,15 @@ int pmc_gpu_get_counters(pmc_gpu *h, int64_t *n_samp, int64_t *n_coag,
                          int64_t *n_warn, int64_t *n_recheck,
                          int64_t *n_flip, double *max_rel_err);
 int pmc_gpu_reset_counters(pmc_gpu *h);
+/* Particle-steps per box since the last reset: the sum over steps of the
+ * box's particle count at the start of each step. */
+int pmc_gpu_get_part_steps(pmc_gpu *h, int64_t *part_steps);
+/* Per-kernel timing with HIP events on the handle's stream: when on, every
+ * launch of the named kernels is bracketed by events; totals accumulate
+ * until the next pmc_gpu_set_timing call. */
+int pmc_gpu_set_timing(pmc_gpu *h, int on);
+int pmc_gpu_get_kernel_time(pmc_gpu *h, const char *kernel, double *total_ms,
+                            int64_t *launches);
 /* validation on: every FP32 test is also evaluated in FP64 to count decision
  * flips and the max relative kernel error */
 int pmc_gpu_set_validation(pmc_gpu *h, int on);

@@ -38,6 +38,10 @@ ABI = {
     "pmc_gpu_set_rng": (_i, [_p, _u64, _u32]),
     "pmc_gpu_set_next_id": (_i, [_p, _i, _i64, _i64, _i64]),
     "pmc_gpu_get_next_id": (_i, [_p, _i, _p]),
+    "pmc_gpu_reserve": (_i, [_p, _p]),
+    "pmc_gpu_get_part_steps": (_i, [_p, _p]),
+    "pmc_gpu_set_timing": (_i, [_p, _i]),
+    "pmc_gpu_get_kernel_time": (_i, [_p, ctypes.c_char_p, _p, _p]),
     "pmc_gpu_upload": (_i, [_p, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "pmc_gpu_set_bin_grid": (_i, [_p, _i, _i, _d, _d]),
     "pmc_gpu_get_n_part": (_i, [_p, _i, _p]),
@@ -217,6 +221,24 @@ class CoagGPU:
 
     def reset_counters(self):
         _check(load().pmc_gpu_reset_counters(self._h))
+
+    def part_steps(self):
+        out = np.zeros(self.n_box, dtype=np.int64)
+        _check(load().pmc_gpu_get_part_steps(self._h, _ptr(out)))
+        return out
+
+    def reserve(self, capacity):
+        cap = np.ascontiguousarray(capacity, dtype=np.int64)
+        _check(load().pmc_gpu_reserve(self._h, _ptr(cap)))
+
+    def set_timing(self, on: bool):
+        _check(load().pmc_gpu_set_timing(self._h, int(on)))
+
+    def kernel_time(self, name: str):
+        ms = np.zeros(1)
+        n = np.zeros(1, dtype=np.int64)
+        _check(load().pmc_gpu_get_kernel_time(self._h, name.encode(), _ptr(ms), _ptr(n)))
+        return float(ms[0]), int(n[0])
 
     # -- inspection ------------------------------------------------------
     def grid(self, box, max_bin=1024):

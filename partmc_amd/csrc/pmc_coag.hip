@@ -94,7 +94,58 @@ struct pmc_gpu {
   uint64_t seed = 0;
   uint32_t step = 0;
   ParallelWork work{};
+  // per-kernel HIP-event timing
+  int timing = 0;
+  struct Timed {
+    std::string name;
+    hipEvent_t start, stop;
+  };
+  std::vector<Timed> pending;
+  std::vector<std::pair<std::string, std::pair<double, int64_t>>> totals;
 };
+
+static void timing_begin(pmc_gpu *h, const char *name, hipEvent_t *start) {
+  (void)name;
+  if (!h->timing) return;
+  (void)hipEventCreate(start);
+  (void)hipEventRecord(*start, h->stream);
+}
+
+static void timing_end(pmc_gpu *h, const char *name, hipEvent_t start) {
+  if (!h->timing) return;
+  hipEvent_t stop;
+  (void)hipEventCreate(&stop);
+  (void)hipEventRecord(stop, h->stream);
+  h->pending.push_back({name, start, stop});
+}
+
+static void timing_collect(pmc_gpu *h) {
+  if (h->pending.empty()) return;
+  (void)hipStreamSynchronize(h->stream);
+  for (auto &t : h->pending) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t.start, t.stop);
+    bool found = false;
+    for (auto &e : h->totals)
+      if (e.first == t.name) {
+        e.second.first += ms;
+        e.second.second += 1;
+        found = true;
+      }
+    if (!found) h->totals.push_back({t.name, {(double)ms, 1}});
+    (void)hipEventDestroy(t.start);
+    (void)hipEventDestroy(t.stop);
+  }
+  h->pending.clear();
+}
+
+#define LAUNCH_TIMED(h, name, ...)                 \
+  do {                                             \
+    hipEvent_t ev_start_;                          \
+    timing_begin(h, name, &ev_start_);             \
+    hipLaunchKernelGGL(__VA_ARGS__);               \
+    timing_end(h, name, ev_start_);                \
+  } while (0)
 
 static void free_particles(pmc_gpu *h) {
   Arrays &A = h->A;
@@ -608,11 +659,11 @@ int pmc_gpu_mc_coag_async(pmc_gpu *h, int kernel_type, double del_t) {
     if (h->cap[b] == 0) return fail(ERR_STATE, "box %d has no particle storage: upload first", b);
   HIPCHECK(hipSetDevice(h->device));
   if (h->mode == PMC_MODE_EXACT) {
-    hipLaunchKernelGGL(k_prepare, dim3(h->n_box), dim3(SORT_THREADS), 0, h->stream, h->d_box, h->A,
-                       -1);
+    LAUNCH_TIMED(h, "k_prepare", k_prepare, dim3(h->n_box), dim3(SORT_THREADS), 0, h->stream,
+                 h->d_box, h->A, -1);
     if ((e = enqueue_tables(h, kernel_type))) return e;
-    hipLaunchKernelGGL(k_exact_sweep, dim3(h->n_box), dim3(64), 0, h->stream, h->d_box, h->A,
-                       kernel_type, del_t, h->seed, h->step);
+    LAUNCH_TIMED(h, "k_exact_sweep", k_exact_sweep, dim3(h->n_box), dim3(64), 0, h->stream,
+                 h->d_box, h->A, kernel_type, del_t, h->seed, h->step);
     HIPCHECK(hipGetLastError());
   } else {
     e = parallel_step(h->work, h->A, h->d_box, h->n_box, h->cap, h->max_bin, h->stream,
@@ -674,9 +725,62 @@ int pmc_gpu_reset_counters(pmc_gpu *h) {
     B.n_samp = B.n_coag = B.n_warn = B.n_recheck = B.n_flip = 0;
     B.max_rel_err = 0.0;
     B.n_rounds = B.n_dropped = 0;
+    B.part_steps = 0;
   }
   HIPCHECK(hipMemcpy(h->d_box, hb.data(), sizeof(BoxDesc) * h->n_box, hipMemcpyHostToDevice));
   return 0;
+}
+
+int pmc_gpu_get_part_steps(pmc_gpu *h, int64_t *part_steps) {
+  if (!h || !part_steps) return fail(ERR_ARG, "get_part_steps: bad arguments");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  std::vector<BoxDesc> hb(h->n_box);
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
+  for (int b = 0; b < h->n_box; ++b) part_steps[b] = hb[b].part_steps;
+  return 0;
+}
+
+int pmc_gpu_set_timing(pmc_gpu *h, int on) {
+  if (!h) return fail(ERR_ARG, "null handle");
+  HIPCHECK(hipSetDevice(h->device));
+  timing_collect(h);
+  h->totals.clear();
+  h->timing = on ? 1 : 0;
+  return 0;
+}
+
+int pmc_gpu_get_kernel_time(pmc_gpu *h, const char *kernel, double *total_ms, int64_t *launches) {
+  if (!h || !kernel) return fail(ERR_ARG, "get_kernel_time: bad arguments");
+  HIPCHECK(hipSetDevice(h->device));
+  timing_collect(h);
+  double ms = 0.0;
+  int64_t n = 0;
+  for (auto &e : h->totals)
+    if (e.first == kernel) {
+      ms = e.second.first;
+      n = e.second.second;
+    }
+  if (total_ms) *total_ms = ms;
+  if (launches) *launches = n;
+  return 0;
+}
+
+int pmc_gpu_reserve(pmc_gpu *h, const int64_t *capacity) {
+  if (!h || !capacity) return fail(ERR_ARG, "reserve: bad arguments");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  std::vector<int64_t> nc = h->cap;
+  bool grow = false;
+  for (int b = 0; b < h->n_box; ++b) {
+    int64_t want = std::max<int64_t>(capacity[b], 16);
+    if (want > nc[b]) {
+      nc[b] = want;
+      grow = true;
+    }
+  }
+  if (!grow) return 0;
+  return layout(h, nc);
 }
 
 int pmc_gpu_get_grid(pmc_gpu *h, int box, int *n_bin, double *edges, double *centers,

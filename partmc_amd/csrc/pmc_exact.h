@@ -945,6 +945,7 @@ k_exact_sweep(BoxDesc *boxes, Arrays A, int type, double del_t, uint64_t seed,
     s_env = B->env;
     B->step_samp = 0;
     B->step_coag = 0;
+    B->part_steps += B->n_part;
   }
   __syncthreads();
   if (B->error) return;

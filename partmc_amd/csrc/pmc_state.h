@@ -23,6 +23,7 @@ struct BoxDesc {
   double max_rel_err;
   // counters of the last step (exact mode)
   int64_t step_samp, step_coag;
+  int64_t part_steps;        // sum over steps of n_part at step start
   int n_part;
   int n_bin;
   int pool_cur, pool_top;
