@@ -172,8 +172,8 @@ static int alloc_tables(pmc_gpu *h, int classes) {
   Arrays &A = h->A;
   if (A.kw) (void)hipFree(A.kw);
   if (A.cc) (void)hipFree(A.cc);
-  size_t n = (size_t)h->n_box * MAX_CLASS * MAX_CLASS * h->max_bin * h->max_bin;
-  (void)classes;
+  size_t n = (size_t)h->n_box * classes * classes * h->max_bin * h->max_bin;
+  A.tab_classes = classes;
   HIPCHECK(dmalloc(&A.kw, n));
   HIPCHECK(dmalloc(&A.cc, n));
   h->tab_classes = classes;
@@ -259,7 +259,8 @@ static int layout(pmc_gpu *h, const std::vector<int64_t> &new_cap) {
   h->A = A;
   h->cap = new_cap;
   HIPCHECK(hipMemcpy(h->d_box, hb.data(), sizeof(BoxDesc) * h->n_box, hipMemcpyHostToDevice));
-  return parallel_layout(h->work, h->A, h->n_box, new_cap, h->max_bin);
+  parallel_free_arrays(h->work);  // re-laid out lazily by the next parallel step
+  return 0;
 }
 
 template <class T>
@@ -333,6 +334,9 @@ int pmc_gpu_create(pmc_gpu **out, int device, int n_box, int n_spec, const doubl
   h->cap.assign(n_box, 0);
   err = parallel_init(h->work, n_box, max_bin);
   if (err) return fail(err, "parallel work allocation failed");
+  h->work.tbegin = [](void *c, const char *n, hipEvent_t *e) { timing_begin((pmc_gpu *)c, n, e); };
+  h->work.tend = [](void *c, const char *n, hipEvent_t e) { timing_end((pmc_gpu *)c, n, e); };
+  h->work.tctx = h;
   *out = h;
   return 0;
 }
@@ -357,7 +361,11 @@ int pmc_gpu_set_env(pmc_gpu *h, int box, double temp, double pressure, double ad
   int e = check_box(h, box);
   if (e) return e;
   Env env{temp, pressure, additive_coeff};
-  return SET_FIELD(h, box, env, env);
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  if ((e = SET_FIELD(h, box, env, env))) return e;
+  int zero = 0;
+  return SET_FIELD(h, box, terms_kernel, zero);
 }
 
 int pmc_gpu_set_weights(pmc_gpu *h, int box, int n_group, int n_class, const int *type,
@@ -379,12 +387,23 @@ int pmc_gpu_set_weights(pmc_gpu *h, int box, int n_group, int n_class, const int
     w.expo[i] = exponent[i];
   }
   HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
   e = SET_FIELD(h, box, w, w);
   if (e) return e;
   int zero = 0;
   e = SET_FIELD(h, box, tables_valid, zero);
   if (e) return e;
   // class count changes the key space of the lists
+  if ((e = SET_FIELD(h, box, xlists_valid, zero))) return e;
+  if ((e = SET_FIELD(h, box, pkeys_valid, zero))) return e;
+  if (n_class > h->tab_classes) {
+    if ((e = alloc_tables(h, n_class))) return e;
+    if ((e = parallel_classes(h->work, h->n_box, n_class))) return fail(e, "%s", parallel_last_error());
+    std::vector<BoxDesc> hb(h->n_box);
+    HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
+    for (auto &B : hb) B.tables_valid = 0;
+    HIPCHECK(hipMemcpy(h->d_box, hb.data(), sizeof(BoxDesc) * h->n_box, hipMemcpyHostToDevice));
+  }
   return SET_FIELD(h, box, valid_sort, zero);
 }
 
@@ -507,6 +526,9 @@ int pmc_gpu_upload(pmc_gpu *h, int box, int n_part, int capacity, const double *
   }
   hb.n_part = n_part;
   hb.valid_sort = 0;
+  hb.xlists_valid = 0;
+  hb.pkeys_valid = 0;
+  hb.terms_kernel = 0;
   hb.n_info = 0;
   hb.error = 0;
   HIPCHECK(hipMemcpy(h->d_box + box, &hb, sizeof(BoxDesc), hipMemcpyHostToDevice));
@@ -642,7 +664,7 @@ static int enqueue_tables(pmc_gpu *h, int kernel_type) {
   hipLaunchKernelGGL(k_kbounds, gb, dim3(256), 0, h->stream, h->d_box, h->A, kernel_type);
   hipLaunchKernelGGL(k_mark_bounds, dim3((h->n_box + 63) / 64), dim3(64), 0, h->stream, h->d_box,
                      h->n_box, kernel_type);
-  dim3 gt((MAX_CLASS * MAX_CLASS * mb * mb + 255) / 256, h->n_box);
+  dim3 gt((h->tab_classes * h->tab_classes * mb * mb + 255) / 256, h->n_box);
   hipLaunchKernelGGL(k_pair_tables, gt, dim3(256), 0, h->stream, h->d_box, h->A);
   hipLaunchKernelGGL(k_mark_tables, dim3((h->n_box + 63) / 64), dim3(64), 0, h->stream, h->d_box,
                      h->n_box);

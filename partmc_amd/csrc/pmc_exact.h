@@ -181,9 +181,10 @@ k_prepare(BoxDesc *boxes, Arrays A, int only_box) {
     }
     __syncthreads();
   } else {
-    // r_min / r_max
+    // r_min / r_max; lists left stale by the parallel mode count as unsorted
+    const bool use_lists = B->valid_sort && B->xlists_valid;
     double rmin = 0.0, rmax = 0.0;
-    if (!B->valid_sort) {
+    if (!use_lists) {
       double lo = 1e300, hi = -1e300;
       for (int i = t; i < s_n; i += SORT_THREADS) {
         double r = vol2rad(A.vtot[base + i]);
@@ -208,7 +209,7 @@ k_prepare(BoxDesc *boxes, Arrays A, int only_box) {
     if (t == 0) {
       B->regridded = 0;
       const int nb = B->n_bin;
-      if (B->valid_sort) {
+      if (use_lists) {
         int i_min = 0, i_max = 0;
         for (int b = 1; b <= nb; ++b) {
           int tot = 0;
@@ -230,7 +231,7 @@ k_prepare(BoxDesc *boxes, Arrays A, int only_box) {
       }
       bool done = false;
       if (rmax == 0.0) {
-        if (!B->valid_sort) {
+        if (!use_lists) {
           B->n_bin = 0;
           B->bounds_valid = 0;
           B->tables_valid = 0;
@@ -262,7 +263,7 @@ k_prepare(BoxDesc *boxes, Arrays A, int only_box) {
             B->regridded = 1;
             s_need_sort = 1;
           }
-        } else if (!B->valid_sort) {
+        } else if (!use_lists) {
           s_need_sort = 1;
         }
       }
@@ -342,7 +343,11 @@ k_prepare(BoxDesc *boxes, Arrays A, int only_box) {
     }
   }
   __syncthreads();
-  if (t == 0) B->valid_sort = 1;
+  if (t == 0) {
+    B->valid_sort = 1;
+    B->xlists_valid = 1;
+    B->pkeys_valid = 0;
+  }
 }
 
 // ------------------------------------------------------------- k bounds
@@ -417,7 +422,7 @@ __global__ void k_pair_tables(BoxDesc *boxes, Arrays A) {
   int cc = coag_dest_class(w, centers, b1, c1, c2);
   double f_max = max_coag_num_conc_factor(w, edges, b1, b2, c1, c2, cc);
   double k_max = A.kmax[(int64_t)box * A.max_bin * A.max_bin + (b1 - 1) + nb * (b2 - 1)];
-  int64_t o = (int64_t)box * MAX_CLASS * MAX_CLASS * A.max_bin * A.max_bin + idx;
+  int64_t o = (int64_t)box * A.tab_classes * A.tab_classes * A.max_bin * A.max_bin + idx;
   A.kw[o] = k_max * f_max;
   A.cc[o] = (int8_t)cc;
 }
@@ -430,7 +435,7 @@ __global__ void k_mark_tables(BoxDesc *boxes, int n_box) {
 
 __device__ inline int64_t table_index(const Arrays &A, int box, int nb, int nc,
                                       int b1, int b2, int c1, int c2) {
-  return (int64_t)box * MAX_CLASS * MAX_CLASS * A.max_bin * A.max_bin +
+  return (int64_t)box * A.tab_classes * A.tab_classes * A.max_bin * A.max_bin +
          (int64_t)((c1 - 1) * nc + (c2 - 1)) * nb * nb + (int64_t)(b1 - 1) * nb +
          (b2 - 1);
 }

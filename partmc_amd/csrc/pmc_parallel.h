@@ -177,25 +177,6 @@ __global__ void k_rng_draw(uint64_t seed, uint32_t box, uint32_t step, uint32_t 
   else out[i] = (double)rand_int(s, (int)p1);
 }
 
-// ------------------------------------------------------ parallel mode
-struct ParallelWork {
-  int ready;
-};
-
-static std::string g_parallel_error;
-inline const char *parallel_last_error() { return g_parallel_error.c_str(); }
-inline int parallel_init(ParallelWork &W, int, int) {
-  W.ready = 0;
-  return 0;
-}
-inline int parallel_layout(ParallelWork &, const Arrays &, int, const std::vector<int64_t> &, int) {
-  return 0;
-}
-inline void parallel_free(ParallelWork &) {}
-inline int parallel_step(ParallelWork &, const Arrays &, BoxDesc *, int, const std::vector<int64_t> &,
-                         int, hipStream_t, int, double, uint64_t, uint32_t, int, double, int) {
-  g_parallel_error = "parallel mode not built";
-  return ERR_UNSUPPORTED;
-}
-
 }  // namespace pmc
+
+#include "pmc_parallel_step.h"

@@ -1,0 +1,1238 @@
+// pmc_parallel_step.h -- the parallel-mode coagulation step.
+//
+// Per step and for every box at once:
+//   1. keys: each particle caches its (bin, class) list key, its total
+//      volume and (FP32 mode) its FP64-derived kernel terms; they change
+//      only when the particle is created, the grid is remade, or the
+//      environment or kernel type changes.  Live list counts are kept
+//      exact by atomics, so aero_sorted_remake_if_needed's occupancy test
+//      (src/aero_sorted.F90:264-292,341-351) needs no particle pass.
+//   2. sort: a stable counting sort of the keys (tile histograms, one
+//      device-wide scan, tile-local radix ranks) gives every (bin, class)
+//      list in particle order.
+//   3. pairs: one thread per bin pair draws n_samp ~ Poisson(k_max dt n1 n2)
+//      (compute_n_samp, src/coagulation.F90:648-702) from the pair's own
+//      stream; a device scan lays the tests of all pairs of all boxes out in
+//      the reference's loop order (src/coagulation.F90:74-94).
+//   4. tests: one thread per test pair draws its two entries
+//      (find_rand_pair, :766-803), evaluates the weighted kernel (FP64, or
+//      FP32 with an FP64 recheck inside the band) and accepts with
+//      probability k/k_max (maybe_coag_pair, :712-759).
+//   5. claims: an accepted test claims both particles with atomicMin of its
+//      sequential index; a test wins when it holds both claims, i.e. no
+//      earlier accepted test in the reference order touched either
+//      particle.  Winners merge (coagulate_weighting + coagulate,
+//      :810-985); losers are re-drawn in another round against the
+//      particles still untouched this step.
+//   6. compaction: slots freed by merges are filled from the tail.
+// The step is deterministic: every draw is addressed by (box, step, site,
+// ordinal) and every conflict is resolved by sequential index.
+#pragma once
+
+#include <hipcub/hipcub.hpp>
+
+#include <string>
+#include <vector>
+
+namespace pmc {
+
+constexpr int TILE_THREADS = 256;
+constexpr int TILE_ITEMS = 16;
+constexpr int TILE = TILE_THREADS * TILE_ITEMS;
+constexpr uint32_t UNCLAIMED = 0xFFFFFFFFu;
+constexpr uint8_t ST_LIVE = 0, ST_HOLE = 1, ST_NEW = 2;
+constexpr int MAX_ROUNDS = 16;
+constexpr int MAX_REDRAW = 64;
+
+struct PW {  // device view of the parallel work arrays
+  int K_stride;     // keys per box in the histogram layout
+  int64_t P_stride; // pair slots per box
+  int tab_classes, max_bin;
+  int *key;         // [total_cap] list key, -1 for no key
+  uint8_t *status;  // [total_cap]
+  uint32_t *claim;  // [total_cap]
+  float4 *terms;    // [total_cap]
+  int *perm;        // [total_cap] global slots sorted by (box, key)
+  int *tile_box, *tile_start, *tiles_in_box;
+  int64_t *hist_off;  // [n_box]
+  uint32_t *hist, *hist_scan;
+  int *lstart, *lcnt, *live;  // [n_box * K_stride]
+  uint32_t *pair_n, *pair_exc;  // [n_box * P_stride (+1)]
+  int *acc, *acc_scan, *tp1, *tp2;
+  uint32_t *tid, *pend;
+  uint32_t *ev_t;
+  int *ev_p1, *ev_p2, *ev_box, *ev_out;
+  int *f_app, *f_id, *f_info, *f_hole, *f_lose;
+  int *s_app, *s_id, *s_info, *s_hole, *s_lose;
+  int *box_first;
+  uint64_t *holes, *holes_sorted;  // (box << 32 | slot)
+  int *tail_live;
+};
+
+struct ParallelWork {
+  int n_box = 0;
+  int64_t T_cap = 1 << 22;
+  int64_t hole_cap = 0;
+  int64_t H = 0, NP = 0;
+  int n_tiles = 0;
+  PW d{};
+  void *scan_tmp = nullptr;
+  size_t scan_tmp_bytes = 0;
+  void *sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  std::vector<int> tiles_in_box;
+  uint32_t *h_u32 = nullptr;  // pinned scalars
+  int *h_i32 = nullptr;
+  // optional per-kernel timing hooks of the owning handle
+  void (*tbegin)(void *, const char *, hipEvent_t *) = nullptr;
+  void (*tend)(void *, const char *, hipEvent_t) = nullptr;
+  void *tctx = nullptr;
+  int64_t n_dropped = 0;
+};
+
+#define PLAUNCH(W, name, ...)                                   \
+  do {                                                          \
+    hipEvent_t ev_;                                             \
+    if ((W).tbegin) (W).tbegin((W).tctx, name, &ev_);           \
+    hipLaunchKernelGGL(__VA_ARGS__);                            \
+    if ((W).tend) (W).tend((W).tctx, name, ev_);                \
+  } while (0)
+
+static std::string g_parallel_error;
+inline const char *parallel_last_error() { return g_parallel_error.c_str(); }
+
+#define PCHECK(expr)                                                       \
+  do {                                                                     \
+    hipError_t e_ = (expr);                                                \
+    if (e_ != hipSuccess) {                                                \
+      g_parallel_error = std::string(#expr) + ": " + hipGetErrorString(e_); \
+      return ERR_HIP;                                                      \
+    }                                                                      \
+  } while (0)
+
+template <class T>
+static hipError_t pmalloc(T **p, size_t n) {
+  return hipMalloc((void **)p, (n > 0 ? n : 1) * sizeof(T));
+}
+
+// ------------------------------------------------------------ kernels
+__device__ inline int par_key(int bin, int cls, int nb) { return (bin - 1) + nb * (cls - 1); }
+
+// keys (and optionally FP32 terms and cached volumes) for every particle of
+// boxes whose keys are stale; terms for boxes whose terms are stale
+__global__ void __launch_bounds__(TILE_THREADS)
+k_par_rekey(BoxDesc *boxes, Arrays A, PW W, int type, int want_terms, int phase) {
+  const int tile = blockIdx.x;
+  const int box = W.tile_box[tile];
+  BoxDesc *B = &boxes[box];
+  if (B->error) return;
+  const bool do_keys = phase == 0 ? (!B->pkeys_valid && B->valid_sort && B->n_bin > 0)
+                                  : (B->rekey != 0);
+  const bool do_terms = want_terms && phase == 0 && B->terms_kernel != type;
+  if (!do_keys && !do_terms) return;
+  const int n = B->n_part;
+  const int nb = B->n_bin;
+  const Env env = B->env;
+  const int64_t base = B->off;
+  int *live = W.live + (int64_t)box * W.K_stride;
+  for (int it = 0; it < TILE_ITEMS; ++it) {
+    int i = W.tile_start[tile] + it * TILE_THREADS + threadIdx.x;
+    if (i >= n) continue;
+    int64_t p = base + i;
+    double v = particle_volume(A, p);
+    A.vtot[p] = v;
+    if (do_keys) {
+      int b = bin_find_log(nb, B->log_lo, B->log_hi, vol2rad(v));
+      if (b < 1 || b > nb) {
+        atomicExch(&B->error, 708651144);
+        continue;
+      }
+      int k = par_key(b, A.klass[p], nb);
+      W.key[p] = k;
+      atomicAdd(&live[k], 1);
+    }
+    if (do_terms) {
+      double m = 0.0;
+      for (int s = 0; s < A.n_spec; ++s) m += A.vol[(int64_t)s * A.total_cap + p] * A.density[s];
+      W.terms[p] = particle_terms(type, v, m / v, env);
+    }
+  }
+}
+
+__global__ void k_par_zero_live(BoxDesc *boxes, PW W, int n_box, int phase) {
+  int box = blockIdx.y;
+  BoxDesc *B = &boxes[box];
+  bool do_keys = phase == 0 ? (!B->pkeys_valid && B->valid_sort && B->n_bin > 0) : (B->rekey != 0);
+  if (!do_keys || B->error) return;
+  for (int k = threadIdx.x; k < W.K_stride; k += blockDim.x) W.live[(int64_t)box * W.K_stride + k] = 0;
+}
+
+__global__ void k_par_finish_rekey(BoxDesc *boxes, int n_box, int type, int want_terms, int phase) {
+  int box = blockIdx.x * blockDim.x + threadIdx.x;
+  if (box >= n_box) return;
+  BoxDesc *B = &boxes[box];
+  if (phase == 0) {
+    if (!B->pkeys_valid && B->valid_sort && B->n_bin > 0) B->pkeys_valid = 1;
+    // FP64 steps do not keep the terms of new particles current
+    B->terms_kernel = want_terms ? type : 0;
+  } else if (B->rekey) {
+    B->rekey = 0;
+    B->pkeys_valid = 1;
+  }
+}
+
+// radius extremes of boxes without a valid sort (r_min/r_max loop of
+// src/aero_sorted.F90:293-304); positive doubles order like their bits
+__global__ void __launch_bounds__(TILE_THREADS)
+k_par_minmax(BoxDesc *boxes, Arrays A, PW W) {
+  const int tile = blockIdx.x;
+  const int box = W.tile_box[tile];
+  BoxDesc *B = &boxes[box];
+  if (B->error || (B->valid_sort && B->pkeys_valid)) return;
+  const int n = B->n_part;
+  unsigned long long lo = ~0ull, hi = 0ull;
+  for (int it = 0; it < TILE_ITEMS; ++it) {
+    int i = W.tile_start[tile] + it * TILE_THREADS + threadIdx.x;
+    if (i >= n) continue;
+    double r = vol2rad(particle_volume(A, B->off + i));
+    unsigned long long bits = (unsigned long long)__double_as_longlong(r);
+    lo = min(lo, bits);
+    hi = max(hi, bits);
+  }
+  if (hi) {
+    atomicMin(&B->rmin_bits, lo);
+    atomicMax(&B->rmax_bits, hi);
+  }
+}
+
+__global__ void k_par_reset_minmax(BoxDesc *boxes, int n_box) {
+  int box = blockIdx.x * blockDim.x + threadIdx.x;
+  if (box >= n_box) return;
+  boxes[box].rmin_bits = ~0ull;
+  boxes[box].rmax_bits = 0ull;
+}
+
+// aero_sorted_remake_if_needed, src/aero_sorted.F90:214-373, one thread
+// per box, occupancy from the live counts
+__global__ void k_par_remake(BoxDesc *boxes, Arrays A, PW W, int n_box) {
+  int box = blockIdx.x * blockDim.x + threadIdx.x;
+  if (box >= n_box) return;
+  BoxDesc *B = &boxes[box];
+  B->regridded = 0;
+  B->rekey = 0;
+  if (B->error) return;
+  double *edges = A.edges + (int64_t)box * (A.max_bin + 1);
+  double *centers = A.centers + (int64_t)box * A.max_bin;
+  double *widths = A.widths + (int64_t)box * A.max_bin;
+  const int nb = B->n_bin, nc = B->w.n_class;
+  const bool use_counts = B->valid_sort && B->pkeys_valid;
+  double rmin = 0.0, rmax = 0.0;
+  if (use_counts) {
+    const int *live = W.live + (int64_t)box * W.K_stride;
+    int i_min = 0, i_max = 0;
+    for (int b = 1; b <= nb; ++b) {
+      int tot = 0;
+      for (int c = 1; c <= nc; ++c) tot += live[par_key(b, c, nb)];
+      if (tot > 0) {
+        if (i_min == 0) i_min = b;
+        i_max = b;
+      }
+    }
+    if (i_min == 0) {
+      if (nb > 0) {
+        rmin = edges[nb];
+        rmax = edges[0];
+      }
+    } else {
+      rmin = edges[i_min - 1];
+      rmax = edges[i_max];
+    }
+  } else if (B->n_part > 0) {
+    rmin = __longlong_as_double((long long)B->rmin_bits);
+    rmax = __longlong_as_double((long long)B->rmax_bits);
+  }
+  if (rmax == 0.0) {
+    if (!use_counts) {
+      B->n_bin = 0;
+      B->bounds_valid = 0;
+      B->tables_valid = 0;
+    }
+    B->valid_sort = 1;
+    B->pkeys_valid = 1;
+    return;
+  }
+  bool need = (nb < 1) || (rmin / edges[0] < 3.0) || (edges[nb] / rmax < 3.0);
+  if (need) {
+    double gmin = rmin / 10.0, gmax = rmax * 10.0;
+    double nbf = ceil((log10(gmax) - log10(gmin)) * 10.0);
+    if (nbf < 0.0) {
+      B->error = 538534122;
+      return;
+    }
+    if (nbf > (double)A.max_bin) {
+      B->error = ERR_CAPACITY;
+      return;
+    }
+    int nn = (int)nbf;
+    make_log_grid(nn, gmin, gmax, edges, centers, widths);
+    B->n_bin = nn;
+    B->log_lo = log(edges[0]);
+    B->log_hi = log(edges[nn]);
+    B->bounds_valid = 0;
+    B->tables_valid = 0;
+    B->regridded = 1;
+    B->rekey = 1;
+  } else if (!use_counts) {
+    B->rekey = 1;
+  }
+  B->valid_sort = 1;
+  B->xlists_valid = 0;
+}
+
+// tile histogram of keys into hist[box][key][tile]
+__global__ void __launch_bounds__(TILE_THREADS)
+k_par_hist(BoxDesc *boxes, PW W) {
+  const int tile = blockIdx.x;
+  const int box = W.tile_box[tile];
+  BoxDesc *B = &boxes[box];
+  extern __shared__ int s_h[];
+  const int K = W.K_stride;
+  for (int k = threadIdx.x; k < K; k += TILE_THREADS) s_h[k] = 0;
+  __syncthreads();
+  const int n = B->error ? 0 : B->n_part;
+  const int64_t base = B->off;
+  for (int it = 0; it < TILE_ITEMS; ++it) {
+    int i = W.tile_start[tile] + it * TILE_THREADS + threadIdx.x;
+    if (i < n) atomicAdd(&s_h[W.key[base + i]], 1);
+  }
+  __syncthreads();
+  const int t_in_box = W.tile_start[tile] / TILE;
+  const int nt = W.tiles_in_box[box];
+  uint32_t *h = W.hist + W.hist_off[box];
+  for (int k = threadIdx.x; k < K; k += TILE_THREADS) h[(int64_t)k * nt + t_in_box] = (uint32_t)s_h[k];
+}
+
+// list starts and counts of every (box, key) from the scanned histogram
+__global__ void k_par_lists(BoxDesc *boxes, PW W, int n_box) {
+  const int box = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= W.K_stride) return;
+  const int nt = W.tiles_in_box[box];
+  const int64_t o = W.hist_off[box];
+  const uint32_t base = W.hist_scan[o];
+  const uint32_t s = W.hist_scan[o + (int64_t)k * nt];
+  const uint32_t e = W.hist_scan[o + (int64_t)k * nt + nt - 1] + W.hist[o + (int64_t)k * nt + nt - 1];
+  W.lstart[(int64_t)box * W.K_stride + k] = (int)(s - base);
+  W.lcnt[(int64_t)box * W.K_stride + k] = (int)(e - s);
+}
+
+// stable scatter: the tile's keys are radix-sorted (stable) so each item's
+// rank among equal keys of the tile is its sorted position minus the
+// position of the first item with that key.
+__global__ void __launch_bounds__(TILE_THREADS)
+k_par_scatter(BoxDesc *boxes, PW W, int key_bits) {
+  using BRS = hipcub::BlockRadixSort<int, TILE_THREADS, TILE_ITEMS, int>;
+  __shared__ typename BRS::TempStorage tmp;
+  __shared__ int s_keys[TILE];
+  const int tile = blockIdx.x;
+  const int box = W.tile_box[tile];
+  BoxDesc *B = &boxes[box];
+  const int n = B->error ? 0 : B->n_part;
+  const int64_t base = B->off;
+  const int sentinel = (1 << key_bits) - 1;
+  int keys[TILE_ITEMS], vals[TILE_ITEMS];
+  for (int it = 0; it < TILE_ITEMS; ++it) {
+    int i = W.tile_start[tile] + threadIdx.x * TILE_ITEMS + it;
+    vals[it] = i;
+    keys[it] = (i < n) ? W.key[base + i] : sentinel;
+  }
+  BRS(tmp).Sort(keys, vals, 0, key_bits);
+  for (int it = 0; it < TILE_ITEMS; ++it) s_keys[threadIdx.x * TILE_ITEMS + it] = keys[it];
+  __syncthreads();
+  const int t_in_box = W.tile_start[tile] / TILE;
+  const int nt = W.tiles_in_box[box];
+  const int64_t o = W.hist_off[box];
+  const uint32_t bbase = W.hist_scan[o];
+  for (int it = 0; it < TILE_ITEMS; ++it) {
+    int pos = threadIdx.x * TILE_ITEMS + it;
+    int k = keys[it];
+    if (k == sentinel) continue;
+    // first position of key k in the sorted tile (binary search)
+    int lo = 0, hi = pos;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (s_keys[mid] < k) lo = mid + 1; else hi = mid;
+    }
+    int rank = pos - lo;
+    uint32_t dst = W.hist_scan[o + (int64_t)k * nt + t_in_box] - bbase + (uint32_t)rank;
+    W.perm[base + dst] = (int)(base + vals[it]);
+  }
+}
+
+__device__ inline void decode_pair(int64_t q, int tc, int mb, int *c1, int *c2, int *b1, int *b2) {
+  int64_t mb2 = (int64_t)mb * mb;
+  int cls = (int)(q / mb2);
+  int rem = (int)(q % mb2);
+  *c1 = cls / tc + 1;
+  *c2 = cls % tc + 1;
+  *b1 = rem / mb + 1;
+  *b2 = rem % mb + 1;
+}
+
+// n_samp of every (box, c1, c2, b1, b2), compute_n_samp
+// src/coagulation.F90:648-702 on the pair's own Poisson stream
+__global__ void k_par_pairs(BoxDesc *boxes, Arrays A, PW W, double del_t, uint64_t seed,
+                            uint32_t step) {
+  const int box = blockIdx.y;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= W.P_stride) return;
+  BoxDesc *B = &boxes[box];
+  uint32_t ns = 0;
+  if (!B->error) {
+    int c1, c2, b1, b2;
+    decode_pair(q, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
+    const int nb = B->n_bin, nc = B->w.n_class;
+    if (c1 <= nc && c2 <= c1 && b1 <= nb && b2 <= nb && (c1 != c2 || b2 >= b1)) {
+      const int *cnt = W.lcnt + (int64_t)box * W.K_stride;
+      int n1 = cnt[par_key(b1, c1, nb)], n2 = cnt[par_key(b2, c2, nb)];
+      if (n1 > 0 && n2 > 0) {
+        bool same = (b1 == b2) && (c1 == c2);
+        double n_possible = same ? (double)n1 * ((double)n2 - 1.0) / 2.0 : (double)n1 * (double)n2;
+        double k_max = A.kw[table_index(A, box, nb, nc, b1, b2, c1, c2)];
+        double mean = k_max * del_t * n_possible;
+        Stream ps;
+        ps.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), 0, RNG_POISSON);
+        ns = (uint32_t)rand_poisson(ps, mean);
+        if (ns) atomicAdd((unsigned long long *)&B->n_samp, (unsigned long long)ns);
+      }
+    }
+  }
+  W.pair_n[(int64_t)box * W.P_stride + q] = ns;
+}
+
+__device__ inline int64_t find_pair(const uint32_t *exc, int64_t n, uint32_t t) {
+  // last index k with exc[k] <= t  (exc has n + 1 entries, exc[n] = total)
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if (exc[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ inline void atomic_max_double(double *addr, double v) {
+  unsigned long long *a = (unsigned long long *)addr;
+  unsigned long long nv = (unsigned long long)__double_as_longlong(v);
+  atomicMax(a, nv);  // v >= 0
+}
+
+// one thread per test: draw, evaluate, accept, claim
+__global__ void __launch_bounds__(256)
+k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, uint32_t step,
+            int round, uint32_t t0, int n, const uint32_t *pend, int precision, double band,
+            int validation) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t t = pend ? pend[i] : t0 + (uint32_t)i;
+  W.acc[i] = 0;
+  W.tid[i] = t;
+  const int64_t NP = (int64_t)n_box * W.P_stride;
+  const int64_t k = find_pair(W.pair_exc, NP, t);
+  const int box = (int)(k / W.P_stride);
+  BoxDesc *B = &boxes[box];
+  if (B->error) return;
+  int c1, c2, b1, b2;
+  decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
+  const uint32_t j = t - W.pair_exc[k];
+  const int nb = B->n_bin, nc = B->w.n_class;
+  const int64_t ti = table_index(A, box, nb, nc, b1, b2, c1, c2);
+  const int cc = A.cc[ti];
+  const double accept_factor = 1.0 / A.kw[ti];
+  const bool same = (b1 == b2) && (c1 == c2);
+  const int64_t lb = (int64_t)box * W.K_stride;
+  const int key1 = par_key(b1, c1, nb), key2 = par_key(b2, c2, nb);
+  const int n1 = W.lcnt[lb + key1], n2 = W.lcnt[lb + key2];
+  const int s1 = W.lstart[lb + key1], s2 = W.lstart[lb + key2];
+  const int64_t base = B->off;
+  Stream ts;
+  ts.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), j,
+          round == 0 ? RNG_TEST : RNG_RETRY + (uint32_t)round);
+  int p1 = -1, p2 = -1;
+  if (round == 0) {
+    int i1 = rand_int(ts, n1);
+    int i2;
+    if (same) {
+      i2 = rand_int(ts, n2 - 1);
+      if (i2 == i1) i2 = n2;
+    } else {
+      i2 = rand_int(ts, n2);
+    }
+    p1 = W.perm[base + s1 + i1 - 1];
+    p2 = W.perm[base + s2 + i2 - 1];
+  } else {
+    int i1 = 0;
+    for (int a = 0; a < MAX_REDRAW; ++a) {
+      int c = rand_int(ts, n1);
+      int p = W.perm[base + s1 + c - 1];
+      if (W.status[p] == ST_LIVE) {
+        i1 = c;
+        p1 = p;
+        break;
+      }
+    }
+    if (p1 >= 0 && (!same || n2 > 1)) {
+      for (int a = 0; a < MAX_REDRAW; ++a) {
+        int c;
+        if (same) {
+          c = rand_int(ts, n2 - 1);
+          if (c == i1) c = n2;
+        } else {
+          c = rand_int(ts, n2);
+        }
+        int p = W.perm[base + s2 + c - 1];
+        if (W.status[p] == ST_LIVE && p != p1) {
+          p2 = p;
+          break;
+        }
+      }
+    }
+    if (p1 < 0 || p2 < 0) {
+      atomicAdd(&B->n_dropped, 1);
+      return;
+    }
+  }
+  const Weights &w = B->w;
+  const Env env = B->env;
+  double u = ts.uniform();
+  bool accept;
+  double pr;
+  if (precision == 0) {
+    double k64;
+    int err = weighted_kernel64(A, w, env, type, p1, p2, c1, c2, cc, &k64);
+    if (err) atomicExch(&B->error, err);
+    pr = k64 * accept_factor;
+    accept = u < pr;
+  } else {
+    float k32 = weighted_kernel32(type, w, env, W.terms[p1], W.terms[p2], c1, c2, cc);
+    double p32 = (double)k32 * accept_factor;
+    bool need64 = !(fabs(u - p32) >= band) || validation;
+    double k64 = 0.0;
+    if (need64) {
+      int err = weighted_kernel64(A, w, env, type, p1, p2, c1, c2, cc, &k64);
+      if (err) atomicExch(&B->error, err);
+    }
+    if (!(fabs(u - p32) >= band)) {
+      atomicAdd((unsigned long long *)&B->n_recheck, 1ull);
+      pr = k64 * accept_factor;
+      accept = u < pr;
+    } else {
+      pr = p32;
+      accept = u < p32;
+      if (validation && (accept != (u < k64 * accept_factor)))
+        atomicAdd((unsigned long long *)&B->n_flip, 1ull);
+    }
+    if (validation && k64 > 0.0) atomic_max_double(&B->max_rel_err, fabs((double)k32 - k64) / k64);
+  }
+  if (!(pr <= 1.0)) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
+  if (accept) {
+    W.acc[i] = 1;
+    W.tp1[i] = p1;
+    W.tp2[i] = p2;
+    atomicMin(&W.claim[p1], t);
+    atomicMin(&W.claim[p2], t);
+  }
+}
+
+// compact accepted tests into the event list (in test order)
+__global__ void k_par_gather(PW W, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !W.acc[i]) return;
+  int e = W.acc_scan[i];
+  W.ev_t[e] = W.tid[i];
+  W.ev_p1[e] = W.tp1[i];
+  W.ev_p2[e] = W.tp2[i];
+}
+
+// outcome bits of ev_out
+constexpr int O_WIN = 1, O_RM1 = 2, O_RM2 = 4, O_NEW = 8, O_SLOT1 = 16, O_SLOT2 = 32,
+              O_APPEND = 64, O_ID1 = 128, O_ID2 = 256, O_NEWID = 512, O_INFO1 = 1024,
+              O_INFO2 = 2048;
+
+struct MergeCtx {
+  int box, c1, c2, cc, b1, b2;
+  uint32_t site, j;
+};
+
+__device__ inline MergeCtx merge_ctx(BoxDesc *boxes, const PW &W, int n_box, uint32_t t) {
+  MergeCtx m;
+  const int64_t NP = (int64_t)n_box * W.P_stride;
+  const int64_t k = find_pair(W.pair_exc, NP, t);
+  m.box = (int)(k / W.P_stride);
+  decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &m.c1, &m.c2, &m.b1, &m.b2);
+  m.j = t - W.pair_exc[k];
+  return m;
+}
+
+// coagulate_weighting (src/coagulation.F90:810-922): decisions only
+__device__ inline int merge_decide(const Arrays &A, const Weights &w, Stream &s, int64_t p1,
+                                   int64_t p2, int c1, int c2, int cc, int *new_group) {
+  double r1 = vol2rad(particle_volume(A, p1));
+  double r2 = vol2rad(particle_volume(A, p2));
+  double rc = vol2rad(rad2vol(r1) + rad2vol(r2));
+  double nc1 = nc_at_radius(w, c1, r1);
+  double nc2 = nc_at_radius(w, c2, r2);
+  double ncc = nc_at_radius(w, cc, rc);
+  double pdf[MAX_GROUP];
+  for (int g = 0; g < w.n_group; ++g) {
+    int idx = g + w.n_group * (cc - 1);
+    pdf[g] = 1.0 / weight_nc(w.type[idx], w.mag[idx], w.expo[idx], rc);
+  }
+  *new_group = sample_cts_pdf(s, w.n_group, pdf);
+  double nc_min = fmin(nc1, fmin(nc2, ncc));
+  bool rm1 = s.uniform() < nc_min / nc1;
+  bool rm2 = s.uniform() < nc_min / nc2;
+  bool cr = s.uniform() < nc_min / ncc;
+  int o = 0;
+  if (rm1) o |= O_RM1;
+  if (rm2) o |= O_RM2;
+  if (cr) o |= O_NEW;
+  bool id1_lost, id2_lost;
+  if (cr) {
+    id1_lost = false;
+    id2_lost = false;
+    if (rm1 && rm2) {
+      if (particle_volume(A, p1) > particle_volume(A, p2)) id2_lost = true;
+      else id1_lost = true;
+    }
+    if (rm1 && !id1_lost) o |= O_ID1;
+    else if (rm2 && !id2_lost) o |= O_ID2;
+    else o |= O_NEWID;
+    // the new particle takes a freed slot when there is one
+    if (rm1) o |= O_SLOT1;
+    else if (rm2) o |= O_SLOT2;
+    else o |= O_APPEND;
+  } else {
+    id1_lost = rm1;
+    id2_lost = rm2;
+  }
+  if (rm1 && id1_lost) o |= O_INFO1;
+  if (rm2 && id2_lost) o |= O_INFO2;
+  return o;
+}
+
+__global__ void k_par_resolve(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, uint64_t seed,
+                              uint32_t step) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  uint32_t t = W.ev_t[e];
+  int p1 = W.ev_p1[e], p2 = W.ev_p2[e];
+  MergeCtx m = merge_ctx(boxes, W, n_box, t);
+  W.ev_box[e] = m.box;
+  bool win = W.claim[p1] == t && W.claim[p2] == t;
+  int o = 0;
+  if (win) {
+    BoxDesc *B = &boxes[m.box];
+    const int nb = B->n_bin, nc = B->w.n_class;
+    int cc = A.cc[table_index(A, m.box, nb, nc, m.b1, m.b2, m.c1, m.c2)];
+    Stream s;
+    s.open(seed, m.box, step, pair_site(nb, nc, m.b1, m.b2, m.c1, m.c2), m.j, RNG_MERGE);
+    int g;
+    o = merge_decide(A, B->w, s, p1, p2, m.c1, m.c2, cc, &g) | O_WIN;
+  }
+  W.ev_out[e] = o;
+  W.f_app[e] = (o & O_APPEND) ? 1 : 0;
+  W.f_id[e] = (o & O_NEWID) ? 1 : 0;
+  W.f_info[e] = ((o & O_INFO1) ? 1 : 0) + ((o & O_INFO2) ? 1 : 0);
+  W.f_hole[e] = ((o & O_RM1) && !(o & O_SLOT1) ? 1 : 0) + ((o & O_RM2) && !(o & O_SLOT2) ? 1 : 0);
+  W.f_lose[e] = win ? 0 : 1;
+}
+
+// first event of every box (events are in test order, so box-major)
+__global__ void k_par_box_first(PW W, int E) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  if (e == 0 || W.ev_box[e - 1] != W.ev_box[e]) W.box_first[W.ev_box[e]] = e;
+}
+
+__device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc *B, int64_t slot,
+                                          const Rec &r, int nb, int type, int want_terms) {
+  double v = 0.0;
+  double m = 0.0;
+  for (int s = 0; s < A.n_spec; ++s) {
+    A.vol[(int64_t)s * A.total_cap + slot] = r.vol[s];
+    v += r.vol[s];
+    m += r.vol[s] * A.density[s];
+  }
+  A.vtot[slot] = v;
+  A.id[slot] = r.id;
+  A.group[slot] = r.group;
+  A.klass[slot] = r.klass;
+  A.nprim[slot] = r.nprim;
+  A.ncomp[slot] = r.ncomp;
+  for (int c = 0; c < MAX_COMP; ++c) {
+    A.csrc[(int64_t)c * A.total_cap + slot] = r.csrc[c];
+    A.ctime[(int64_t)c * A.total_cap + slot] = r.ctime[c];
+  }
+  A.least[slot] = r.least;
+  A.greatest[slot] = r.greatest;
+  W.status[slot] = ST_NEW;
+  int b = bin_find_log(nb, B->log_lo, B->log_hi, vol2rad(v));
+  if (b < 1 || b > nb) {
+    atomicExch(&B->error, 134572570);
+    W.key[slot] = 0;
+    return;
+  }
+  int k = par_key(b, r.klass, nb);
+  W.key[slot] = k;
+  if (want_terms) W.terms[slot] = particle_terms(type, v, m / v, B->env);
+}
+
+__device__ inline void load_rec(const Arrays &A, int64_t p, Rec &r) {
+  for (int s = 0; s < A.n_spec; ++s) r.vol[s] = A.vol[(int64_t)s * A.total_cap + p];
+  r.id = A.id[p];
+  r.group = A.group[p];
+  r.klass = A.klass[p];
+  r.nprim = A.nprim[p];
+  r.ncomp = A.ncomp[p];
+  for (int c = 0; c < MAX_COMP; ++c) {
+    r.csrc[c] = A.csrc[(int64_t)c * A.total_cap + p];
+    r.ctime[c] = A.ctime[(int64_t)c * A.total_cap + p];
+  }
+  r.least = A.least[p];
+  r.greatest = A.greatest[p];
+}
+
+// aero_particle_coagulate for the parallel path (same rules as rec_coagulate)
+__device__ inline int par_rec_coagulate(int n_spec, Stream &s, const Rec &p1, const Rec &p2,
+                                        Rec &o) {
+  for (int k = 0; k < n_spec; ++k) o.vol[k] = p1.vol[k] + p2.vol[k];
+  o.group = 0;
+  o.klass = 0;
+  o.id = 0;
+  int n1 = p1.ncomp, n2 = p2.ncomp;
+  if (p1.nprim < n1) return 465791384;
+  if (p2.nprim < n2) return 465791385;
+  for (int c = 0; c < MAX_COMP; ++c) {
+    o.csrc[c] = 0;
+    o.ctime[c] = 0.0;
+  }
+  if (n1 + n2 > MAX_COMP) {
+    int n1_new = prob_round(s, (double)p1.nprim / (double)(p1.nprim + p2.nprim) * (double)MAX_COMP);
+    int n2_new = MAX_COMP - n1_new;
+    int sample[MAX_COMP];
+    sample_without_replacement(s, n1_new, n1, sample);
+    for (int i = 0; i < n1_new; ++i) {
+      o.csrc[i] = p1.csrc[sample[i] - 1];
+      o.ctime[i] = p1.ctime[sample[i] - 1];
+    }
+    sample_without_replacement(s, n2_new, n2, sample);
+    for (int i = 0; i < n2_new; ++i) {
+      o.csrc[i + n1_new] = p2.csrc[sample[i] - 1];
+      o.ctime[i + n1_new] = p2.ctime[sample[i] - 1];
+    }
+    o.ncomp = MAX_COMP;
+  } else {
+    for (int i = 0; i < n1; ++i) {
+      o.csrc[i] = p1.csrc[i];
+      o.ctime[i] = p1.ctime[i];
+    }
+    for (int i = 0; i < n2; ++i) {
+      o.csrc[n1 + i] = p2.csrc[i];
+      o.ctime[n1 + i] = p2.ctime[i];
+    }
+    o.ncomp = n1 + n2;
+  }
+  o.least = fmin(p1.least, p2.least);
+  o.greatest = fmax(p1.greatest, p2.greatest);
+  o.nprim = p1.nprim + p2.nprim;
+  return 0;
+}
+
+// apply the winners' merges (coagulate, src/coagulation.F90:928-985)
+__global__ void k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, uint64_t seed,
+                            uint32_t step, int type, int want_terms, int64_t hole_base) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  int o = W.ev_out[e];
+  if (!(o & O_WIN)) return;
+  uint32_t t = W.ev_t[e];
+  int64_t p1 = W.ev_p1[e], p2 = W.ev_p2[e];
+  MergeCtx m = merge_ctx(boxes, W, n_box, t);
+  BoxDesc *B = &boxes[m.box];
+  const int nb = B->n_bin, nc = B->w.n_class;
+  const int cc = A.cc[table_index(A, m.box, nb, nc, m.b1, m.b2, m.c1, m.c2)];
+  int *live = W.live + (int64_t)m.box * W.K_stride;
+  const int first = W.box_first[m.box];
+  Stream s;
+  s.open(seed, m.box, step, pair_site(nb, nc, m.b1, m.b2, m.c1, m.c2), m.j, RNG_MERGE);
+  int new_group;
+  (void)merge_decide(A, B->w, s, p1, p2, m.c1, m.c2, cc, &new_group);
+  Rec pt1, pt2, ptc;
+  load_rec(A, p1, pt1);
+  load_rec(A, p2, pt2);
+  int64_t info_other = 0;
+  int64_t slot = -1;
+  if (o & O_NEW) {
+    int err = par_rec_coagulate(A.n_spec, s, pt1, pt2, ptc);
+    if (err) atomicExch(&B->error, err);
+    ptc.group = new_group;
+    ptc.klass = cc;
+    if (o & O_ID1) ptc.id = pt1.id;
+    else if (o & O_ID2) ptc.id = pt2.id;
+    else {
+      int64_t rank = W.s_id[e] - W.s_id[first];
+      ptc.id = (B->next_id + rank - 1) * B->id_stride + B->id_offset + 1;
+    }
+    info_other = ptc.id;
+    if (o & O_SLOT1) slot = p1;
+    else if (o & O_SLOT2) slot = p2;
+    else {
+      int64_t rank = W.s_app[e] - W.s_app[first];
+      if (B->n_part + rank >= B->cap) {
+        atomicExch(&B->error, ERR_CAPACITY);
+        return;
+      }
+      slot = B->off + B->n_part + rank;
+    }
+  }
+  // removals: live counts and holes
+  int64_t hrank = hole_base + W.s_hole[e];
+  if (o & O_RM1) {
+    atomicSub(&live[W.key[p1]], 1);
+    if (!(o & O_SLOT1)) {
+      W.status[p1] = ST_HOLE;
+      W.holes[hrank++] = ((uint64_t)m.box << 32) | (uint64_t)(p1 - B->off);
+    }
+  }
+  if (o & O_RM2) {
+    atomicSub(&live[W.key[p2]], 1);
+    if (!(o & O_SLOT2)) {
+      W.status[p2] = ST_HOLE;
+      W.holes[hrank++] = ((uint64_t)m.box << 32) | (uint64_t)(p2 - B->off);
+    }
+  }
+  if (slot >= 0) {
+    write_new_particle(A, W, B, slot, ptc, nb, type, want_terms);
+    atomicAdd(&live[W.key[slot]], 1);
+  }
+  // removal records (aero_info), in event order within the box
+  int64_t irank = B->n_info + (W.s_info[e] - W.s_info[first]);
+  if (o & O_INFO1) {
+    if (irank < B->info_cap) {
+      A.info_id[B->info_off + irank] = pt1.id;
+      A.info_action[B->info_off + irank] = 2;
+      A.info_other[B->info_off + irank] = info_other;
+    }
+    irank++;
+  }
+  if (o & O_INFO2) {
+    if (irank < B->info_cap) {
+      A.info_id[B->info_off + irank] = pt2.id;
+      A.info_action[B->info_off + irank] = 2;
+      A.info_other[B->info_off + irank] = info_other;
+    }
+  }
+}
+
+// per-box totals of a round: appended slots, new ids, records, winners
+__global__ void k_par_box_totals(BoxDesc *boxes, PW W, int n_box, int E) {
+  int box = blockIdx.x * blockDim.x + threadIdx.x;
+  if (box >= n_box) return;
+  int first = W.box_first[box];
+  if (first < 0) return;
+  int last = first;
+  {
+    int lo = first, hi = E - 1;  // last event of the box (ev_box sorted)
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (W.ev_box[mid] == box) lo = mid; else hi = mid - 1;
+    }
+    last = lo;
+  }
+  BoxDesc *B = &boxes[box];
+  int app = W.s_app[last] + W.f_app[last] - W.s_app[first];
+  int ids = W.s_id[last] + W.f_id[last] - W.s_id[first];
+  int info = W.s_info[last] + W.f_info[last] - W.s_info[first];
+  int lose = W.s_lose[last] + W.f_lose[last] - W.s_lose[first];
+  int wins = (last - first + 1) - lose;
+  B->n_part += app;
+  B->next_id += ids;
+  B->n_info += info;
+  B->n_coag += wins;
+  B->step_coag += wins;
+}
+
+__global__ void k_par_losers(PW W, int E) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  if (W.f_lose[e]) W.pend[W.s_lose[e]] = W.ev_t[e];
+  W.claim[W.ev_p1[e]] = UNCLAIMED;
+  W.claim[W.ev_p2[e]] = UNCLAIMED;
+}
+
+__global__ void k_fill_int(int *p, int64_t n, int v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_step_begin(BoxDesc *boxes, int n_box) {
+  int box = blockIdx.x * blockDim.x + threadIdx.x;
+  if (box >= n_box) return;
+  BoxDesc *B = &boxes[box];
+  B->step_samp = 0;
+  B->step_coag = 0;
+  B->part_steps += B->n_part;
+}
+
+__global__ void k_step_samp(BoxDesc *boxes, PW W, int n_box) {
+  // step_samp = sum of this step's pair counts of the box
+  int box = blockIdx.x * blockDim.x + threadIdx.x;
+  if (box >= n_box) return;
+  int64_t o = (int64_t)box * W.P_stride;
+  boxes[box].step_samp = (int64_t)W.pair_exc[o + W.P_stride] - (int64_t)W.pair_exc[o];
+}
+
+// tail-fill compaction, one block per box: holes (sorted by slot) below the
+// final count receive the live particles of the tail, in slot order
+__global__ void __launch_bounds__(256)
+k_par_compact(BoxDesc *boxes, Arrays A, PW W, int n_holes) {
+  const int box = blockIdx.x;
+  BoxDesc *B = &boxes[box];
+  __shared__ int s_lo, s_hi, s_cnt;
+  if (threadIdx.x == 0) {
+    // hole range of this box in the sorted list
+    uint64_t key = (uint64_t)box << 32;
+    int lo = 0, hi = n_holes;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (W.holes_sorted[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    int a = lo;
+    lo = a;
+    hi = n_holes;
+    uint64_t key2 = (uint64_t)(box + 1) << 32;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (W.holes_sorted[mid] < key2) lo = mid + 1; else hi = mid;
+    }
+    s_lo = a;
+    s_hi = lo;
+    s_cnt = 0;
+  }
+  __syncthreads();
+  const int H = s_hi - s_lo;
+  if (H == 0) return;
+  const int n = B->n_part;
+  const int n_final = n - H;
+  const int64_t base = B->off;
+  // live slots of the tail [n_final, n), in slot order (block scan)
+  using BS = hipcub::BlockScan<int, 256>;
+  __shared__ typename BS::TempStorage tmp;
+  __shared__ int s_run;
+  if (threadIdx.x == 0) s_run = 0;
+  __syncthreads();
+  int *tail = W.tail_live + (int64_t)blockIdx.x * 0;  // per-box region below
+  for (int c0 = n_final; c0 < n; c0 += 256) {
+    int i = c0 + threadIdx.x;
+    int f = (i < n && W.status[base + i] != ST_HOLE) ? 1 : 0;
+    int r, tot;
+    BS(tmp).ExclusiveSum(f, r, tot);
+    if (f) W.tail_live[(int64_t)s_lo + s_run + r] = i;
+    __syncthreads();
+    if (threadIdx.x == 0) s_run += tot;
+    __syncthreads();
+  }
+  (void)tail;
+  // head holes are the first entries of the sorted range with slot < n_final
+  for (int j = threadIdx.x; j < H; j += 256) {
+    int hole = (int)(W.holes_sorted[s_lo + j] & 0xFFFFFFFFu);
+    if (hole >= n_final) continue;
+    int from = W.tail_live[s_lo + j];
+    int64_t pf = base + from, pt = base + hole;
+    copy_particle(A, pf, pt);
+    W.key[pt] = W.key[pf];
+    W.terms[pt] = W.terms[pf];
+    W.status[pt] = W.status[pf];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) B->n_part = n_final;
+}
+
+__global__ void k_par_invalidate(BoxDesc *boxes, int n_box) {
+  int box = blockIdx.x * blockDim.x + threadIdx.x;
+  if (box >= n_box) return;
+  boxes[box].pkeys_valid = 0;
+  boxes[box].terms_kernel = 0;
+}
+
+__global__ void k_par_end(BoxDesc *boxes, int n_box) {
+  int box = blockIdx.x * blockDim.x + threadIdx.x;
+  if (box >= n_box) return;
+  BoxDesc *B = &boxes[box];
+  B->valid_sort = 1;
+  B->xlists_valid = 0;
+  B->n_rounds += 0;
+}
+
+// ------------------------------------------------------------ host side
+inline int parallel_init(ParallelWork &W, int n_box, int max_bin) {
+  W.n_box = n_box;
+  W.d.max_bin = max_bin;
+  W.d.tab_classes = 1;
+  if (hipHostMalloc((void **)&W.h_u32, 64 * sizeof(uint32_t)) != hipSuccess) return ERR_HIP;
+  if (hipHostMalloc((void **)&W.h_i32, 64 * sizeof(int)) != hipSuccess) return ERR_HIP;
+  return 0;
+}
+
+inline void parallel_free_arrays(ParallelWork &W) {
+  PW &d = W.d;
+  void *ptrs[] = {d.key,  d.status, d.claim, d.terms, d.perm, d.tile_box, d.tile_start,
+                  d.tiles_in_box, d.hist_off, d.hist, d.hist_scan, d.lstart, d.lcnt, d.live,
+                  d.pair_n, d.pair_exc, d.acc, d.acc_scan, d.tp1, d.tp2, d.tid, d.pend, d.ev_t,
+                  d.ev_p1, d.ev_p2, d.ev_box, d.ev_out, d.f_app, d.f_id, d.f_info, d.f_hole,
+                  d.f_lose, d.s_app, d.s_id, d.s_info, d.s_hole, d.s_lose, d.box_first,
+                  d.holes, d.holes_sorted, d.tail_live, W.scan_tmp, W.sort_tmp};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  int ks = d.K_stride;
+  int64_t ps = d.P_stride;
+  int tc = d.tab_classes, mb = d.max_bin;
+  d = PW{};
+  d.K_stride = ks;
+  d.P_stride = ps;
+  d.tab_classes = tc;
+  d.max_bin = mb;
+  W.scan_tmp = W.sort_tmp = nullptr;
+  W.scan_tmp_bytes = W.sort_tmp_bytes = 0;
+}
+
+inline void parallel_free(ParallelWork &W) {
+  parallel_free_arrays(W);
+  if (W.h_u32) (void)hipHostFree(W.h_u32);
+  if (W.h_i32) (void)hipHostFree(W.h_i32);
+  W.h_u32 = nullptr;
+  W.h_i32 = nullptr;
+}
+
+// (re)allocate everything sized by particle capacities and the class count
+inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
+                           const std::vector<int64_t> &cap, int max_bin) {
+  parallel_free_arrays(W);
+  PW &d = W.d;
+  d.max_bin = max_bin;
+  d.K_stride = max_bin * d.tab_classes;
+  d.P_stride = (int64_t)d.tab_classes * d.tab_classes * max_bin * max_bin;
+  const int64_t tc = A.total_cap;
+  PCHECK(pmalloc(&d.key, tc));
+  PCHECK(pmalloc(&d.status, tc));
+  PCHECK(pmalloc(&d.claim, tc));
+  PCHECK(pmalloc(&d.terms, tc));
+  PCHECK(pmalloc(&d.perm, tc));
+  // tiles over each box's capacity
+  std::vector<int> tb, ts;
+  std::vector<int64_t> hoff(n_box);
+  W.tiles_in_box.assign(n_box, 0);
+  int64_t H = 0;
+  for (int b = 0; b < n_box; ++b) {
+    int nt = (int)((cap[b] + TILE - 1) / TILE);
+    if (nt < 1) nt = 1;
+    W.tiles_in_box[b] = nt;
+    for (int t = 0; t < nt; ++t) {
+      tb.push_back(b);
+      ts.push_back(t * TILE);
+    }
+    hoff[b] = H;
+    H += (int64_t)d.K_stride * nt;
+  }
+  W.n_tiles = (int)tb.size();
+  W.H = H;
+  PCHECK(pmalloc(&d.tile_box, tb.size()));
+  PCHECK(pmalloc(&d.tile_start, ts.size()));
+  PCHECK(pmalloc(&d.tiles_in_box, n_box));
+  PCHECK(pmalloc(&d.hist_off, n_box));
+  PCHECK(hipMemcpy(d.tile_box, tb.data(), tb.size() * sizeof(int), hipMemcpyHostToDevice));
+  PCHECK(hipMemcpy(d.tile_start, ts.data(), ts.size() * sizeof(int), hipMemcpyHostToDevice));
+  PCHECK(hipMemcpy(d.tiles_in_box, W.tiles_in_box.data(), n_box * sizeof(int), hipMemcpyHostToDevice));
+  PCHECK(hipMemcpy(d.hist_off, hoff.data(), n_box * sizeof(int64_t), hipMemcpyHostToDevice));
+  PCHECK(pmalloc(&d.hist, H));
+  PCHECK(pmalloc(&d.hist_scan, H));
+  PCHECK(pmalloc(&d.lstart, (size_t)n_box * d.K_stride));
+  PCHECK(pmalloc(&d.lcnt, (size_t)n_box * d.K_stride));
+  PCHECK(pmalloc(&d.live, (size_t)n_box * d.K_stride));
+  PCHECK(hipMemset(d.live, 0, sizeof(int) * (size_t)n_box * d.K_stride));
+  W.NP = (int64_t)n_box * d.P_stride;
+  PCHECK(pmalloc(&d.pair_n, W.NP + 1));
+  PCHECK(pmalloc(&d.pair_exc, W.NP + 1));
+  PCHECK(hipMemset(d.pair_n, 0, sizeof(uint32_t) * (W.NP + 1)));
+  const int64_t T = W.T_cap;
+  PCHECK(pmalloc(&d.acc, T + 1));
+  PCHECK(pmalloc(&d.acc_scan, T + 1));
+  PCHECK(pmalloc(&d.tp1, T));
+  PCHECK(pmalloc(&d.tp2, T));
+  PCHECK(pmalloc(&d.tid, T));
+  PCHECK(pmalloc(&d.pend, T));
+  PCHECK(pmalloc(&d.ev_t, T));
+  PCHECK(pmalloc(&d.ev_p1, T));
+  PCHECK(pmalloc(&d.ev_p2, T));
+  PCHECK(pmalloc(&d.ev_box, T));
+  PCHECK(pmalloc(&d.ev_out, T));
+  int **fl[] = {&d.f_app, &d.f_id, &d.f_info, &d.f_hole, &d.f_lose,
+                &d.s_app, &d.s_id, &d.s_info, &d.s_hole, &d.s_lose};
+  for (int **f : fl) PCHECK(pmalloc(f, T + 1));
+  PCHECK(pmalloc(&d.box_first, n_box));
+  int64_t total_cap = 0;
+  for (int b = 0; b < n_box; ++b) total_cap += cap[b];
+  W.hole_cap = total_cap + 1024;
+  PCHECK(pmalloc(&d.holes, W.hole_cap));
+  PCHECK(pmalloc(&d.holes_sorted, W.hole_cap));
+  PCHECK(pmalloc(&d.tail_live, W.hole_cap));
+  // scan scratch: the largest of the histogram, pair and test scans
+  size_t b1 = 0, b2 = 0, b3 = 0;
+  PCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, d.hist, d.hist_scan, (int)std::max<int64_t>(H, 1)));
+  PCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, d.pair_n, d.pair_exc, (int)(W.NP + 1)));
+  PCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, d.acc, d.acc_scan, (int)(T + 1)));
+  W.scan_tmp_bytes = std::max(b1, std::max(b2, b3));
+  PCHECK(hipMalloc(&W.scan_tmp, W.scan_tmp_bytes));
+  PCHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, W.sort_tmp_bytes, d.holes, d.holes_sorted,
+                                           (int)W.hole_cap));
+  PCHECK(hipMalloc(&W.sort_tmp, W.sort_tmp_bytes));
+  hipLaunchKernelGGL(k_fill_u32, dim3((tc + 255) / 256), dim3(256), 0, 0, d.claim, tc, UNCLAIMED);
+  hipLaunchKernelGGL(k_fill_u8, dim3((tc + 255) / 256), dim3(256), 0, 0, d.status, tc, ST_LIVE);
+  PCHECK(hipGetLastError());
+  PCHECK(hipDeviceSynchronize());
+  return 0;
+}
+
+inline int parallel_classes(ParallelWork &W, int n_box, int classes) {
+  (void)n_box;
+  W.d.tab_classes = classes;
+  // arrays are re-laid out at the next parallel_layout call; mark stale
+  W.d.K_stride = W.d.max_bin * classes;
+  W.d.P_stride = (int64_t)classes * classes * W.d.max_bin * W.d.max_bin;
+  g_parallel_error = "";
+  return 0;
+}
+
+// the whole parallel step; the host reads back the test and event counts
+inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n_box,
+                         const std::vector<int64_t> &cap, int max_bin, hipStream_t st, int type,
+                         double del_t, uint64_t seed, uint32_t step, int precision, double band,
+                         int validation) {
+  PW &d = W.d;
+  if (d.key == nullptr || d.K_stride != max_bin * d.tab_classes || W.NP != (int64_t)n_box * d.P_stride) {
+    int e = parallel_layout(W, A, n_box, cap, max_bin);
+    if (e) return e;
+    hipLaunchKernelGGL(k_par_invalidate, dim3((n_box + 63) / 64), dim3(64), 0, st, d_box, n_box);
+  }
+  const int want_terms = (precision != 0) ? 1 : 0;
+  const dim3 gb((n_box + 63) / 64), bb(64);
+  const dim3 gt(W.n_tiles), bt(TILE_THREADS);
+  hipLaunchKernelGGL(k_step_begin, gb, bb, 0, st, d_box, n_box);
+  // 1. keys for boxes with a valid grid but stale keys, radius extremes for
+  // unsorted boxes, the remake test, and keys again after a new grid
+  hipLaunchKernelGGL(k_par_zero_live, dim3(1, n_box), dim3(256), 0, st, d_box, d, n_box, 0);
+  hipLaunchKernelGGL(k_par_rekey, gt, bt, 0, st, d_box, A, d, type, want_terms, 0);
+  hipLaunchKernelGGL(k_par_finish_rekey, gb, bb, 0, st, d_box, n_box, type, want_terms, 0);
+  hipLaunchKernelGGL(k_par_reset_minmax, gb, bb, 0, st, d_box, n_box);
+  hipLaunchKernelGGL(k_par_minmax, gt, bt, 0, st, d_box, A, d);
+  hipLaunchKernelGGL(k_par_remake, gb, bb, 0, st, d_box, A, d, n_box);
+  hipLaunchKernelGGL(k_par_zero_live, dim3(1, n_box), dim3(256), 0, st, d_box, d, n_box, 1);
+  hipLaunchKernelGGL(k_par_rekey, gt, bt, 0, st, d_box, A, d, type, 0, 1);
+  hipLaunchKernelGGL(k_par_finish_rekey, gb, bb, 0, st, d_box, n_box, type, 0, 1);
+  // 2. majorant and pair tables (shared with the exact mode)
+  {
+    const int mb = max_bin;
+    hipLaunchKernelGGL(k_kbounds, dim3((mb * mb + 255) / 256, n_box), dim3(256), 0, st, d_box, A, type);
+    hipLaunchKernelGGL(k_mark_bounds, gb, bb, 0, st, d_box, n_box, type);
+    hipLaunchKernelGGL(k_pair_tables,
+                       dim3((d.tab_classes * d.tab_classes * mb * mb + 255) / 256, n_box), dim3(256),
+                       0, st, d_box, A);
+    hipLaunchKernelGGL(k_mark_tables, gb, bb, 0, st, d_box, n_box);
+  }
+  // 3. stable counting sort of the keys
+  PLAUNCH(W, "k_par_hist", k_par_hist, gt, bt, d.K_stride * sizeof(int), st, d_box, d);
+  PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.hist, d.hist_scan,
+                                          (int)W.H, st));
+  hipLaunchKernelGGL(k_par_lists, dim3((d.K_stride + 255) / 256, n_box), dim3(256), 0, st, d_box, d,
+                     n_box);
+  int key_bits = 1;
+  while ((1 << key_bits) <= d.K_stride) key_bits++;
+  PLAUNCH(W, "k_par_scatter", k_par_scatter, gt, bt, 0, st, d_box, d, key_bits);
+  // 4. Poisson test counts per bin pair, laid out in reference order
+  PLAUNCH(W, "k_par_pairs", k_par_pairs, dim3((unsigned)((d.P_stride + 255) / 256), n_box),
+          dim3(256), 0, st, d_box, A, d, del_t, seed, step);
+  PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.pair_n, d.pair_exc,
+                                          (int)(W.NP + 1), st));
+  hipLaunchKernelGGL(k_step_samp, gb, bb, 0, st, d_box, d, n_box);
+  PCHECK(hipMemcpyAsync(W.h_u32, d.pair_exc + W.NP, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  PCHECK(hipStreamSynchronize(st));
+  const uint32_t T = W.h_u32[0];
+  hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((A.total_cap + 255) / 256)), dim3(256), 0, st,
+                     d.status, A.total_cap, ST_LIVE);
+  // 5. tests, claims and merges, in batches of T_cap tests
+  int64_t n_holes = 0;
+  for (uint32_t t0 = 0; t0 < T; t0 += (uint32_t)W.T_cap) {
+    int n = (int)std::min<int64_t>((int64_t)T - t0, W.T_cap);
+    const uint32_t *pend = nullptr;
+    for (int round = 0; n > 0; ++round) {
+      if (round >= MAX_ROUNDS) break;
+      PLAUNCH(W, "k_par_tests", k_par_tests, dim3((n + 255) / 256), dim3(256), 0, st, d_box, A, d,
+              n_box, type, seed, step, round, t0, n, pend, precision, band, validation);
+      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.acc + n, 1, 0);
+      PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.acc, d.acc_scan, n + 1, st));
+      PCHECK(hipMemcpyAsync(W.h_i32, d.acc_scan + n, sizeof(int), hipMemcpyDeviceToHost, st));
+      PCHECK(hipStreamSynchronize(st));
+      const int E = W.h_i32[0];
+      if (E == 0) break;
+      hipLaunchKernelGGL(k_par_gather, dim3((n + 255) / 256), dim3(256), 0, st, d, n);
+      hipLaunchKernelGGL(k_fill_int, dim3((n_box + 255) / 256), dim3(256), 0, st, d.box_first,
+                         (int64_t)n_box, -1);
+      hipLaunchKernelGGL(k_par_resolve, dim3((E + 255) / 256), dim3(256), 0, st, d_box, A, d, n_box,
+                         E, seed, step);
+      hipLaunchKernelGGL(k_par_box_first, dim3((E + 255) / 256), dim3(256), 0, st, d, E);
+      int *fs[5] = {d.f_app, d.f_id, d.f_info, d.f_hole, d.f_lose};
+      int *ss[5] = {d.s_app, d.s_id, d.s_info, d.s_hole, d.s_lose};
+      for (int q = 0; q < 5; ++q) {
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, fs[q] + E, 1, 0);
+        PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, fs[q], ss[q], E + 1, st));
+      }
+      PLAUNCH(W, "k_par_apply", k_par_apply, dim3((E + 255) / 256), dim3(256), 0, st, d_box, A, d,
+              n_box, E, seed, step, type, want_terms, n_holes);
+      hipLaunchKernelGGL(k_par_box_totals, gb, bb, 0, st, d_box, d, n_box, E);
+      hipLaunchKernelGGL(k_par_losers, dim3((E + 255) / 256), dim3(256), 0, st, d, E);
+      PCHECK(hipMemcpyAsync(W.h_i32 + 1, d.s_lose + E, sizeof(int), hipMemcpyDeviceToHost, st));
+      PCHECK(hipMemcpyAsync(W.h_i32 + 2, d.s_hole + E, sizeof(int), hipMemcpyDeviceToHost, st));
+      PCHECK(hipStreamSynchronize(st));
+      n_holes += W.h_i32[2];
+      n = W.h_i32[1];
+      pend = d.pend;
+      if (round + 1 >= MAX_ROUNDS) W.n_dropped += n;
+      if (n > 0) {
+        // pending tests must not alias the array the next round reads
+        PCHECK(hipMemcpyAsync(d.tid, d.pend, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+        pend = d.tid;
+        // k_par_tests overwrites tid[i] with the same value it reads
+      }
+    }
+  }
+  // 6. compaction
+  if (n_holes > 0) {
+    PCHECK(hipcub::DeviceRadixSort::SortKeys(W.sort_tmp, W.sort_tmp_bytes, d.holes, d.holes_sorted,
+                                             (int)n_holes, 0, 64, st));
+    PLAUNCH(W, "k_par_compact", k_par_compact, dim3(n_box), dim3(256), 0, st, d_box, A, d,
+            (int)n_holes);
+  }
+  hipLaunchKernelGGL(k_par_end, gb, bb, 0, st, d_box, n_box);
+  PCHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace pmc

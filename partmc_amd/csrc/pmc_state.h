@@ -28,6 +28,12 @@ struct BoxDesc {
   int n_bin;
   int pool_cur, pool_top;
   int valid_sort, bounds_valid, tables_valid, regridded;
+  int xlists_valid;          // exact-mode lists and forward maps are current
+  int pkeys_valid;           // parallel-mode keys and live counts are current
+  int rekey;                 // parallel mode: recompute every key this step
+  int terms_kernel;          // kernel type the FP32 terms were made for (0 = none)
+  int64_t n_accel_pairs;     // parallel mode: pairs the per-particle path would take
+  unsigned long long rmin_bits, rmax_bits;  // radius extremes (ordered bits)
   int bounds_kernel;         // kernel type the cached bounds were made for
   int grid_req_n_bin;        // explicit grid request (-1 = none)
   double grid_req_min, grid_req_max;
@@ -42,6 +48,7 @@ struct BoxDesc {
 struct Arrays {
   int64_t total_cap;
   int n_spec, max_bin, max_keys;
+  int tab_classes;           // class dimension of the pair tables
   double d_min, d_max;       // min/max species density (Brownian bounds)
   double *density;           // [n_spec]
   double *vol;               // [n_spec][total_cap]
@@ -59,7 +66,7 @@ struct Arrays {
   double *edges;             // [n_box][max_bin + 1]
   double *centers, *widths;  // [n_box][max_bin]
   double *kmin, *kmax;       // [n_box][max_bin * max_bin]
-  double *kw;                // [n_box][MAX_CLASS^2 * max_bin^2] weighted k_max
+  double *kw;                // [n_box][tab_classes^2 * max_bin^2] weighted k_max
   int8_t *cc;                // same layout, destination class
   int *sort_scratch;         // [n_box][SORT_THREADS][max_keys]
   // removal log
