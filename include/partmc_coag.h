@@ -142,6 +142,9 @@ int pmc_gpu_get_counters(pmc_gpu *h, int64_t *n_samp, int64_t *n_coag,
                          int64_t *n_warn, int64_t *n_recheck,
                          int64_t *n_flip, double *max_rel_err);
 int pmc_gpu_reset_counters(pmc_gpu *h);
+/* Largest |p32 - p64| of the acceptance probability p = k/k_max seen in
+ * validation mode, per box (the quantity the recheck band must exceed). */
+int pmc_gpu_get_error_stats(pmc_gpu *h, double *max_p_err);
 /* Particle-steps per box since the last reset: the sum over steps of the
  * box's particle count at the start of each step. */
 int pmc_gpu_get_part_steps(pmc_gpu *h, int64_t *part_steps);

@@ -40,6 +40,7 @@ ABI = {
     "pmc_gpu_get_next_id": (_i, [_p, _i, _p]),
     "pmc_gpu_reserve": (_i, [_p, _p]),
     "pmc_gpu_get_part_steps": (_i, [_p, _p]),
+    "pmc_gpu_get_error_stats": (_i, [_p, _p]),
     "pmc_gpu_set_timing": (_i, [_p, _i]),
     "pmc_gpu_get_kernel_time": (_i, [_p, ctypes.c_char_p, _p, _p]),
     "pmc_gpu_upload": (_i, [_p, _i, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
@@ -221,6 +222,11 @@ class CoagGPU:
 
     def reset_counters(self):
         _check(load().pmc_gpu_reset_counters(self._h))
+
+    def error_stats(self):
+        out = np.zeros(self.n_box)
+        _check(load().pmc_gpu_get_error_stats(self._h, _ptr(out)))
+        return {"max_p_err": out}
 
     def part_steps(self):
         out = np.zeros(self.n_box, dtype=np.int64)

@@ -746,10 +746,21 @@ int pmc_gpu_reset_counters(pmc_gpu *h) {
   for (auto &B : hb) {
     B.n_samp = B.n_coag = B.n_warn = B.n_recheck = B.n_flip = 0;
     B.max_rel_err = 0.0;
+    B.max_p_err = 0.0;
     B.n_rounds = B.n_dropped = 0;
     B.part_steps = 0;
   }
   HIPCHECK(hipMemcpy(h->d_box, hb.data(), sizeof(BoxDesc) * h->n_box, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int pmc_gpu_get_error_stats(pmc_gpu *h, double *max_p_err) {
+  if (!h || !max_p_err) return fail(ERR_ARG, "get_error_stats: bad arguments");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  std::vector<BoxDesc> hb(h->n_box);
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
+  for (int b = 0; b < h->n_box; ++b) max_p_err[b] = hb[b].max_p_err;
   return 0;
 }
 

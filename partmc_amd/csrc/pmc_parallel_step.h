@@ -147,7 +147,12 @@ k_par_rekey(BoxDesc *boxes, Arrays A, PW W, int type, int want_terms, int phase)
         atomicExch(&B->error, 708651144);
         continue;
       }
-      int k = par_key(b, A.klass[p], nb);
+      int cls = A.klass[p];
+      if (cls < 1 || cls > B->w.n_class) {
+        atomicExch(&B->error, 233133947);  // weight class out of range
+        continue;
+      }
+      int k = par_key(b, cls, nb);
       W.key[p] = k;
       atomicAdd(&live[k], 1);
     }
@@ -531,7 +536,10 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
       if (validation && (accept != (u < k64 * accept_factor)))
         atomicAdd((unsigned long long *)&B->n_flip, 1ull);
     }
-    if (validation && k64 > 0.0) atomic_max_double(&B->max_rel_err, fabs((double)k32 - k64) / k64);
+    if (validation && k64 > 0.0) {
+      atomic_max_double(&B->max_rel_err, fabs((double)k32 - k64) / k64);
+      atomic_max_double(&B->max_p_err, fabs(p32 - k64 * accept_factor));
+    }
   }
   if (!(pr <= 1.0)) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
   if (accept) {

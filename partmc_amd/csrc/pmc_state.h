@@ -21,6 +21,7 @@ struct BoxDesc {
   // counters accumulated since the last reset
   int64_t n_samp, n_coag, n_warn, n_recheck, n_flip;
   double max_rel_err;
+  double max_p_err;
   // counters of the last step (exact mode)
   int64_t step_samp, step_coag;
   int64_t part_steps;        // sum over steps of n_part at step start
