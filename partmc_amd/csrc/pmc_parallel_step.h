@@ -51,6 +51,7 @@ struct PW {  // device view of the parallel work arrays
   int *key;         // [total_cap] list key, -1 for no key
   uint8_t *status;  // [total_cap]
   uint32_t *claim;  // [total_cap]
+  uint32_t *dead_t; // [total_cap] test index of the winner that consumed it
   float4 *terms;    // [total_cap]
   int *perm;        // [total_cap] global slots sorted by (box, key)
   int *tile_box, *tile_start, *tiles_in_box;
@@ -62,7 +63,7 @@ struct PW {  // device view of the parallel work arrays
   uint32_t *tid, *pend;
   uint32_t *ev_t;
   int *ev_p1, *ev_p2, *ev_box, *ev_out;
-  int *f_app, *f_id, *f_info, *f_hole, *f_lose;
+  int *f_app, *f_id, *f_info, *f_hole, *f_lose, *f_pend, *s_pend;
   int *s_app, *s_id, *s_info, *s_hole, *s_lose;
   int *box_first;
   uint64_t *holes, *holes_sorted;  // (box << 32 | slot)
@@ -441,6 +442,8 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
   const uint32_t t = pend ? pend[i] : t0 + (uint32_t)i;
   W.acc[i] = 0;
   W.tid[i] = t;
+  W.tp1[i] = -1;
+  W.tp2[i] = -1;
   const int64_t NP = (int64_t)n_box * W.P_stride;
   const int64_t k = find_pair(W.pair_exc, NP, t);
   const int box = (int)(k / W.P_stride);
@@ -506,6 +509,8 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
       return;
     }
   }
+  W.tp1[i] = p1;
+  W.tp2[i] = p2;
   const Weights &w = B->w;
   const Env env = B->env;
   double u = ts.uniform();
@@ -544,8 +549,6 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
   if (!(pr <= 1.0)) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
   if (accept) {
     W.acc[i] = 1;
-    W.tp1[i] = p1;
-    W.tp2[i] = p2;
     atomicMin(&W.claim[p1], t);
     atomicMin(&W.claim[p2], t);
   }
@@ -806,6 +809,7 @@ __global__ void k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, ui
   // removals: live counts and holes
   int64_t hrank = hole_base + W.s_hole[e];
   if (o & O_RM1) {
+    W.dead_t[p1] = t;
     atomicSub(&live[W.key[p1]], 1);
     if (!(o & O_SLOT1)) {
       W.status[p1] = ST_HOLE;
@@ -813,6 +817,7 @@ __global__ void k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, ui
     }
   }
   if (o & O_RM2) {
+    W.dead_t[p2] = t;
     atomicSub(&live[W.key[p2]], 1);
     if (!(o & O_SLOT2)) {
       W.status[p2] = ST_HOLE;
@@ -870,12 +875,38 @@ __global__ void k_par_box_totals(BoxDesc *boxes, PW W, int n_box, int E) {
   B->step_coag += wins;
 }
 
-__global__ void k_par_losers(PW W, int E) {
+__global__ void k_par_release(PW W, int E) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
-  if (W.f_lose[e]) W.pend[W.s_lose[e]] = W.ev_t[e];
   W.claim[W.ev_p1[e]] = UNCLAIMED;
   W.claim[W.ev_p2[e]] = UNCLAIMED;
+}
+
+// Tests of the round that must be drawn again: accepted tests that lost a
+// claim, and any other non-winning test that drew a particle an earlier
+// winner (smaller sequential index) consumed -- in the sequential chain
+// that test would have drawn from the particles still present.
+__device__ inline bool consumed_before(const PW &W, int p, uint32_t t) {
+  return p >= 0 && W.status[p] != ST_LIVE && W.dead_t[p] < t;
+}
+
+__global__ void k_par_pending(PW W, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t t = W.tid[i];
+  int f = 0;
+  if (W.acc[i]) {
+    f = (W.ev_out[W.acc_scan[i]] & O_WIN) ? 0 : 1;
+  } else if (W.tp1[i] >= 0) {
+    f = (consumed_before(W, W.tp1[i], t) || consumed_before(W, W.tp2[i], t)) ? 1 : 0;
+  }
+  W.f_pend[i] = f;
+}
+
+__global__ void k_par_gather_pending(PW W, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (W.f_pend[i]) W.pend[W.s_pend[i]] = W.tid[i];
 }
 
 __global__ void k_fill_int(int *p, int64_t n, int v) {
@@ -1006,7 +1037,8 @@ inline void parallel_free_arrays(ParallelWork &W) {
                   d.tiles_in_box, d.hist_off, d.hist, d.hist_scan, d.lstart, d.lcnt, d.live,
                   d.pair_n, d.pair_exc, d.acc, d.acc_scan, d.tp1, d.tp2, d.tid, d.pend, d.ev_t,
                   d.ev_p1, d.ev_p2, d.ev_box, d.ev_out, d.f_app, d.f_id, d.f_info, d.f_hole,
-                  d.f_lose, d.s_app, d.s_id, d.s_info, d.s_hole, d.s_lose, d.box_first,
+                  d.f_lose, d.f_pend, d.s_pend, d.dead_t, d.s_app, d.s_id, d.s_info, d.s_hole,
+                  d.s_lose, d.box_first,
                   d.holes, d.holes_sorted, d.tail_live, W.scan_tmp, W.sort_tmp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -1042,6 +1074,7 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.key, tc));
   PCHECK(pmalloc(&d.status, tc));
   PCHECK(pmalloc(&d.claim, tc));
+  PCHECK(pmalloc(&d.dead_t, tc));
   PCHECK(pmalloc(&d.terms, tc));
   PCHECK(pmalloc(&d.perm, tc));
   // tiles over each box's capacity
@@ -1092,8 +1125,8 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.ev_p2, T));
   PCHECK(pmalloc(&d.ev_box, T));
   PCHECK(pmalloc(&d.ev_out, T));
-  int **fl[] = {&d.f_app, &d.f_id, &d.f_info, &d.f_hole, &d.f_lose,
-                &d.s_app, &d.s_id, &d.s_info, &d.s_hole, &d.s_lose};
+  int **fl[] = {&d.f_app, &d.f_id, &d.f_info, &d.f_hole, &d.f_lose, &d.f_pend,
+                &d.s_app, &d.s_id, &d.s_info, &d.s_hole, &d.s_lose, &d.s_pend};
   for (int **f : fl) PCHECK(pmalloc(f, T + 1));
   PCHECK(pmalloc(&d.box_first, n_box));
   int64_t total_cap = 0;
@@ -1215,8 +1248,12 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
       PLAUNCH(W, "k_par_apply", k_par_apply, dim3((E + 255) / 256), dim3(256), 0, st, d_box, A, d,
               n_box, E, seed, step, type, want_terms, n_holes);
       hipLaunchKernelGGL(k_par_box_totals, gb, bb, 0, st, d_box, d, n_box, E);
-      hipLaunchKernelGGL(k_par_losers, dim3((E + 255) / 256), dim3(256), 0, st, d, E);
-      PCHECK(hipMemcpyAsync(W.h_i32 + 1, d.s_lose + E, sizeof(int), hipMemcpyDeviceToHost, st));
+      hipLaunchKernelGGL(k_par_release, dim3((E + 255) / 256), dim3(256), 0, st, d, E);
+      hipLaunchKernelGGL(k_par_pending, dim3((n + 255) / 256), dim3(256), 0, st, d, n);
+      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.f_pend + n, 1, 0);
+      PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.f_pend, d.s_pend, n + 1, st));
+      hipLaunchKernelGGL(k_par_gather_pending, dim3((n + 255) / 256), dim3(256), 0, st, d, n);
+      PCHECK(hipMemcpyAsync(W.h_i32 + 1, d.s_pend + n, sizeof(int), hipMemcpyDeviceToHost, st));
       PCHECK(hipMemcpyAsync(W.h_i32 + 2, d.s_hole + E, sizeof(int), hipMemcpyDeviceToHost, st));
       PCHECK(hipStreamSynchronize(st));
       n_holes += W.h_i32[2];
