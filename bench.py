@@ -45,10 +45,69 @@ WORKLOADS = {
 }
 
 
-def per_test_bytes(kernel, n_spec):
-    """Algorithmic HBM bytes of one test pair evaluated from raw particle data:
-    two particles' species volumes plus two perm entries."""
-    return 2 * (8 * n_spec + 4)
+# Algorithmic HBM bytes per unit of work of each timed kernel (the minimum
+# a kernel must move): keys, perm entries, per-particle terms, species
+# volumes and the per-pair tables it reads or writes.
+def kernel_bytes(name, n_spec, precision, part_steps_per_launch, tests_per_launch,
+                 pairs_per_launch, events_per_launch):
+    if name == "k_par_hist":
+        return 4.0 * part_steps_per_launch                   # read key
+    if name == "k_par_scatter":
+        return 8.0 * part_steps_per_launch                   # read key, write perm
+    if name == "k_par_pairs":
+        return 20.0 * pairs_per_launch                       # 2 counts, k_max, n_samp
+    if name in ("k_par_tests", "k_exact_sweep"):
+        per = 8.0 + (32.0 if precision == "fp32" else 2 * 8.0 * n_spec)
+        return per * tests_per_launch                        # 2 perm + 2 particles
+    if name == "k_par_apply":
+        return 3.0 * (8.0 * n_spec + 28.0) * events_per_launch
+    if name == "k_prepare":
+        return 20.0 * part_steps_per_launch
+    return 0.0
+
+
+TIMED = ("k_prepare", "k_exact_sweep", "k_par_hist", "k_par_scatter", "k_par_pairs",
+         "k_par_tests", "k_par_apply", "k_par_compact")
+
+
+def run_gpu(pops, kernel, dt, mode, precision, steps, warmup, seed=20261202):
+    g = CoagGPU(len(pops), pops[0].density, max_bin=256)
+    g.reserve([p.n_part + p.n_part // 4 + 1024 for p in pops])
+    g.set_rng(seed, 0)
+    for i, pop in enumerate(pops):
+        g.load_population(i, pop)
+    g.set_mode(partmc_amd.MODE_EXACT if mode == "exact" else partmc_amd.MODE_PARALLEL,
+               partmc_amd.PREC_FP64 if precision == "fp64" else partmc_amd.PREC_FP32_RECHECK,
+               1e-5)
+    for _ in range(warmup):
+        g.mc_coag_async(kernel, dt)
+    g.sync()
+    g.reset_counters()
+    g.set_timing(True)
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.mc_coag_async(kernel, dt)
+    g.sync()
+    elapsed = time.perf_counter() - t0
+    c = g.counters()
+    out = {
+        "elapsed": elapsed,
+        "n_samp": int(c["n_samp"].sum()),
+        "n_coag": int(c["n_coag"].sum()),
+        "n_warn": int(c["n_warn"].sum()),
+        "n_recheck": int(c["n_recheck"].sum()),
+        "n_flip": int(c["n_flip"].sum()),
+        "part_steps": int(g.part_steps().sum()),
+        "timings": {},
+    }
+    for name in TIMED:
+        ms, n = g.kernel_time(name)
+        if n:
+            out["timings"][name] = (ms, n)
+    g.set_timing(False)
+    g.close()
+    return out
 
 
 def cpu_baseline(pops, kernel, dt, budget_s=15.0):
@@ -86,11 +145,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
-    ap.add_argument("--mode", default="exact", choices=["exact", "parallel"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--workload", default="c4", choices=sorted(WORKLOADS))
+    ap.add_argument("--mode", default="parallel", choices=["exact", "parallel"])
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32"])
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -103,48 +163,31 @@ def main():
     t_gen = time.perf_counter()
     pops = build(args.scale)
     t_gen = time.perf_counter() - t_gen
-    g = CoagGPU(len(pops), pops[0].density, max_bin=256)
-    g.reserve([2 * p.n_part + 1024 for p in pops])
-    g.set_rng(20261202, 0)
-    for i, pop in enumerate(pops):
-        g.load_population(i, pop)
-    g.set_mode(partmc_amd.MODE_EXACT if args.mode == "exact" else partmc_amd.MODE_PARALLEL,
-               partmc_amd.PREC_FP64 if args.precision == "fp64" else partmc_amd.PREC_FP32_RECHECK,
-               1e-5)
-    for _ in range(args.warmup):
-        g.mc_coag_async(kernel, dt)
-    g.sync()
-    g.reset_counters()
-    g.set_timing(True)
-    g.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        g.mc_coag_async(kernel, dt)
-    g.sync()
-    elapsed = time.perf_counter() - t0
-    c = g.counters()
-    psteps = int(g.part_steps().sum())
-    n_samp = int(c["n_samp"].sum())
-    n_coag = int(c["n_coag"].sum())
-    timings = {}
-    for name in ("k_prepare", "k_exact_sweep", "k_par_sort", "k_par_tests", "k_par_pairs"):
-        ms, n = g.kernel_time(name)
-        if n:
-            timings[name] = (ms, n)
-    dom = max(timings, key=lambda k: timings[k][0]) if timings else None
+    n_spec = pops[0].n_spec
+    r = run_gpu(pops, kernel, dt, args.mode, args.precision, args.steps, args.warmup)
+    elapsed = r["elapsed"]
+    n_samp, n_coag, psteps = r["n_samp"], r["n_coag"], r["part_steps"]
+    timings = r["timings"]
+    # the plain-FP64 comparison pass on the same populations and streams
+    r64 = None
+    if args.mode == "parallel" and args.precision == "fp32" and not args.no_fp64_compare:
+        r64 = run_gpu(pops, kernel, dt, args.mode, "fp64", args.steps, args.warmup)
     roofline = None
-    if dom:
+    if timings:
+        dom = max(timings, key=lambda k: timings[k][0])
         ms, n = timings[dom]
+        pairs = len(pops) * 256 * 256
+        traffic = kernel_bytes(dom, n_spec, args.precision, psteps / args.steps,
+                               n_samp / max(n, 1), pairs, n_coag / max(n, 1))
+        if dom in ("k_par_tests", "k_par_apply"):
+            traffic = kernel_bytes(dom, n_spec, args.precision, psteps / args.steps,
+                                   n_samp / n, pairs, n_coag / n)
         avg_s = ms / n / 1e3
-        n_spec = pops[0].n_spec
-        if dom in ("k_exact_sweep", "k_par_tests"):
-            traffic = n_samp * per_test_bytes(kernel, n_spec) / n
-        else:
-            traffic = psteps / n * 12.0
         achieved = traffic / avg_s / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                    "avg_launch_ms": ms / n, "algorithmic_bytes_per_launch": traffic}
+                    "avg_launch_ms": ms / n, "launches": n,
+                    "algorithmic_bytes_per_launch": traffic}
     base = None if args.no_cpu_baseline else cpu_baseline(pops, kernel, dt, args.cpu_budget)
     total_particles = sum(p.n_part for p in pops)
     line = {
@@ -167,11 +210,20 @@ def main():
         "sim_s_per_wall_s": dt * args.steps / elapsed,
         "test_pairs": n_samp,
         "coag_events": n_coag,
-        "kmax_warnings": int(c["n_warn"].sum()),
-        "fp64_rechecks": int(c["n_recheck"].sum()),
-        "decision_flips": int(c["n_flip"].sum()),
-        "max_rel_kernel_err": float(c["max_rel_err"].max()),
+        "kmax_warnings": r["n_warn"],
+        "fp64_rechecks": r["n_recheck"],
         "kernel_ms": {k: v[0] / v[1] for k, v in timings.items()},
+        "kernel_total_ms": {k: v[0] for k, v in timings.items()},
+        "fp64_compare": None if r64 is None else {
+            "test_pairs_per_s": r64["n_samp"] / r64["elapsed"],
+            "ms_per_step": r64["elapsed"] / args.steps * 1e3,
+            "k_par_tests_ms": r64["timings"].get("k_par_tests", (0.0, 1))[0] / max(
+                r64["timings"].get("k_par_tests", (0.0, 1))[1], 1),
+            "same_decisions": r64["n_samp"] == n_samp and r64["n_coag"] == n_coag,
+            "ratio_step": (n_samp / elapsed) / max(r64["n_samp"] / r64["elapsed"], 1e-30),
+            "ratio_test_kernel": (r64["timings"].get("k_par_tests", (0.0, 1))[0] /
+                                  max(timings.get("k_par_tests", (1e-30, 1))[0], 1e-30)),
+        },
         "roofline": roofline,
         "cpu_baseline": base,
         "setup_s": {"generate": t_gen},
