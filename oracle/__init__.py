@@ -79,6 +79,7 @@ def lib():
             "orc_soln_additive_exp": (None, [_i, _p, _d, _d, _d, _d, _p, _p]),
             "orc_soln_constant_exp": (None, [_i, _p, _d, _d, _d, _p, _p]),
             "orc_histogram_1d": (None, [_i, _p, _p, _i, _p, _p, _p]),
+            "orc_sample_pmf": (None, [_u64, _i, _d, _d, _i64, _i, _p]),
             "orc_box_create": (_p, [_i, _p, _i, _i, _p, _p, _p]),
             "orc_box_destroy": (None, [_p]),
             "orc_box_set_env": (None, [_p, _d, _d, _d]),
@@ -143,6 +144,12 @@ def philox(ctr, key):
     o = np.zeros(4, dtype=np.uint32)
     lib().orc_philox4x32_10(_ptr(c), _ptr(k), _ptr(o))
     return o
+
+
+def sample_pmf(seed, which, p1, p2, n, kmax):
+    counts = np.zeros(kmax + 1, dtype=np.int64)
+    lib().orc_sample_pmf(seed, which, p1, p2, n, kmax, _ptr(counts))
+    return counts
 
 
 def bin_grid_make_log(n_bin, rmin, rmax):

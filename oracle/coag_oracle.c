@@ -1792,3 +1792,18 @@ int orc_mc_coag(orc_box *b, int type, double del_t, uint32_t step,
             }
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* batch samplers for the pmf tests (test/rand protocol)               */
+/* ------------------------------------------------------------------ */
+void orc_sample_pmf(uint64_t seed, int which, double p1, double p2, int64_t n,
+                    int kmax, int64_t *counts) {
+    for (int k = 0; k <= kmax; ++k) counts[k] = 0;
+    orc_stream s;
+    for (int64_t i = 0; i < n; ++i) {
+        orc_stream_open(&s, seed, 0, (uint32_t)(i >> 32), 0, (uint32_t)i, 1);
+        int k = (which == 0) ? orc_rand_poisson(&s, p1)
+                             : orc_rand_binomial(&s, (int)p1, p2);
+        if (k >= 0 && k <= kmax) counts[k]++;
+    }
+}

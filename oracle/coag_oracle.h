@@ -131,6 +131,11 @@ void orc_soln_constant_exp(int n_bin, const double *centers, double time,
 void orc_histogram_1d(int n_bin, const double *edges, const double *widths,
                       int n, const double *x, const double *w, double *out);
 
+/* pmf sampling: n independent variates (which 0: Poisson(p1), 1:
+ * binomial(int(p1), p2)), each from its own stream; counts[0..kmax]. */
+void orc_sample_pmf(uint64_t seed, int which, double p1, double p2, int64_t n,
+                    int kmax, int64_t *counts);
+
 /* ---- the particle box (aero_state restatement) ---- */
 typedef struct orc_box orc_box;
 
