@@ -36,6 +36,33 @@ def command(out: str = LIB) -> list:
             "-o", out] + SOURCES
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "coag_driver")
+
+
+def build_fortran(verbose: bool = False):
+    """The Fortran ISO_C_BINDING module and its driver program (amdflang),
+    linked against the library with an $ORIGIN-relative runpath."""
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        if verbose:
+            print("amdflang not found: Fortran driver not built", flush=True)
+        return None
+    import tempfile
+    with tempfile.TemporaryDirectory() as mod_dir:
+        cmd = [fc, "-O2", "-J", mod_dir, "-o", DRIVER + ".tmp",
+               os.path.join(FORTRAN, "pmc_coag_gpu.F90"), os.path.join(FORTRAN, "coag_driver.F90"),
+               "-L" + HERE, "-lpartmc_coag", "-Wl,-rpath,$ORIGIN/.."]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("amdflang failed building the Fortran driver")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     cmd = command(tmp)
@@ -46,6 +73,7 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libpartmc_coag.so")
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
 
 
