@@ -98,6 +98,7 @@ def lib():
             "orc_box_get_info": (None, [_p, _p, _p, _p]),
             "orc_box_clear_info": (None, [_p]),
             "orc_box_n_warn_kmax": (_i64, [_p]),
+            "orc_box_n_accel": (_i64, [_p]),
             "orc_box_regridded": (_i, [_p]),
             "orc_mc_coag": (_i, [_p, _i, _d, _u32, _p, _p]),
         }
@@ -327,6 +328,11 @@ class Box:
     @property
     def n_warn_kmax(self):
         return lib().orc_box_n_warn_kmax(self._h)
+
+    @property
+    def n_accel(self):
+        """Bin pairs that took the per-particle (accelerated) path."""
+        return lib().orc_box_n_accel(self._h)
 
     @property
     def regridded(self):

@@ -875,6 +875,7 @@ struct orc_box {
     int *info_action;
 
     int64_t n_warn_kmax;
+    int64_t n_accel;       /* bin pairs that took the per-particle path */
     int regridded;
 };
 
@@ -934,6 +935,7 @@ int64_t orc_box_get_next_id(const orc_box *b) { return b->next_id; }
 int orc_box_n_part(const orc_box *b) { return b->n; }
 int orc_box_n_bin(const orc_box *b) { return b->n_bin; }
 int64_t orc_box_n_warn_kmax(const orc_box *b) { return b->n_warn_kmax; }
+int64_t orc_box_n_accel(const orc_box *b) { return b->n_accel; }
 int orc_box_regridded(const orc_box *b) { return b->regridded; }
 
 static void ensure_cap(orc_box *b, int need) {
@@ -1746,6 +1748,7 @@ static int try_per_particle_coag(orc_box *b, int type, double k_max,
         *tot_n_coag += n_coag;
     }
     *succeeded = 1;
+    b->n_accel++;
     return 0;
 }
 

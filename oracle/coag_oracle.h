@@ -175,6 +175,7 @@ void orc_box_get_info(const orc_box *b, int64_t *id, int *action,
 void orc_box_clear_info(orc_box *b);
 /* counters of "kernel upper bound estimation is too tight" warnings */
 int64_t orc_box_n_warn_kmax(const orc_box *b);
+int64_t orc_box_n_accel(const orc_box *b);
 /* whether the last mc_coag call made a new grid */
 int orc_box_regridded(const orc_box *b);
 

@@ -40,6 +40,7 @@ ABI = {
     "pmc_gpu_get_next_id": (_i, [_p, _i, _p]),
     "pmc_gpu_reserve": (_i, [_p, _p]),
     "pmc_gpu_get_part_steps": (_i, [_p, _p]),
+    "pmc_gpu_get_path_stats": (_i, [_p, _p, _p, _p]),
     "pmc_gpu_get_error_stats": (_i, [_p, _p]),
     "pmc_gpu_set_timing": (_i, [_p, _i]),
     "pmc_gpu_get_kernel_time": (_i, [_p, ctypes.c_char_p, _p, _p]),
@@ -232,6 +233,16 @@ class CoagGPU:
         out = np.zeros(self.n_box, dtype=np.int64)
         _check(load().pmc_gpu_get_part_steps(self._h, _ptr(out)))
         return out
+
+    def path_stats(self):
+        """Per box: pairs on the per-particle path and dropped draws; for
+        the handle: claim rounds run and tests cut by the round limit."""
+        acc = np.zeros(self.n_box, dtype=np.int64)
+        drop = np.zeros(self.n_box, dtype=np.int64)
+        rounds = np.zeros(2, dtype=np.int64)
+        _check(load().pmc_gpu_get_path_stats(self._h, _ptr(acc), _ptr(drop), _ptr(rounds)))
+        return {"n_accel_pairs": acc, "n_dropped": drop, "n_rounds": int(rounds[0]),
+                "n_cut": int(rounds[1])}
 
     def reserve(self, capacity):
         cap = np.ascontiguousarray(capacity, dtype=np.int64)

@@ -748,8 +748,11 @@ int pmc_gpu_reset_counters(pmc_gpu *h) {
     B.max_rel_err = 0.0;
     B.max_p_err = 0.0;
     B.n_rounds = B.n_dropped = 0;
+    B.n_accel_pairs = 0;
     B.part_steps = 0;
   }
+  h->work.n_rounds = 0;
+  h->work.n_dropped = 0;
   HIPCHECK(hipMemcpy(h->d_box, hb.data(), sizeof(BoxDesc) * h->n_box, hipMemcpyHostToDevice));
   return 0;
 }
@@ -761,6 +764,23 @@ int pmc_gpu_get_error_stats(pmc_gpu *h, double *max_p_err) {
   std::vector<BoxDesc> hb(h->n_box);
   HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
   for (int b = 0; b < h->n_box; ++b) max_p_err[b] = hb[b].max_p_err;
+  return 0;
+}
+
+int pmc_gpu_get_path_stats(pmc_gpu *h, int64_t *n_accel_pairs, int64_t *n_dropped,
+                           int64_t *rounds) {
+  if (!h || !n_accel_pairs || !n_dropped || !rounds)
+    return fail(ERR_ARG, "get_path_stats: bad arguments");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  std::vector<BoxDesc> hb(h->n_box);
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
+  for (int b = 0; b < h->n_box; ++b) {
+    n_accel_pairs[b] = hb[b].n_accel_pairs;
+    n_dropped[b] = hb[b].n_dropped;
+  }
+  rounds[0] = h->work.n_rounds;
+  rounds[1] = h->work.n_dropped;
   return 0;
 }
 

@@ -1017,6 +1017,7 @@ k_exact_sweep(BoxDesc *boxes, Arrays A, int type, double del_t, uint64_t seed,
                 if (!(max_new / min_tv > 1.5)) {
                   accel = true;
                   if (lane == 0) {
+                    B->n_accel_pairs += 1;
                     for (int e = E.n_entry(bt, ct); e >= 1 && !err; --e) {
                       int dom = E.entry(bt, ct, e);
                       Rec target, source;

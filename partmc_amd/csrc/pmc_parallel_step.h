@@ -68,7 +68,31 @@ struct PW {  // device view of the parallel work arrays
   int *box_first;
   uint64_t *holes, *holes_sorted;  // (box << 32 | slot)
   int *tail_live;
+  // accelerated per-particle path (try_per_particle_coag): one test slot per
+  // target particle, one record per slot of a batch
+  uint8_t *pair_acc;           // [NP] 0 per-set; 1 targets in (b1, c1); 2 targets in (b2, c2)
+  uint32_t *pair_accn;         // [NP + 1] target slots of accelerated pairs
+  uint32_t *acc_exc;           // [NP + 1] their exclusive scan
+  int *trec;                   // [T_cap] record of a test slot, -1 for per-set tests
+  int *tns;                    // [T_cap] n_samp (+ ACC_WARN) of an attempt without effect
+  int *ev_rec;                 // [T_cap] record of an event, -1 for per-set events
+  int acc_cap;                 // records per batch
+  double *r_vol;               // [n_spec][acc_cap] the virtual source particle
+  int *r_nprim, *r_ncomp;      // [acc_cap]
+  int *r_csrc;                 // [MAX_COMP][acc_cap]
+  double *r_ctime;             // [MAX_COMP][acc_cap]
+  double *r_least, *r_greatest;
+  int *r_nsamp, *r_ncoag, *r_ncopies, *r_nrm, *r_head, *r_warn;  // [acc_cap]
+  int *rm_buf;                 // chunks: [0] next chunk or -1, [1..RM_CHUNK) removed slots
+  int *rm_top;                 // [1] chunk allocator of the round
+  int rm_cap;
+  uint32_t *misc;              // [4] batch bounds from k_batch_end
 };
+
+constexpr int ACC_CAP = 1 << 19;
+constexpr int RM_CHUNK = 16;
+constexpr int RM_CAP = 1 << 24;
+constexpr int ACC_WARN = 1 << 30;
 
 struct ParallelWork {
   int n_box = 0;
@@ -88,7 +112,8 @@ struct ParallelWork {
   void (*tbegin)(void *, const char *, hipEvent_t *) = nullptr;
   void (*tend)(void *, const char *, hipEvent_t) = nullptr;
   void *tctx = nullptr;
-  int64_t n_dropped = 0;
+  int64_t n_dropped = 0;  // tests still pending when the round limit was reached
+  int64_t n_rounds = 0;   // claim rounds run
 };
 
 #define PLAUNCH(W, name, ...)                                   \
@@ -385,6 +410,98 @@ __device__ inline void decode_pair(int64_t q, int tc, int mb, int *c1, int *c2, 
   *b2 = rem % mb + 1;
 }
 
+__device__ inline void load_rec(const Arrays &A, int64_t p, Rec &r) {
+  for (int s = 0; s < A.n_spec; ++s) r.vol[s] = A.vol[(int64_t)s * A.total_cap + p];
+  r.id = A.id[p];
+  r.group = A.group[p];
+  r.klass = A.klass[p];
+  r.nprim = A.nprim[p];
+  r.ncomp = A.ncomp[p];
+  for (int c = 0; c < MAX_COMP; ++c) {
+    r.csrc[c] = A.csrc[(int64_t)c * A.total_cap + p];
+    r.ctime[c] = A.ctime[(int64_t)c * A.total_cap + p];
+  }
+  r.least = A.least[p];
+  r.greatest = A.greatest[p];
+}
+
+// aero_particle_coagulate for the parallel path (same rules as rec_coagulate)
+__device__ inline int par_rec_coagulate(int n_spec, Stream &s, const Rec &p1, const Rec &p2,
+                                        Rec &o) {
+  for (int k = 0; k < n_spec; ++k) o.vol[k] = p1.vol[k] + p2.vol[k];
+  o.group = 0;
+  o.klass = 0;
+  o.id = 0;
+  int n1 = p1.ncomp, n2 = p2.ncomp;
+  if (p1.nprim < n1) return 465791384;
+  if (p2.nprim < n2) return 465791385;
+  for (int c = 0; c < MAX_COMP; ++c) {
+    o.csrc[c] = 0;
+    o.ctime[c] = 0.0;
+  }
+  if (n1 + n2 > MAX_COMP) {
+    int n1_new = prob_round(s, (double)p1.nprim / (double)(p1.nprim + p2.nprim) * (double)MAX_COMP);
+    int n2_new = MAX_COMP - n1_new;
+    int sample[MAX_COMP];
+    sample_without_replacement(s, n1_new, n1, sample);
+    for (int i = 0; i < n1_new; ++i) {
+      o.csrc[i] = p1.csrc[sample[i] - 1];
+      o.ctime[i] = p1.ctime[sample[i] - 1];
+    }
+    sample_without_replacement(s, n2_new, n2, sample);
+    for (int i = 0; i < n2_new; ++i) {
+      o.csrc[i + n1_new] = p2.csrc[sample[i] - 1];
+      o.ctime[i + n1_new] = p2.ctime[sample[i] - 1];
+    }
+    o.ncomp = MAX_COMP;
+  } else {
+    for (int i = 0; i < n1; ++i) {
+      o.csrc[i] = p1.csrc[i];
+      o.ctime[i] = p1.ctime[i];
+    }
+    for (int i = 0; i < n2; ++i) {
+      o.csrc[n1 + i] = p2.csrc[i];
+      o.ctime[n1 + i] = p2.ctime[i];
+    }
+    o.ncomp = n1 + n2;
+  }
+  o.least = fmin(p1.least, p2.least);
+  o.greatest = fmax(p1.greatest, p2.greatest);
+  o.nprim = p1.nprim + p2.nprim;
+  return 0;
+}
+
+// try_per_particle_coag's tests (src/coagulation.F90:150-215): monotone
+// decreasing weights, target weight strictly below the source weight with
+// the destination class the target's, at least one expected source per
+// target, and at most 1.5x growth.  Returns 0 (per-set path), 1 (targets in
+// (b1, c1)) or 2 (targets in (b2, c2)).
+__device__ inline int accel_kind(const Weights &w, const double *edges, int b1, int b2, int c1,
+                                 int c2, int cc, double k_max, double del_t, const int *cnt, int nb,
+                                 double *n_spt_out) {
+  for (int i = 0; i < w.n_group * w.n_class; ++i)
+    if (w.type[i] == WEIGHT_POWER && w.expo[i] > 0.0) return 0;
+  double a1 = nc_at_radius(w, c1, edges[b1 - 1]), a2 = nc_at_radius(w, c1, edges[b1]);
+  double d1 = nc_at_radius(w, c2, edges[b2 - 1]), d2 = nc_at_radius(w, c2, edges[b2]);
+  double nc1_min = fmin(a1, a2), nc1_max = fmax(a1, a2);
+  double nc2_min = fmin(d1, d2), nc2_max = fmax(d1, d2);
+  int kind = 0, bt = 0, bs = 0, cs = 0;
+  if (nc1_max < nc2_min && cc == c1) {
+    kind = 1; bt = b1; bs = b2; cs = c2;
+  }
+  if (nc2_max < nc1_min && cc == c2) {
+    kind = 2; bt = b2; bs = b1; cs = c1;
+  }
+  if (!kind) return 0;
+  double n_spt = k_max * del_t * (double)cnt[par_key(bs, cs, nb)];
+  if (n_spt < 1.0) return 0;
+  double min_tv = rad2vol(edges[bt - 1]);
+  double max_sv = rad2vol(edges[bs]);
+  if ((min_tv + max_sv * n_spt) / min_tv > 1.5) return 0;
+  *n_spt_out = n_spt;
+  return kind;
+}
+
 // n_samp of every (box, c1, c2, b1, b2), compute_n_samp
 // src/coagulation.F90:648-702 on the pair's own Poisson stream
 __global__ void k_par_pairs(BoxDesc *boxes, Arrays A, PW W, double del_t, uint64_t seed,
@@ -394,6 +511,7 @@ __global__ void k_par_pairs(BoxDesc *boxes, Arrays A, PW W, double del_t, uint64
   if (q >= W.P_stride) return;
   BoxDesc *B = &boxes[box];
   uint32_t ns = 0;
+  int acc = 0;
   if (!B->error) {
     int c1, c2, b1, b2;
     decode_pair(q, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
@@ -402,18 +520,34 @@ __global__ void k_par_pairs(BoxDesc *boxes, Arrays A, PW W, double del_t, uint64
       const int *cnt = W.lcnt + (int64_t)box * W.K_stride;
       int n1 = cnt[par_key(b1, c1, nb)], n2 = cnt[par_key(b2, c2, nb)];
       if (n1 > 0 && n2 > 0) {
-        bool same = (b1 == b2) && (c1 == c2);
-        double n_possible = same ? (double)n1 * ((double)n2 - 1.0) / 2.0 : (double)n1 * (double)n2;
-        double k_max = A.kw[table_index(A, box, nb, nc, b1, b2, c1, c2)];
-        double mean = k_max * del_t * n_possible;
-        Stream ps;
-        ps.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), 0, RNG_POISSON);
-        ns = (uint32_t)rand_poisson(ps, mean);
-        if (ns) atomicAdd((unsigned long long *)&B->n_samp, (unsigned long long)ns);
+        const int64_t ti = table_index(A, box, nb, nc, b1, b2, c1, c2);
+        double k_max = A.kw[ti];
+        const double *edges = A.edges + (int64_t)box * (A.max_bin + 1);
+        double n_spt;
+        acc = accel_kind(B->w, edges, b1, b2, c1, c2, A.cc[ti], k_max, del_t, cnt, nb, &n_spt);
+        if (acc) {
+          // one test slot per target particle (the targets' loop,
+          // src/coagulation.F90:220-237)
+          ns = (uint32_t)(acc == 1 ? n1 : n2);
+          atomicAdd((unsigned long long *)&B->n_accel_pairs, 1ull);
+        } else {
+          bool same = (b1 == b2) && (c1 == c2);
+          double n_possible = same ? (double)n1 * ((double)n2 - 1.0) / 2.0 : (double)n1 * (double)n2;
+          double mean = k_max * del_t * n_possible;
+          Stream ps;
+          ps.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), 0, RNG_POISSON);
+          ns = (uint32_t)rand_poisson(ps, mean);
+          if (ns) {
+            atomicAdd((unsigned long long *)&B->n_samp, (unsigned long long)ns);
+            atomicAdd((unsigned long long *)&B->step_samp, (unsigned long long)ns);
+          }
+        }
       }
     }
   }
   W.pair_n[(int64_t)box * W.P_stride + q] = ns;
+  W.pair_acc[(int64_t)box * W.P_stride + q] = (uint8_t)acc;
+  W.pair_accn[(int64_t)box * W.P_stride + q] = acc ? ns : 0u;
 }
 
 __device__ inline int64_t find_pair(const uint32_t *exc, int64_t n, uint32_t t) {
@@ -440,12 +574,15 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t t = pend ? pend[i] : t0 + (uint32_t)i;
+  const int64_t NP = (int64_t)n_box * W.P_stride;
+  const int64_t k = find_pair(W.pair_exc, NP, t);
+  if (W.pair_acc[k]) return;  // a target slot: k_par_acc_tests
   W.acc[i] = 0;
   W.tid[i] = t;
   W.tp1[i] = -1;
   W.tp2[i] = -1;
-  const int64_t NP = (int64_t)n_box * W.P_stride;
-  const int64_t k = find_pair(W.pair_exc, NP, t);
+  W.trec[i] = -1;
+  W.tns[i] = 0;
   const int box = (int)(k / W.P_stride);
   BoxDesc *B = &boxes[box];
   if (B->error) return;
@@ -554,6 +691,399 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
   }
 }
 
+// ------------------------------------------------ accelerated target slots
+// try_per_particle_coag (src/coagulation.F90:150-237) runs its targets
+// e = n_target .. 1; here slot j of an accelerated pair is target entry
+// e = n_target - j, so the slots keep that order.  One thread per target
+// samples the virtual source particle (sample_source_particle, :333-517)
+// from the step's source list, skipping particles already consumed this
+// step and those it removed itself; it claims the target and every source
+// it removes.  The merge itself (coag_target_with_source, :522-582) is
+// applied when the event wins.
+struct AccPair {
+  int box, c1, c2, b1, b2, bt, ct, bs, cs, cc, nb, nc;
+  int n_t, n_s, st, ss, e;
+  double k_max;
+  uint32_t site;
+};
+
+__device__ inline AccPair acc_pair(BoxDesc *boxes, const Arrays &A, const PW &W, int64_t k,
+                                   uint32_t t) {
+  AccPair a;
+  a.box = (int)(k / W.P_stride);
+  decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &a.c1, &a.c2, &a.b1, &a.b2);
+  const BoxDesc *B = &boxes[a.box];
+  a.nb = B->n_bin;
+  a.nc = B->w.n_class;
+  const int64_t ti = table_index(A, a.box, a.nb, a.nc, a.b1, a.b2, a.c1, a.c2);
+  a.k_max = A.kw[ti];
+  a.cc = A.cc[ti];
+  if (W.pair_acc[k] == 1) {
+    a.bt = a.b1; a.ct = a.c1; a.bs = a.b2; a.cs = a.c2;
+  } else {
+    a.bt = a.b2; a.ct = a.c2; a.bs = a.b1; a.cs = a.c1;
+  }
+  const int64_t lb = (int64_t)a.box * W.K_stride;
+  const int kt = par_key(a.bt, a.ct, a.nb), ks = par_key(a.bs, a.cs, a.nb);
+  a.n_t = W.lcnt[lb + kt];
+  a.n_s = W.lcnt[lb + ks];
+  a.st = W.lstart[lb + kt];
+  a.ss = W.lstart[lb + ks];
+  a.site = pair_site(a.nb, a.nc, a.b1, a.b2, a.c1, a.c2);
+  a.e = a.n_t - (int)(t - W.pair_exc[k]);
+  return a;
+}
+
+// the stream kinds of a target's draws; re-draws use their own kinds
+__device__ inline uint32_t acc_kind(uint32_t kind, int round) {
+  return round == 0 ? kind : RNG_ACC_RETRY + 4u * (uint32_t)round + (kind - RNG_ACC_POISSON);
+}
+
+// aero_weight_array_rand_group (src/aero_weight_array.F90:496-514)
+__device__ inline int rand_group_w(Stream &s, const Weights &w, int cls, double r) {
+  double pdf[MAX_GROUP];
+  for (int g = 0; g < w.n_group; ++g) {
+    int idx = g + w.n_group * (cls - 1);
+    pdf[g] = 1.0 / weight_nc(w.type[idx], w.mag[idx], w.expo[idx], r);
+  }
+  return sample_cts_pdf(s, w.n_group, pdf);
+}
+
+__device__ inline double rec_vol(int n_spec, const Rec &r) {
+  double v = 0.0;
+  for (int s = 0; s < n_spec; ++s) v += r.vol[s];
+  return v;
+}
+
+// the sources a target removes, in a chain of RM_CHUNK-int chunks
+struct RmChain {
+  int head, cur, fill, n;
+  uint64_t bloom;
+};
+
+__device__ inline uint64_t rm_bit(int slot) {
+  return 1ull << ((((uint32_t)slot * 2654435761u) >> 26) & 63u);
+}
+
+__device__ inline bool rm_push(const PW &W, RmChain &c, int slot) {
+  if (c.fill == RM_CHUNK) {
+    int nc = atomicAdd(W.rm_top, RM_CHUNK);
+    if (nc + RM_CHUNK > W.rm_cap) return false;
+    W.rm_buf[nc] = -1;
+    if (c.cur >= 0) W.rm_buf[c.cur] = nc;
+    else c.head = nc;
+    c.cur = nc;
+    c.fill = 1;
+  }
+  W.rm_buf[c.cur + c.fill] = slot;
+  c.fill++;
+  c.n++;
+  c.bloom |= rm_bit(slot);
+  return true;
+}
+
+__device__ inline bool rm_has(const PW &W, const RmChain &c, int slot) {
+  if (!(c.bloom & rm_bit(slot))) return false;
+  int left = c.n;
+  for (int ch = c.head; ch >= 0 && left > 0; ch = W.rm_buf[ch]) {
+    int m = left < RM_CHUNK - 1 ? left : RM_CHUNK - 1;
+    for (int q = 1; q <= m; ++q)
+      if (W.rm_buf[ch + q] == slot) return true;
+    left -= m;
+  }
+  return false;
+}
+
+// visit the removed sources of record r in removal order
+template <class F>
+__device__ inline void rm_walk(const PW &W, int r, F f) {
+  int left = W.r_nrm[r];
+  for (int ch = W.r_head[r]; ch >= 0 && left > 0; ch = W.rm_buf[ch]) {
+    int m = left < RM_CHUNK - 1 ? left : RM_CHUNK - 1;
+    for (int q = 1; q <= m; ++q) f(W.rm_buf[ch + q]);
+    left -= m;
+  }
+}
+
+__device__ inline void rec_store_source(const PW &W, int n_spec, int r, const Rec &s) {
+  for (int k = 0; k < n_spec; ++k) W.r_vol[(int64_t)k * W.acc_cap + r] = s.vol[k];
+  W.r_nprim[r] = s.nprim;
+  W.r_ncomp[r] = s.ncomp;
+  for (int c = 0; c < MAX_COMP; ++c) {
+    W.r_csrc[(int64_t)c * W.acc_cap + r] = s.csrc[c];
+    W.r_ctime[(int64_t)c * W.acc_cap + r] = s.ctime[c];
+  }
+  W.r_least[r] = s.least;
+  W.r_greatest[r] = s.greatest;
+}
+
+__device__ inline void rec_load_source(const PW &W, int n_spec, int r, Rec &s) {
+  for (int k = 0; k < n_spec; ++k) s.vol[k] = W.r_vol[(int64_t)k * W.acc_cap + r];
+  s.id = 0;
+  s.group = 0;
+  s.klass = 0;
+  s.nprim = W.r_nprim[r];
+  s.ncomp = W.r_ncomp[r];
+  for (int c = 0; c < MAX_COMP; ++c) {
+    s.csrc[c] = W.r_csrc[(int64_t)c * W.acc_cap + r];
+    s.ctime[c] = W.r_ctime[(int64_t)c * W.acc_cap + r];
+  }
+  s.least = W.r_least[r];
+  s.greatest = W.r_greatest[r];
+}
+
+// coag_target_with_source up to the copy count (src/coagulation.F90:
+// 522-582 with aero_state_dup_particle's prob_round, src/aero_state.F90:
+// 488-503): the merged target, its bin and the number of copies
+__device__ inline int acc_merge(int n_spec, const Weights &w, Stream &fin, const Rec &tp,
+                                const Rec &src, int cc, int nb, double log_lo, double log_hi,
+                                Rec &np, int *n_copies, int *new_bin) {
+  double old_nc = nc_at_radius(w, tp.klass, vol2rad(rec_vol(n_spec, tp)));
+  int err = par_rec_coagulate(n_spec, fin, tp, src, np);
+  if (err) return err;
+  np.id = tp.id;
+  double r_new = vol2rad(rec_vol(n_spec, np));
+  np.group = rand_group_w(fin, w, cc, r_new);
+  np.klass = cc;
+  *new_bin = bin_find_log(nb, log_lo, log_hi, r_new);
+  double new_nc = nc_at_radius(w, np.klass, r_new);
+  *n_copies = prob_round(fin, old_nc / new_nc);
+  return 0;
+}
+
+// Round 0 runs one thread per target slot of the batch (record a maps to
+// its slot through acc_exc); later rounds run over the pending slots.
+__global__ void __launch_bounds__(64)
+k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed,
+                uint32_t step, int round, uint32_t t0, int n, const uint32_t *pend,
+                double del_t, uint32_t acc_base, int n_acc) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t NP = (int64_t)n_box * W.P_stride;
+  uint32_t t;
+  int i;
+  int64_t k;
+  if (pend) {
+    if (idx >= n) return;
+    i = idx;
+    t = pend[i];
+    k = find_pair(W.pair_exc, NP, t);
+    if (!W.pair_acc[k]) return;
+  } else {
+    if (idx >= n_acc) return;
+    const uint32_t a = acc_base + (uint32_t)idx;
+    k = find_pair(W.acc_exc, NP, a);
+    t = W.pair_exc[k] + (a - W.acc_exc[k]);
+    i = (int)(t - t0);
+  }
+  const int rec = (int)(W.acc_exc[k] + (t - W.pair_exc[k]) - acc_base);
+  W.acc[i] = 0;
+  W.tid[i] = t;
+  W.tp1[i] = -1;
+  W.tp2[i] = -1;
+  W.trec[i] = rec;
+  W.tns[i] = 0;
+  const AccPair P = acc_pair(boxes, A, W, k, t);
+  BoxDesc *B = &boxes[P.box];
+  if (B->error) return;
+  const int64_t base = B->off;
+  const int p = W.perm[base + P.st + P.e - 1];
+  if (W.status[p] != ST_LIVE) return;  // consumed earlier in this step
+  W.tp1[i] = p;
+  const int ns = A.n_spec;
+  const Weights &w = B->w;
+  const Env env = B->env;
+  const double n_spt = P.k_max * del_t * (double)P.n_s;
+  const double accept_factor = 1.0 / P.k_max;
+  Rec target, source, pi;
+  load_rec(A, p, target);
+  const int64_t target_id = target.id;
+  Stream aps, smp, fin;
+  aps.open(seed, P.box, step, P.site, (uint32_t)P.e, acc_kind(RNG_ACC_POISSON, round));
+  smp.open(seed, P.box, step, P.site, (uint32_t)P.e, acc_kind(RNG_ACC_SAMPLE, round));
+  fin.open(seed, P.box, step, P.site, (uint32_t)P.e, acc_kind(RNG_ACC_FINISH, round));
+  const double nc_target = nc_at_radius(w, target.klass, vol2rad(rec_vol(ns, target)));
+  const double nc_src_min = nc_at_radius(w, P.cs, A.edges[(int64_t)P.box * (A.max_bin + 1) + P.bs - 1]);
+  const double prm = nc_target / nc_src_min;
+  if (!(prm <= 1.0)) {
+    atomicExch(&B->error, 653606684);
+    return;
+  }
+  const int n_remove = rand_poisson(aps, prm * n_spt);
+  const int n_extra = rand_poisson(aps, (1.0 - prm) * n_spt);
+  const int n_total = n_remove + n_extra;
+  int n_avg = 0, n_samp = 0;
+  double prob_coag_tot = 0.0;
+  double vol_sq[MAX_SPEC], vol_cv[MAX_SPEC];
+  for (int q = 0; q < ns; ++q) {
+    vol_sq[q] = 0.0;
+    vol_cv[q] = 0.0;
+    source.vol[q] = 0.0;
+  }
+  source.id = 0;
+  source.group = source.klass = source.nprim = source.ncomp = 0;
+  for (int c = 0; c < MAX_COMP; ++c) {
+    source.csrc[c] = 0;
+    source.ctime[c] = 0.0;
+  }
+  source.least = source.greatest = 0.0;
+  RmChain chain{-1, -1, RM_CHUNK, 0, 0ull};
+  const int64_t sbase = base + P.ss;
+  for (int i_samp = 1; i_samp <= n_total; ++i_samp) {
+    if (chain.n >= P.n_s) break;  // every source removed
+    if (n_samp > n_remove && n_avg >= 2) {
+      double cvmax = 0.0;
+      for (int q = 0; q < ns; ++q) {
+        double vm = source.vol[q] / (double)n_avg;
+        if (vm > 0.0) vol_cv[q] = sqrt(vol_sq[q] / (double)n_avg - vm * vm) / vm;
+        if (q == 0 || vol_cv[q] > cvmax) cvmax = vol_cv[q];
+      }
+      double m95 = cvmax / sqrt((double)n_avg) * student_t_95_coeff(n_avg);
+      if (m95 < 0.1) break;
+    }
+    // a uniform draw among the sources still present
+    int ps = -1;
+    for (int a = 0; a < MAX_REDRAW; ++a) {
+      int c = rand_int(smp, P.n_s);
+      int q = W.perm[sbase + c - 1];
+      if (W.status[q] == ST_LIVE && !rm_has(W, chain, q)) {
+        ps = q;
+        break;
+      }
+    }
+    if (ps < 0) {
+      atomicAdd(&B->n_dropped, 1);
+      break;
+    }
+    n_samp++;
+    load_rec(A, ps, pi);
+    double kk;
+    int err = weighted_kernel64(A, w, env, type, ps, p, P.cs, P.ct, P.ct, &kk);
+    if (err) {
+      atomicExch(&B->error, err);
+      return;
+    }
+    double prob_coag = kk * accept_factor;
+    prob_coag_tot += prob_coag;
+    if (smp.uniform() < prob_coag) {
+      n_avg++;
+      err = par_rec_coagulate(ns, smp, source, pi, source);
+      if (err) {
+        atomicExch(&B->error, err);
+        return;
+      }
+      for (int q = 0; q < ns; ++q) vol_sq[q] += pi.vol[q] * pi.vol[q];
+      if (i_samp <= n_remove) {
+        double nc_i = nc_at_radius(w, pi.klass, vol2rad(rec_vol(ns, pi)));
+        double prob_remove_i = nc_target / nc_i;
+        if (smp.uniform() < prob_remove_i / prm) {
+          if (!rm_push(W, chain, ps)) {
+            atomicExch(&B->error, ERR_CAPACITY);
+            return;
+          }
+          atomicMin(&W.claim[ps], t);
+        }
+      }
+    }
+  }
+  int n_coag = 0, warn = 0, n_copies = -1;
+  if (n_avg > 0) {
+    double prob_coag_mean = prob_coag_tot / (double)n_samp;
+    if (!(prob_coag_mean <= 1.0)) warn = 1;
+    n_coag = rand_binomial(fin, n_total, prob_coag_mean);
+    for (int q = 0; q < ns; ++q) source.vol[q] = source.vol[q] * ((double)n_coag / (double)n_avg);
+    source.nprim = prob_round(fin, (double)source.nprim * ((double)n_coag / (double)n_avg));
+    const int n_comp = source.ncomp;
+    const int n_comp_new = source.nprim < MAX_COMP ? source.nprim : MAX_COMP;
+    int sample[MAX_COMP];
+    int cs_src[MAX_COMP];
+    double ct_src[MAX_COMP];
+    for (int c = 0; c < MAX_COMP; ++c) {
+      cs_src[c] = source.csrc[c];
+      ct_src[c] = source.ctime[c];
+    }
+    if (n_comp_new < n_comp) {
+      sample_without_replacement(fin, n_comp_new, n_comp, sample);
+      for (int c = 0; c < n_comp_new; ++c) {
+        source.csrc[c] = cs_src[sample[c] - 1];
+        source.ctime[c] = ct_src[sample[c] - 1];
+      }
+      source.ncomp = n_comp_new;
+    } else if (n_comp < MAX_COMP) {
+      int i_comp = 1;
+      while (i_comp <= n_comp_new) {
+        if (i_comp + n_comp <= n_comp_new) {
+          for (int c = 1; c <= n_comp; ++c) {
+            source.csrc[i_comp + c - 2] = cs_src[c - 1];
+            source.ctime[i_comp + c - 2] = ct_src[c - 1];
+          }
+          i_comp += n_comp;
+        } else {
+          int sample_size = n_comp_new - i_comp + 1;
+          sample_without_replacement(fin, sample_size, n_comp, sample);
+          for (int c = 1; c <= sample_size; ++c) {
+            source.csrc[i_comp + c - 2] = cs_src[sample[c - 1] - 1];
+            source.ctime[i_comp + c - 2] = ct_src[sample[c - 1] - 1];
+          }
+          i_comp = n_comp_new + 1;
+        }
+      }
+      source.ncomp = n_comp_new;
+    }
+    if (source.nprim < source.ncomp) {
+      atomicExch(&B->error, 808144130);
+      return;
+    }
+    if (n_coag > 0) {
+      Stream ft;
+      ft.open(seed, P.box, step, P.site, (uint32_t)P.e, acc_kind(RNG_ACC_TARGET, round));
+      Rec np;
+      int nbin;
+      int err = acc_merge(ns, w, ft, target, source, P.cc, P.nb, B->log_lo, B->log_hi, np,
+                          &n_copies, &nbin);
+      if (err) {
+        atomicExch(&B->error, err);
+        return;
+      }
+    }
+  }
+  (void)target_id;
+  if (n_coag == 0 && chain.n == 0) {
+    W.tns[i] = n_samp | (warn ? ACC_WARN : 0);
+    return;
+  }
+  if (n_coag > 0) rec_store_source(W, ns, rec, source);
+  W.r_nsamp[rec] = n_samp;
+  W.r_ncoag[rec] = n_coag;
+  W.r_ncopies[rec] = n_copies;
+  W.r_nrm[rec] = chain.n;
+  W.r_head[rec] = chain.head;
+  W.r_warn[rec] = warn;
+  W.acc[i] = 1;
+  atomicMin(&W.claim[p], t);
+}
+
+// the largest batch end t1 <= min(T, t0 + T_cap) whose target slots fit the
+// records; out[0] = t1, out[1] = acc_count(t0), out[2] = acc_count(t1)
+__device__ inline uint32_t acc_count(const PW &W, int64_t NP, uint32_t T, uint32_t t) {
+  if (t >= T) return W.acc_exc[NP];
+  int64_t k = find_pair(W.pair_exc, NP, t);
+  return W.acc_exc[k] + (W.pair_acc[k] ? t - W.pair_exc[k] : 0u);
+}
+
+__global__ void k_batch_end(PW W, int64_t NP, uint32_t T, uint32_t t0, uint32_t t_cap,
+                            uint32_t a_cap, uint32_t *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t a0 = acc_count(W, NP, T, t0);
+  uint32_t lo = t0 + 1, hi = (T - t0 < t_cap) ? T : t0 + t_cap;
+  while (lo < hi) {  // largest t1 in [lo, hi] with acc_count(t1) - a0 <= a_cap
+    uint32_t mid = lo + (hi - lo + 1) / 2;
+    if (acc_count(W, NP, T, mid) - a0 <= a_cap) lo = mid; else hi = mid - 1;
+  }
+  out[0] = lo;
+  out[1] = a0;
+  out[2] = acc_count(W, NP, T, lo);
+}
+
 // compact accepted tests into the event list (in test order)
 __global__ void k_par_gather(PW W, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -562,6 +1092,7 @@ __global__ void k_par_gather(PW W, int n) {
   W.ev_t[e] = W.tid[i];
   W.ev_p1[e] = W.tp1[i];
   W.ev_p2[e] = W.tp2[i];
+  W.ev_rec[e] = W.trec[i];
 }
 
 // outcome bits of ev_out
@@ -639,6 +1170,22 @@ __global__ void k_par_resolve(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, 
   int p1 = W.ev_p1[e], p2 = W.ev_p2[e];
   MergeCtx m = merge_ctx(boxes, W, n_box, t);
   W.ev_box[e] = m.box;
+  const int r = W.ev_rec[e];
+  if (r >= 0) {
+    // a target event wins when it holds the target and every removed source
+    bool win = W.claim[p1] == t;
+    rm_walk(W, r, [&](int q) { win = win && W.claim[q] == t; });
+    int nc = W.r_ncopies[r], nrm = W.r_nrm[r];
+    int dups = (win && nc > 1) ? nc - 1 : 0;
+    int gone = win ? nrm + (nc == 0 ? 1 : 0) : 0;
+    W.ev_out[e] = win ? O_WIN : 0;
+    W.f_app[e] = dups;
+    W.f_id[e] = dups;
+    W.f_info[e] = gone;
+    W.f_hole[e] = gone;
+    W.f_lose[e] = win ? 0 : 1;
+    return;
+  }
   bool win = W.claim[p1] == t && W.claim[p2] == t;
   int o = 0;
   if (win) {
@@ -698,78 +1245,112 @@ __device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc 
   if (want_terms) W.terms[slot] = particle_terms(type, v, m / v, B->env);
 }
 
-__device__ inline void load_rec(const Arrays &A, int64_t p, Rec &r) {
-  for (int s = 0; s < A.n_spec; ++s) r.vol[s] = A.vol[(int64_t)s * A.total_cap + p];
-  r.id = A.id[p];
-  r.group = A.group[p];
-  r.klass = A.klass[p];
-  r.nprim = A.nprim[p];
-  r.ncomp = A.ncomp[p];
-  for (int c = 0; c < MAX_COMP; ++c) {
-    r.csrc[c] = A.csrc[(int64_t)c * A.total_cap + p];
-    r.ctime[c] = A.ctime[(int64_t)c * A.total_cap + p];
+// coag_target_with_source and aero_state_dup_particle for a winning target
+// event (src/coagulation.F90:522-582, src/aero_state.F90:488-539): removed
+// sources become holes with a coagulation record naming the target, the
+// target is merged in place and its copies appended, or, when no copy is
+// kept, the target is removed with a weighting record.
+__device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, int n_box, int e,
+                                 uint64_t seed, uint32_t step, int round, int type,
+                                 int want_terms, int64_t hole_base) {
+  const uint32_t t = W.ev_t[e];
+  const int64_t k = find_pair(W.pair_exc, (int64_t)n_box * W.P_stride, t);
+  const AccPair P = acc_pair(boxes, A, W, k, t);
+  BoxDesc *B = &boxes[P.box];
+  const int r = W.ev_rec[e];
+  const int64_t p = W.ev_p1[e];
+  int *live = W.live + (int64_t)P.box * W.K_stride;
+  const int first = W.box_first[P.box];
+  const int ns = A.n_spec;
+  const int64_t target_id = A.id[p];
+  int64_t hrank = hole_base + W.s_hole[e];
+  int64_t irank = B->n_info + (W.s_info[e] - W.s_info[first]);
+  auto info = [&](int64_t id, int action, int64_t other) {
+    if (irank < B->info_cap) {
+      A.info_id[B->info_off + irank] = id;
+      A.info_action[B->info_off + irank] = action;
+      A.info_other[B->info_off + irank] = other;
+    }
+    irank++;
+  };
+  rm_walk(W, r, [&](int q) {
+    W.dead_t[q] = t;
+    atomicSub(&live[W.key[q]], 1);
+    W.status[q] = ST_HOLE;
+    W.holes[hrank++] = ((uint64_t)P.box << 32) | (uint64_t)(q - B->off);
+    info(A.id[q], 2, target_id);
+  });
+  const unsigned long long n_samp = (unsigned long long)W.r_nsamp[r];
+  const unsigned long long n_coag = (unsigned long long)W.r_ncoag[r];
+  atomicAdd((unsigned long long *)&B->n_samp, n_samp);
+  atomicAdd((unsigned long long *)&B->step_samp, n_samp);
+  atomicAdd((unsigned long long *)&B->n_coag, n_coag);
+  atomicAdd((unsigned long long *)&B->step_coag, n_coag);
+  if (W.r_warn[r]) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
+  if (n_coag == 0) return;
+  const int n_copies = W.r_ncopies[r];
+  Rec tp, src, np;
+  load_rec(A, p, tp);
+  rec_load_source(W, ns, r, src);
+  Stream ft;
+  ft.open(seed, P.box, step, P.site, (uint32_t)P.e, acc_kind(RNG_ACC_TARGET, round));
+  int n_check, nbin;
+  int err = acc_merge(ns, B->w, ft, tp, src, P.cc, P.nb, B->log_lo, B->log_hi, np, &n_check,
+                      &nbin);
+  if (err) {
+    atomicExch(&B->error, err);
+    return;
   }
-  r.least = A.least[p];
-  r.greatest = A.greatest[p];
-}
-
-// aero_particle_coagulate for the parallel path (same rules as rec_coagulate)
-__device__ inline int par_rec_coagulate(int n_spec, Stream &s, const Rec &p1, const Rec &p2,
-                                        Rec &o) {
-  for (int k = 0; k < n_spec; ++k) o.vol[k] = p1.vol[k] + p2.vol[k];
-  o.group = 0;
-  o.klass = 0;
-  o.id = 0;
-  int n1 = p1.ncomp, n2 = p2.ncomp;
-  if (p1.nprim < n1) return 465791384;
-  if (p2.nprim < n2) return 465791385;
-  for (int c = 0; c < MAX_COMP; ++c) {
-    o.csrc[c] = 0;
-    o.ctime[c] = 0.0;
+  if (nbin < 1 || nbin > P.nb) {
+    atomicExch(&B->error, 765620746);
+    return;
   }
-  if (n1 + n2 > MAX_COMP) {
-    int n1_new = prob_round(s, (double)p1.nprim / (double)(p1.nprim + p2.nprim) * (double)MAX_COMP);
-    int n2_new = MAX_COMP - n1_new;
-    int sample[MAX_COMP];
-    sample_without_replacement(s, n1_new, n1, sample);
-    for (int i = 0; i < n1_new; ++i) {
-      o.csrc[i] = p1.csrc[sample[i] - 1];
-      o.ctime[i] = p1.ctime[sample[i] - 1];
-    }
-    sample_without_replacement(s, n2_new, n2, sample);
-    for (int i = 0; i < n2_new; ++i) {
-      o.csrc[i + n1_new] = p2.csrc[sample[i] - 1];
-      o.ctime[i + n1_new] = p2.ctime[sample[i] - 1];
-    }
-    o.ncomp = MAX_COMP;
-  } else {
-    for (int i = 0; i < n1; ++i) {
-      o.csrc[i] = p1.csrc[i];
-      o.ctime[i] = p1.ctime[i];
-    }
-    for (int i = 0; i < n2; ++i) {
-      o.csrc[n1 + i] = p2.csrc[i];
-      o.ctime[n1 + i] = p2.ctime[i];
-    }
-    o.ncomp = n1 + n2;
+  W.dead_t[p] = t;
+  atomicSub(&live[W.key[p]], 1);
+  write_new_particle(A, W, B, p, np, P.nb, type, want_terms);
+  if (n_copies == 0) {
+    W.status[p] = ST_HOLE;
+    W.holes[hrank++] = ((uint64_t)P.box << 32) | (uint64_t)(p - B->off);
+    info(np.id, 4, 0);
+    return;
   }
-  o.least = fmin(p1.least, p2.least);
-  o.greatest = fmax(p1.greatest, p2.greatest);
-  o.nprim = p1.nprim + p2.nprim;
-  return 0;
+  atomicAdd(&live[W.key[p]], 1);
+  const double r_np = vol2rad(A.vtot[p]);
+  int64_t id_rank = W.s_id[e] - W.s_id[first];
+  int64_t app_rank = W.s_app[e] - W.s_app[first];
+  for (int c = 1; c < n_copies; ++c) {
+    np.id = (B->next_id + id_rank - 1) * B->id_stride + B->id_offset + 1;
+    id_rank++;
+    np.group = rand_group_w(ft, B->w, np.klass, r_np);
+    if (B->n_part + app_rank >= B->cap) {
+      atomicExch(&B->error, ERR_CAPACITY);
+      return;
+    }
+    const int64_t slot = B->off + B->n_part + app_rank;
+    app_rank++;
+    write_new_particle(A, W, B, slot, np, P.nb, type, want_terms);
+    atomicAdd(&live[W.key[slot]], 1);
+  }
 }
 
 // apply the winners' merges (coagulate, src/coagulation.F90:928-985)
 __global__ void k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, uint64_t seed,
-                            uint32_t step, int type, int want_terms, int64_t hole_base) {
+                            uint32_t step, int round, int type, int want_terms,
+                            int64_t hole_base) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   int o = W.ev_out[e];
   if (!(o & O_WIN)) return;
+  if (W.ev_rec[e] >= 0) {
+    acc_apply(boxes, A, W, n_box, e, seed, step, round, type, want_terms, hole_base);
+    return;
+  }
   uint32_t t = W.ev_t[e];
   int64_t p1 = W.ev_p1[e], p2 = W.ev_p2[e];
   MergeCtx m = merge_ctx(boxes, W, n_box, t);
   BoxDesc *B = &boxes[m.box];
+  atomicAdd((unsigned long long *)&B->n_coag, 1ull);
+  atomicAdd((unsigned long long *)&B->step_coag, 1ull);
   const int nb = B->n_bin, nc = B->w.n_class;
   const int cc = A.cc[table_index(A, m.box, nb, nc, m.b1, m.b2, m.c1, m.c2)];
   int *live = W.live + (int64_t)m.box * W.K_stride;
@@ -866,20 +1447,18 @@ __global__ void k_par_box_totals(BoxDesc *boxes, PW W, int n_box, int E) {
   int app = W.s_app[last] + W.f_app[last] - W.s_app[first];
   int ids = W.s_id[last] + W.f_id[last] - W.s_id[first];
   int info = W.s_info[last] + W.f_info[last] - W.s_info[first];
-  int lose = W.s_lose[last] + W.f_lose[last] - W.s_lose[first];
-  int wins = (last - first + 1) - lose;
   B->n_part += app;
   B->next_id += ids;
   B->n_info += info;
-  B->n_coag += wins;
-  B->step_coag += wins;
 }
 
 __global__ void k_par_release(PW W, int E) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   W.claim[W.ev_p1[e]] = UNCLAIMED;
-  W.claim[W.ev_p2[e]] = UNCLAIMED;
+  const int r = W.ev_rec[e];
+  if (r >= 0) rm_walk(W, r, [&](int q) { W.claim[q] = UNCLAIMED; });
+  else W.claim[W.ev_p2[e]] = UNCLAIMED;
 }
 
 // Tests of the round that must be drawn again: accepted tests that lost a
@@ -890,13 +1469,26 @@ __device__ inline bool consumed_before(const PW &W, int p, uint32_t t) {
   return p >= 0 && W.status[p] != ST_LIVE && W.dead_t[p] < t;
 }
 
-__global__ void k_par_pending(PW W, int n) {
+__global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t t = W.tid[i];
   int f = 0;
   if (W.acc[i]) {
     f = (W.ev_out[W.acc_scan[i]] & O_WIN) ? 0 : 1;
+  } else if (W.trec[i] >= 0) {
+    // a target slot without effect: final unless its target was consumed
+    // by an earlier winner; its samples count once final
+    f = consumed_before(W, W.tp1[i], t) ? 1 : 0;
+    int v = W.tns[i];
+    if (!f && v) {
+      const int64_t k = find_pair(W.pair_exc, (int64_t)n_box * W.P_stride, t);
+      BoxDesc *B = &boxes[k / W.P_stride];
+      unsigned long long ns = (unsigned long long)(v & (ACC_WARN - 1));
+      atomicAdd((unsigned long long *)&B->n_samp, ns);
+      atomicAdd((unsigned long long *)&B->step_samp, ns);
+      if (v & ACC_WARN) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
+    }
   } else if (W.tp1[i] >= 0) {
     f = (consumed_before(W, W.tp1[i], t) || consumed_before(W, W.tp2[i], t)) ? 1 : 0;
   }
@@ -929,14 +1521,6 @@ __global__ void k_step_begin(BoxDesc *boxes, int n_box) {
   B->step_samp = 0;
   B->step_coag = 0;
   B->part_steps += B->n_part;
-}
-
-__global__ void k_step_samp(BoxDesc *boxes, PW W, int n_box) {
-  // step_samp = sum of this step's pair counts of the box
-  int box = blockIdx.x * blockDim.x + threadIdx.x;
-  if (box >= n_box) return;
-  int64_t o = (int64_t)box * W.P_stride;
-  boxes[box].step_samp = (int64_t)W.pair_exc[o + W.P_stride] - (int64_t)W.pair_exc[o];
 }
 
 // tail-fill compaction, one block per box: holes (sorted by slot) below the
@@ -1031,6 +1615,41 @@ inline int parallel_init(ParallelWork &W, int n_box, int max_bin) {
   return 0;
 }
 
+// the accelerated path's records, allocated when a step first has target slots
+inline void parallel_free_records(ParallelWork &W) {
+  PW &d = W.d;
+  void *ptrs[] = {d.r_vol, d.r_nprim, d.r_ncomp, d.r_csrc, d.r_ctime, d.r_least, d.r_greatest,
+                  d.r_nsamp, d.r_ncoag, d.r_ncopies, d.r_nrm, d.r_head, d.r_warn, d.rm_buf,
+                  d.rm_top};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  d.r_vol = nullptr;
+  d.r_nprim = d.r_ncomp = d.r_csrc = nullptr;
+  d.r_ctime = d.r_least = d.r_greatest = nullptr;
+  d.r_nsamp = d.r_ncoag = d.r_ncopies = d.r_nrm = d.r_head = d.r_warn = nullptr;
+  d.rm_buf = d.rm_top = nullptr;
+  d.acc_cap = 0;
+  d.rm_cap = 0;
+}
+
+inline int parallel_alloc_records(ParallelWork &W, int n_spec) {
+  PW &d = W.d;
+  const int64_t c = ACC_CAP;
+  PCHECK(pmalloc(&d.r_vol, (size_t)n_spec * c));
+  PCHECK(pmalloc(&d.r_csrc, (size_t)MAX_COMP * c));
+  PCHECK(pmalloc(&d.r_ctime, (size_t)MAX_COMP * c));
+  int **iv[] = {&d.r_nprim, &d.r_ncomp, &d.r_nsamp, &d.r_ncoag, &d.r_ncopies, &d.r_nrm,
+                &d.r_head, &d.r_warn};
+  for (int **q : iv) PCHECK(pmalloc(q, c));
+  PCHECK(pmalloc(&d.r_least, c));
+  PCHECK(pmalloc(&d.r_greatest, c));
+  PCHECK(pmalloc(&d.rm_buf, RM_CAP));
+  PCHECK(pmalloc(&d.rm_top, 1));
+  d.acc_cap = ACC_CAP;
+  d.rm_cap = RM_CAP;
+  return 0;
+}
+
 inline void parallel_free_arrays(ParallelWork &W) {
   PW &d = W.d;
   void *ptrs[] = {d.key,  d.status, d.claim, d.terms, d.perm, d.tile_box, d.tile_start,
@@ -1039,9 +1658,11 @@ inline void parallel_free_arrays(ParallelWork &W) {
                   d.ev_p1, d.ev_p2, d.ev_box, d.ev_out, d.f_app, d.f_id, d.f_info, d.f_hole,
                   d.f_lose, d.f_pend, d.s_pend, d.dead_t, d.s_app, d.s_id, d.s_info, d.s_hole,
                   d.s_lose, d.box_first,
-                  d.holes, d.holes_sorted, d.tail_live, W.scan_tmp, W.sort_tmp};
+                  d.holes, d.holes_sorted, d.tail_live, W.scan_tmp, W.sort_tmp,
+                  d.pair_acc, d.pair_accn, d.acc_exc, d.trec, d.tns, d.ev_rec, d.misc};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  parallel_free_records(W);
   int ks = d.K_stride;
   int64_t ps = d.P_stride;
   int tc = d.tab_classes, mb = d.max_bin;
@@ -1113,6 +1734,11 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.pair_n, W.NP + 1));
   PCHECK(pmalloc(&d.pair_exc, W.NP + 1));
   PCHECK(hipMemset(d.pair_n, 0, sizeof(uint32_t) * (W.NP + 1)));
+  PCHECK(pmalloc(&d.pair_acc, W.NP));
+  PCHECK(pmalloc(&d.pair_accn, W.NP + 1));
+  PCHECK(pmalloc(&d.acc_exc, W.NP + 1));
+  PCHECK(hipMemset(d.pair_accn, 0, sizeof(uint32_t) * (W.NP + 1)));
+  PCHECK(pmalloc(&d.misc, 4));
   const int64_t T = W.T_cap;
   PCHECK(pmalloc(&d.acc, T + 1));
   PCHECK(pmalloc(&d.acc_scan, T + 1));
@@ -1125,6 +1751,9 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.ev_p2, T));
   PCHECK(pmalloc(&d.ev_box, T));
   PCHECK(pmalloc(&d.ev_out, T));
+  PCHECK(pmalloc(&d.trec, T));
+  PCHECK(pmalloc(&d.tns, T));
+  PCHECK(pmalloc(&d.ev_rec, T));
   int **fl[] = {&d.f_app, &d.f_id, &d.f_info, &d.f_hole, &d.f_lose, &d.f_pend,
                 &d.s_app, &d.s_id, &d.s_info, &d.s_hole, &d.s_lose, &d.s_pend};
   for (int **f : fl) PCHECK(pmalloc(f, T + 1));
@@ -1212,21 +1841,50 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
           dim3(256), 0, st, d_box, A, d, del_t, seed, step);
   PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.pair_n, d.pair_exc,
                                           (int)(W.NP + 1), st));
-  hipLaunchKernelGGL(k_step_samp, gb, bb, 0, st, d_box, d, n_box);
+  PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.pair_accn, d.acc_exc,
+                                          (int)(W.NP + 1), st));
   PCHECK(hipMemcpyAsync(W.h_u32, d.pair_exc + W.NP, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  PCHECK(hipMemcpyAsync(W.h_u32 + 1, d.acc_exc + W.NP, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   PCHECK(hipStreamSynchronize(st));
   const uint32_t T = W.h_u32[0];
+  const uint32_t T_acc = W.h_u32[1];
+  if (T_acc > 0 && d.r_vol == nullptr) {
+    int e = parallel_alloc_records(W, A.n_spec);
+    if (e) return e;
+  }
   hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((A.total_cap + 255) / 256)), dim3(256), 0, st,
                      d.status, A.total_cap, ST_LIVE);
   // 5. tests, claims and merges, in batches of T_cap tests
   int64_t n_holes = 0;
-  for (uint32_t t0 = 0; t0 < T; t0 += (uint32_t)W.T_cap) {
-    int n = (int)std::min<int64_t>((int64_t)T - t0, W.T_cap);
+  for (uint32_t t0 = 0; t0 < T;) {
+    uint32_t t1 = (uint32_t)std::min<int64_t>((int64_t)T, (int64_t)t0 + W.T_cap);
+    uint32_t a0 = 0, a1 = 0;
+    if (T_acc > 0) {
+      // the batch ends where its target slots would overflow the records
+      hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, d, W.NP, T, t0,
+                         (uint32_t)W.T_cap, (uint32_t)d.acc_cap, d.misc);
+      PCHECK(hipMemcpyAsync(W.h_u32 + 4, d.misc, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      PCHECK(hipStreamSynchronize(st));
+      t1 = W.h_u32[4];
+      a0 = W.h_u32[5];
+      a1 = W.h_u32[6];
+    }
+    const int n_acc = (int)(a1 - a0);
+    int n = (int)(t1 - t0);
+    const uint32_t batch_t0 = t0;
+    t0 = t1;
     const uint32_t *pend = nullptr;
     for (int round = 0; n > 0; ++round) {
       if (round >= MAX_ROUNDS) break;
+      W.n_rounds++;
+      if (n_acc > 0) hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.rm_top, 1, 0);
       PLAUNCH(W, "k_par_tests", k_par_tests, dim3((n + 255) / 256), dim3(256), 0, st, d_box, A, d,
-              n_box, type, seed, step, round, t0, n, pend, precision, band, validation);
+              n_box, type, seed, step, round, batch_t0, n, pend, precision, band, validation);
+      if (n_acc > 0) {
+        const int na = pend ? n : n_acc;
+        PLAUNCH(W, "k_par_acc_tests", k_par_acc_tests, dim3((na + 63) / 64), dim3(64), 0, st,
+                d_box, A, d, n_box, type, seed, step, round, batch_t0, n, pend, del_t, a0, n_acc);
+      }
       hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.acc + n, 1, 0);
       PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.acc, d.acc_scan, n + 1, st));
       PCHECK(hipMemcpyAsync(W.h_i32, d.acc_scan + n, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1246,10 +1904,10 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, fs[q], ss[q], E + 1, st));
       }
       PLAUNCH(W, "k_par_apply", k_par_apply, dim3((E + 255) / 256), dim3(256), 0, st, d_box, A, d,
-              n_box, E, seed, step, type, want_terms, n_holes);
+              n_box, E, seed, step, round, type, want_terms, n_holes);
       hipLaunchKernelGGL(k_par_box_totals, gb, bb, 0, st, d_box, d, n_box, E);
       hipLaunchKernelGGL(k_par_release, dim3((E + 255) / 256), dim3(256), 0, st, d, E);
-      hipLaunchKernelGGL(k_par_pending, dim3((n + 255) / 256), dim3(256), 0, st, d, n);
+      hipLaunchKernelGGL(k_par_pending, dim3((n + 255) / 256), dim3(256), 0, st, d_box, d, n_box, n);
       hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.f_pend + n, 1, 0);
       PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.f_pend, d.s_pend, n + 1, st));
       hipLaunchKernelGGL(k_par_gather_pending, dim3((n + 255) / 256), dim3(256), 0, st, d, n);

@@ -168,3 +168,70 @@ def test_brownian_ensemble_matches_oracle():
     diff = gpu_n.mean() - orc_n.mean()
     se = np.sqrt(gpu_n.var(ddof=1) / n_rep + orc_n.var(ddof=1) / n_rep)
     assert abs(diff) <= 2.0 * se + 1e-9 * orc_n.mean(), (gpu_n.mean(), orc_n.mean(), se)
+
+
+def conc_metrics(vol, pop):
+    """Particle count and the number concentration the particles stand for
+    (aero_weight num_conc = magnitude * r**exponent, src/aero_weight.F90:
+    127-146), in total and above 0.2 um radius."""
+    r = radius(vol)
+    nc = pop.weight_magnitude * r ** pop.weight_exponent
+    return np.array([len(r), nc.sum(), nc[r > 2e-7].sum()])
+
+
+def test_accelerated_ensemble_matches_oracle():
+    """Power weighting (trimodal, exponent -3): bin pairs with a target bin of
+    lower weight take the per-particle path (try_per_particle_coag,
+    src/coagulation.F90:150-241).  Over independent runs, the mean
+    coagulation count, particle count and number concentrations of the
+    parallel GPU chain and the sequential oracle agree within their 95%
+    confidence intervals."""
+    base = small(P.config_c2(), 6000)
+    base.weight_magnitude *= 30.0  # ~30% of the number concentration coagulates
+    n_rep, n_steps, dt = 16, 10, 60.0
+    g = par([base] * n_rep, 4321)
+    _, coag = step_n(g, P.KERNEL_BROWN, dt, n_steps)
+    st = g.path_stats()
+    assert st["n_accel_pairs"].min() > 0
+    assert st["n_cut"] == 0
+    gm = np.array([conc_metrics(g.download(k)["vol"], base) for k in range(n_rep)])
+    gm = np.column_stack([gm, coag.astype(float)])
+    om = []
+    n_accel = 0
+    for k in range(n_rep):
+        ob = oracle_box(base, 9000 + k)
+        c = 0
+        for s in range(n_steps):
+            c += ob.mc_coag(P.KERNEL_BROWN, dt, s)[1]
+        n_accel += ob.n_accel
+        om.append(np.append(conc_metrics(ob.particles()["vol"], base), c))
+    om = np.array(om)
+    assert n_accel > 0
+    diff = gm.mean(axis=0) - om.mean(axis=0)
+    se = np.sqrt(gm.var(axis=0, ddof=1) / n_rep + om.var(axis=0, ddof=1) / n_rep)
+    # the count is conserved to a few particles per run: allow one particle
+    assert np.all(np.abs(diff) <= 2.0 * se + np.array([1.0, 0, 0, 0])), (gm.mean(0), om.mean(0), se)
+
+
+def test_accelerated_invariants():
+    """Identifiers stay unique, every particle stays inside the grid, and
+    every removed source's record names a particle id (the target) that
+    existed."""
+    pop = small(P.config_c2(), 8000)
+    pop.weight_magnitude *= 30.0
+    g = par([pop], 77)
+    step_n(g, P.KERNEL_BROWN, 60.0, 6)
+    assert g.path_stats()["n_accel_pairs"][0] > 0
+    d = g.download(0)
+    assert len(np.unique(d["id"])) == len(d["id"])
+    e, _, _ = g.grid(0)
+    r = radius(d["vol"])
+    assert r.min() >= e[0] and r.max() < e[-1]
+    ids, action, other = g.removed(0)
+    assert len(ids) > 0
+    assert len(np.unique(ids)) == len(ids)
+    assert not np.isin(ids, d["id"]).any()
+    known = np.concatenate([pop.ids(), np.arange(pop.n_part + 1, g.next_id(0))])
+    coag = action == 2
+    assert np.isin(other[coag], known).all()
+    assert np.isin(ids, known).all()
