@@ -69,30 +69,36 @@ struct PW {  // device view of the parallel work arrays
   uint64_t *holes, *holes_sorted;  // (box << 32 | slot)
   int *tail_live;
   // accelerated per-particle path (try_per_particle_coag): one test slot per
-  // target particle, one record per slot of a batch
+  // target particle, one event record per target slot of a batch
   uint8_t *pair_acc;           // [NP] 0 per-set; 1 targets in (b1, c1); 2 targets in (b2, c2)
-  uint32_t *pair_accn;         // [NP + 1] target slots of accelerated pairs
+  uint32_t *pair_accn;         // [NP + 1] target slots (first accelerated pair of a list)
   uint32_t *acc_exc;           // [NP + 1] their exclusive scan
   int *trec;                   // [T_cap] record of a test slot, -1 for per-set tests
-  int *tns;                    // [T_cap] n_samp (+ ACC_WARN) of an attempt without effect
+  int *tns;                    // [T_cap] n_samp (+ ACC_WARN) of a target without effect
   int *ev_rec;                 // [T_cap] record of an event, -1 for per-set events
-  int acc_cap;                 // records per batch
-  double *r_vol;               // [n_spec][acc_cap] the virtual source particle
-  int *r_nprim, *r_ncomp;      // [acc_cap]
-  int *r_csrc;                 // [MAX_COMP][acc_cap]
-  double *r_ctime;             // [MAX_COMP][acc_cap]
-  double *r_least, *r_greatest;
-  int *r_nsamp, *r_ncoag, *r_ncopies, *r_nrm, *r_head, *r_warn;  // [acc_cap]
+  int acc_cap, pool_cap;
+  // event records [acc_cap]
+  int *r_nsamp, *r_ncoag, *r_nrm, *r_head, *r_ndup, *r_dhead, *r_flags, *r_err;
+  // particle pool [pool_cap]: entry r < acc_cap is the final target of
+  // record r; entries from acc_cap on are the copies (aero_state_dup_particle)
+  double *q_vol;               // [n_spec][pool_cap]
+  int *q_group, *q_nprim, *q_ncomp, *q_next;
+  int *q_csrc;                 // [MAX_COMP][pool_cap]
+  double *q_ctime;             // [MAX_COMP][pool_cap]
+  double *q_least, *q_greatest;
+  int *dup_top;                // [1] copy allocator of the round
   int *rm_buf;                 // chunks: [0] next chunk or -1, [1..RM_CHUNK) removed slots
   int *rm_top;                 // [1] chunk allocator of the round
   int rm_cap;
   uint32_t *misc;              // [4] batch bounds from k_batch_end
 };
 
-constexpr int ACC_CAP = 1 << 19;
+constexpr int ACC_CAP = 1 << 18;
+constexpr int DUP_CAP = 1 << 18;
 constexpr int RM_CHUNK = 16;
 constexpr int RM_CAP = 1 << 24;
 constexpr int ACC_WARN = 1 << 30;
+constexpr int RF_WARN = 1, RF_DIED = 2, RF_MERGED = 4;
 
 struct ParallelWork {
   int n_box = 0;
@@ -526,9 +532,8 @@ __global__ void k_par_pairs(BoxDesc *boxes, Arrays A, PW W, double del_t, uint64
         double n_spt;
         acc = accel_kind(B->w, edges, b1, b2, c1, c2, A.cc[ti], k_max, del_t, cnt, nb, &n_spt);
         if (acc) {
-          // one test slot per target particle (the targets' loop,
-          // src/coagulation.F90:220-237)
-          ns = (uint32_t)(acc == 1 ? n1 : n2);
+          // target slots go to the list's first accelerated pair
+          // (k_par_acc_first)
           atomicAdd((unsigned long long *)&B->n_accel_pairs, 1ull);
         } else {
           bool same = (b1 == b2) && (c1 == c2);
@@ -547,7 +552,7 @@ __global__ void k_par_pairs(BoxDesc *boxes, Arrays A, PW W, double del_t, uint64
   }
   W.pair_n[(int64_t)box * W.P_stride + q] = ns;
   W.pair_acc[(int64_t)box * W.P_stride + q] = (uint8_t)acc;
-  W.pair_accn[(int64_t)box * W.P_stride + q] = acc ? ns : 0u;
+  W.pair_accn[(int64_t)box * W.P_stride + q] = 0u;
 }
 
 __device__ inline int64_t find_pair(const uint32_t *exc, int64_t n, uint32_t t) {
@@ -692,46 +697,76 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
 }
 
 // ------------------------------------------------ accelerated target slots
-// try_per_particle_coag (src/coagulation.F90:150-237) runs its targets
-// e = n_target .. 1; here slot j of an accelerated pair is target entry
-// e = n_target - j, so the slots keep that order.  One thread per target
-// samples the virtual source particle (sample_source_particle, :333-517)
-// from the step's source list, skipping particles already consumed this
-// step and those it removed itself; it claims the target and every source
-// it removes.  The merge itself (coag_target_with_source, :522-582) is
-// applied when the event wins.
-struct AccPair {
-  int box, c1, c2, b1, b2, bt, ct, bs, cs, cc, nb, nc;
-  int n_t, n_s, st, ss, e;
-  double k_max;
-  uint32_t site;
-};
+// try_per_particle_coag (src/coagulation.F90:150-241).  In the reference
+// loop a target particle is visited by every accelerated pair that has its
+// current (bin, class) on the target side, in loop order; a target that
+// grows into the next bin is visited again by the later pairs of that bin.
+// Here each target particle is one test slot, placed in the first
+// accelerated pair of its step-start list (targets e = n .. 1, :220-237).
+// Its thread walks the remaining pairs in loop order, following the
+// target's current bin; in each it samples a virtual source particle
+// (sample_source_particle, :333-517) and merges it in
+// (coag_target_with_source, :522-582, with aero_state_dup_particle,
+// src/aero_state.F90:488-539).  The final target, its copies and the
+// removed sources are applied when the event wins all of its claims.
 
-__device__ inline AccPair acc_pair(BoxDesc *boxes, const Arrays &A, const PW &W, int64_t k,
-                                   uint32_t t) {
-  AccPair a;
-  a.box = (int)(k / W.P_stride);
-  decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &a.c1, &a.c2, &a.b1, &a.b2);
-  const BoxDesc *B = &boxes[a.box];
-  a.nb = B->n_bin;
-  a.nc = B->w.n_class;
-  const int64_t ti = table_index(A, a.box, a.nb, a.nc, a.b1, a.b2, a.c1, a.c2);
-  a.k_max = A.kw[ti];
-  a.cc = A.cc[ti];
-  if (W.pair_acc[k] == 1) {
-    a.bt = a.b1; a.ct = a.c1; a.bs = a.b2; a.cs = a.c2;
-  } else {
-    a.bt = a.b2; a.ct = a.c2; a.bs = a.b1; a.cs = a.c1;
+__device__ inline int64_t pair_q(const PW &W, int c1, int c2, int b1, int b2) {
+  return ((int64_t)(c1 - 1) * W.tab_classes + (c2 - 1)) * W.max_bin * W.max_bin +
+         (int64_t)(b1 - 1) * W.max_bin + (b2 - 1);
+}
+
+// first i in [lo, hi] with q0 + (i - 1) * stride > after, or hi + 1
+__device__ inline int first_above(int64_t after, int64_t q0, int64_t stride, int lo, int hi) {
+  if (after < q0) return lo;
+  int64_t i = (after - q0) / stride + 2;
+  return i < lo ? lo : (i > hi + 1 ? hi + 1 : (int)i);
+}
+
+// The next pair after `after` (loop order) that is accelerated with the
+// list (bt, ct) on its target side, or -1.  Candidates in loop order:
+// (ct, c2 < ct, bt, b2) side 1; (ct, ct, b1 < bt, bt) side 2;
+// (ct, ct, bt, b2 > bt) side 1; (c1 > ct, ct, b1, bt) side 2.
+__device__ inline int64_t next_acc_pair(const PW &W, const uint8_t *pacc, int nb, int nc, int ct,
+                                        int bt, int64_t after) {
+  const int64_t mb = W.max_bin;
+  for (int c2 = 1; c2 < ct; ++c2) {
+    int64_t q0 = pair_q(W, ct, c2, bt, 1);
+    for (int b2 = first_above(after, q0, 1, 1, nb); b2 <= nb; ++b2)
+      if (pacc[q0 + b2 - 1] == 1) return q0 + b2 - 1;
   }
-  const int64_t lb = (int64_t)a.box * W.K_stride;
-  const int kt = par_key(a.bt, a.ct, a.nb), ks = par_key(a.bs, a.cs, a.nb);
-  a.n_t = W.lcnt[lb + kt];
-  a.n_s = W.lcnt[lb + ks];
-  a.st = W.lstart[lb + kt];
-  a.ss = W.lstart[lb + ks];
-  a.site = pair_site(a.nb, a.nc, a.b1, a.b2, a.c1, a.c2);
-  a.e = a.n_t - (int)(t - W.pair_exc[k]);
-  return a;
+  {
+    int64_t q0 = pair_q(W, ct, ct, 1, bt);
+    for (int b1 = first_above(after, q0, mb, 1, bt - 1); b1 < bt; ++b1)
+      if (pacc[q0 + (b1 - 1) * mb] == 2) return q0 + (b1 - 1) * mb;
+  }
+  {
+    int64_t q0 = pair_q(W, ct, ct, bt, 1);
+    for (int b2 = first_above(after, q0, 1, bt + 1, nb); b2 <= nb; ++b2)
+      if (pacc[q0 + b2 - 1] == 1) return q0 + b2 - 1;
+  }
+  for (int c1 = ct + 1; c1 <= nc; ++c1) {
+    int64_t q0 = pair_q(W, c1, ct, 1, bt);
+    for (int b1 = first_above(after, q0, mb, 1, nb); b1 <= nb; ++b1)
+      if (pacc[q0 + (b1 - 1) * mb] == 2) return q0 + (b1 - 1) * mb;
+  }
+  return -1;
+}
+
+// the target slots of a list go to its first accelerated pair
+__global__ void k_par_acc_first(BoxDesc *boxes, PW W) {
+  const int box = blockIdx.y;
+  const int key = blockIdx.x * blockDim.x + threadIdx.x;
+  const BoxDesc *B = &boxes[box];
+  const int nb = B->n_bin, nc = B->w.n_class;
+  if (B->error || key >= nb * nc) return;
+  const int bt = key % nb + 1, ct = key / nb + 1;
+  const int n_t = W.lcnt[(int64_t)box * W.K_stride + par_key(bt, ct, nb)];
+  if (n_t == 0) return;
+  const int64_t pb = (int64_t)box * W.P_stride;
+  int64_t q = next_acc_pair(W, W.pair_acc + pb, nb, nc, ct, bt, -1);
+  if (q < 0) return;
+  W.pair_n[pb + q] = (uint32_t)n_t;
+  W.pair_accn[pb + q] = (uint32_t)n_t;
 }
 
 // the stream kinds of a target's draws; re-draws use their own kinds
@@ -753,6 +788,34 @@ __device__ inline double rec_vol(int n_spec, const Rec &r) {
   double v = 0.0;
   for (int s = 0; s < n_spec; ++s) v += r.vol[s];
   return v;
+}
+
+__device__ inline double rec_density(const Arrays &A, const Rec &r) {
+  double m = 0.0;
+  for (int s = 0; s < A.n_spec; ++s) m += r.vol[s] * A.density[s];
+  return m / rec_vol(A.n_spec, r);
+}
+
+// weighted kernel of two detached records (as weighted_kernel64)
+__device__ inline int weighted_kernel_rec(const Arrays &A, const Weights &w, const Env &env,
+                                          int type, const Rec &a, const Rec &b, int c1, int c2,
+                                          int cc, double *k_out) {
+  double v1 = rec_vol(A.n_spec, a), v2 = rec_vol(A.n_spec, b);
+  double k = 0.0;
+  int err = 0;
+  switch (type) {
+    case KERNEL_ADDITIVE: k = env.beta * (v1 + v2); break;
+    case KERNEL_CONSTANT: k = C_BETA_0; break;
+    case KERNEL_ZERO: k = 0.0; break;
+    case KERNEL_BROWN:
+      k = kernel_brown_helper(v1, rec_density(A, a), v2, rec_density(A, b), env.temp,
+                              env.pressure);
+      break;
+    case KERNEL_SEDI: err = kernel_sedi_helper(v1, v2, &k); break;
+    default: err = 200724934;
+  }
+  *k_out = k * coag_num_conc_factor(w, vol2rad(v1), vol2rad(v2), c1, c2, cc);
+  return err;
 }
 
 // the sources a target removes, in a chain of RM_CHUNK-int chunks
@@ -805,49 +868,168 @@ __device__ inline void rm_walk(const PW &W, int r, F f) {
   }
 }
 
-__device__ inline void rec_store_source(const PW &W, int n_spec, int r, const Rec &s) {
-  for (int k = 0; k < n_spec; ++k) W.r_vol[(int64_t)k * W.acc_cap + r] = s.vol[k];
-  W.r_nprim[r] = s.nprim;
-  W.r_ncomp[r] = s.ncomp;
+__device__ inline void pool_store(const PW &W, int n_spec, int i, const Rec &s) {
+  for (int k = 0; k < n_spec; ++k) W.q_vol[(int64_t)k * W.pool_cap + i] = s.vol[k];
+  W.q_group[i] = s.group;
+  W.q_nprim[i] = s.nprim;
+  W.q_ncomp[i] = s.ncomp;
   for (int c = 0; c < MAX_COMP; ++c) {
-    W.r_csrc[(int64_t)c * W.acc_cap + r] = s.csrc[c];
-    W.r_ctime[(int64_t)c * W.acc_cap + r] = s.ctime[c];
+    W.q_csrc[(int64_t)c * W.pool_cap + i] = s.csrc[c];
+    W.q_ctime[(int64_t)c * W.pool_cap + i] = s.ctime[c];
   }
-  W.r_least[r] = s.least;
-  W.r_greatest[r] = s.greatest;
+  W.q_least[i] = s.least;
+  W.q_greatest[i] = s.greatest;
 }
 
-__device__ inline void rec_load_source(const PW &W, int n_spec, int r, Rec &s) {
-  for (int k = 0; k < n_spec; ++k) s.vol[k] = W.r_vol[(int64_t)k * W.acc_cap + r];
-  s.id = 0;
-  s.group = 0;
-  s.klass = 0;
-  s.nprim = W.r_nprim[r];
-  s.ncomp = W.r_ncomp[r];
+// a pool entry as a particle of class `klass` with identifier `id`
+__device__ inline void pool_load(const PW &W, int n_spec, int i, int klass, int64_t id, Rec &s) {
+  for (int k = 0; k < n_spec; ++k) s.vol[k] = W.q_vol[(int64_t)k * W.pool_cap + i];
+  s.id = id;
+  s.group = W.q_group[i];
+  s.klass = klass;
+  s.nprim = W.q_nprim[i];
+  s.ncomp = W.q_ncomp[i];
   for (int c = 0; c < MAX_COMP; ++c) {
-    s.csrc[c] = W.r_csrc[(int64_t)c * W.acc_cap + r];
-    s.ctime[c] = W.r_ctime[(int64_t)c * W.acc_cap + r];
+    s.csrc[c] = W.q_csrc[(int64_t)c * W.pool_cap + i];
+    s.ctime[c] = W.q_ctime[(int64_t)c * W.pool_cap + i];
   }
-  s.least = W.r_least[r];
-  s.greatest = W.r_greatest[r];
+  s.least = W.q_least[i];
+  s.greatest = W.q_greatest[i];
 }
 
-// coag_target_with_source up to the copy count (src/coagulation.F90:
-// 522-582 with aero_state_dup_particle's prob_round, src/aero_state.F90:
-// 488-503): the merged target, its bin and the number of copies
-__device__ inline int acc_merge(int n_spec, const Weights &w, Stream &fin, const Rec &tp,
-                                const Rec &src, int cc, int nb, double log_lo, double log_hi,
-                                Rec &np, int *n_copies, int *new_bin) {
-  double old_nc = nc_at_radius(w, tp.klass, vol2rad(rec_vol(n_spec, tp)));
-  int err = par_rec_coagulate(n_spec, fin, tp, src, np);
-  if (err) return err;
-  np.id = tp.id;
-  double r_new = vol2rad(rec_vol(n_spec, np));
-  np.group = rand_group_w(fin, w, cc, r_new);
-  np.klass = cc;
-  *new_bin = bin_find_log(nb, log_lo, log_hi, r_new);
-  double new_nc = nc_at_radius(w, np.klass, r_new);
-  *n_copies = prob_round(fin, old_nc / new_nc);
+// sample_source_particle (src/coagulation.F90:333-517) against the running
+// target, drawing from the step's source list (list start ss, n_s entries)
+// the particles still present that this target has not removed; returns
+// an error code, 0 on success.  Sources it removes are pushed to `chain`
+// and claimed with index t.
+__device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B, int type,
+                                        const Rec &target, int ct, int cs, double nc_src_min,
+                                        int64_t sbase, int n_s, double n_spt,
+                                        double accept_factor, Stream &aps, Stream &smp,
+                                        Stream &fin, uint32_t t, RmChain &chain, int *n_samp_out,
+                                        int *n_coag_out, int *warn, Rec &source) {
+  const int ns = A.n_spec;
+  const Weights &w = B->w;
+  const Env env = B->env;
+  *n_samp_out = 0;
+  *n_coag_out = 0;
+  const double nc_target = nc_at_radius(w, target.klass, vol2rad(rec_vol(ns, target)));
+  const double prm = nc_target / nc_src_min;
+  if (!(prm <= 1.0)) return 653606684;
+  const int n_remove = rand_poisson(aps, prm * n_spt);
+  const int n_extra = rand_poisson(aps, (1.0 - prm) * n_spt);
+  const int n_total = n_remove + n_extra;
+  int n_avg = 0, n_samp = 0, n_rm_here = 0;
+  double prob_coag_tot = 0.0;
+  double vol_sq[MAX_SPEC], vol_cv[MAX_SPEC];
+  for (int q = 0; q < ns; ++q) {
+    vol_sq[q] = 0.0;
+    vol_cv[q] = 0.0;
+    source.vol[q] = 0.0;
+  }
+  source.id = 0;
+  source.group = source.klass = source.nprim = source.ncomp = 0;
+  for (int c = 0; c < MAX_COMP; ++c) {
+    source.csrc[c] = 0;
+    source.ctime[c] = 0.0;
+  }
+  source.least = source.greatest = 0.0;
+  Rec pi;
+  for (int i_samp = 1; i_samp <= n_total; ++i_samp) {
+    if (n_rm_here >= n_s) break;  // every source removed
+    if (n_samp > n_remove && n_avg >= 2) {
+      double cvmax = 0.0;
+      for (int q = 0; q < ns; ++q) {
+        double vm = source.vol[q] / (double)n_avg;
+        if (vm > 0.0) vol_cv[q] = sqrt(vol_sq[q] / (double)n_avg - vm * vm) / vm;
+        if (q == 0 || vol_cv[q] > cvmax) cvmax = vol_cv[q];
+      }
+      double m95 = cvmax / sqrt((double)n_avg) * student_t_95_coeff(n_avg);
+      if (m95 < 0.1) break;
+    }
+    // a uniform draw among the sources still present
+    int ps = -1;
+    for (int a = 0; a < MAX_REDRAW; ++a) {
+      int c = rand_int(smp, n_s);
+      int q = W.perm[sbase + c - 1];
+      if (W.status[q] == ST_LIVE && !rm_has(W, chain, q)) {
+        ps = q;
+        break;
+      }
+    }
+    if (ps < 0) {
+      atomicAdd(&B->n_dropped, 1);
+      break;
+    }
+    n_samp++;
+    load_rec(A, ps, pi);
+    double kk;
+    int err = weighted_kernel_rec(A, w, env, type, pi, target, cs, ct, ct, &kk);
+    if (err) return err;
+    double prob_coag = kk * accept_factor;
+    prob_coag_tot += prob_coag;
+    if (smp.uniform() < prob_coag) {
+      n_avg++;
+      err = par_rec_coagulate(ns, smp, source, pi, source);
+      if (err) return err;
+      for (int q = 0; q < ns; ++q) vol_sq[q] += pi.vol[q] * pi.vol[q];
+      if (i_samp <= n_remove) {
+        double nc_i = nc_at_radius(w, pi.klass, vol2rad(rec_vol(ns, pi)));
+        double prob_remove_i = nc_target / nc_i;
+        if (smp.uniform() < prob_remove_i / prm) {
+          if (!rm_push(W, chain, ps)) return ERR_CAPACITY;
+          n_rm_here++;
+          atomicMin(&W.claim[ps], t);
+        }
+      }
+    }
+  }
+  *n_samp_out = n_samp;
+  if (n_avg == 0) return 0;
+  double prob_coag_mean = prob_coag_tot / (double)n_samp;
+  if (!(prob_coag_mean <= 1.0)) *warn = 1;
+  int n_coag = rand_binomial(fin, n_total, prob_coag_mean);
+  for (int q = 0; q < ns; ++q) source.vol[q] = source.vol[q] * ((double)n_coag / (double)n_avg);
+  source.nprim = prob_round(fin, (double)source.nprim * ((double)n_coag / (double)n_avg));
+  const int n_comp = source.ncomp;
+  const int n_comp_new = source.nprim < MAX_COMP ? source.nprim : MAX_COMP;
+  int sample[MAX_COMP];
+  int cs_src[MAX_COMP];
+  double ct_src[MAX_COMP];
+  for (int c = 0; c < MAX_COMP; ++c) {
+    cs_src[c] = source.csrc[c];
+    ct_src[c] = source.ctime[c];
+  }
+  if (n_comp_new < n_comp) {
+    sample_without_replacement(fin, n_comp_new, n_comp, sample);
+    for (int c = 0; c < n_comp_new; ++c) {
+      source.csrc[c] = cs_src[sample[c] - 1];
+      source.ctime[c] = ct_src[sample[c] - 1];
+    }
+    source.ncomp = n_comp_new;
+  } else if (n_comp < MAX_COMP) {
+    int i_comp = 1;
+    while (i_comp <= n_comp_new) {
+      if (i_comp + n_comp <= n_comp_new) {
+        for (int c = 1; c <= n_comp; ++c) {
+          source.csrc[i_comp + c - 2] = cs_src[c - 1];
+          source.ctime[i_comp + c - 2] = ct_src[c - 1];
+        }
+        i_comp += n_comp;
+      } else {
+        int sample_size = n_comp_new - i_comp + 1;
+        sample_without_replacement(fin, sample_size, n_comp, sample);
+        for (int c = 1; c <= sample_size; ++c) {
+          source.csrc[i_comp + c - 2] = cs_src[sample[c - 1] - 1];
+          source.ctime[i_comp + c - 2] = ct_src[sample[c - 1] - 1];
+        }
+        i_comp = n_comp_new + 1;
+      }
+    }
+    source.ncomp = n_comp_new;
+  }
+  if (source.nprim < source.ncomp) return 808144130;
+  *n_coag_out = n_coag;
   return 0;
 }
 
@@ -882,182 +1064,120 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
   W.tp2[i] = -1;
   W.trec[i] = rec;
   W.tns[i] = 0;
-  const AccPair P = acc_pair(boxes, A, W, k, t);
-  BoxDesc *B = &boxes[P.box];
+  const int box = (int)(k / W.P_stride);
+  BoxDesc *B = &boxes[box];
   if (B->error) return;
-  const int64_t base = B->off;
-  const int p = W.perm[base + P.st + P.e - 1];
-  if (W.status[p] != ST_LIVE) return;  // consumed earlier in this step
-  W.tp1[i] = p;
+  const int nb = B->n_bin, nc = B->w.n_class;
+  const int64_t lb = (int64_t)box * W.K_stride;
+  const int64_t pb = (int64_t)box * W.P_stride;
+  const uint8_t *pacc = W.pair_acc + pb;
+  const double *edges = A.edges + (int64_t)box * (A.max_bin + 1);
+  int c1, c2, b1, b2;
+  int64_t q = k - pb;
+  decode_pair(q, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
+  int b_cur = pacc[q] == 1 ? b1 : b2;
+  const int ct = pacc[q] == 1 ? c1 : c2;
+  {
+    const int n_t = W.lcnt[lb + par_key(b_cur, ct, nb)];
+    const int e = n_t - (int)(t - W.pair_exc[k]);
+    const int p0 = W.perm[B->off + W.lstart[lb + par_key(b_cur, ct, nb)] + e - 1];
+    if (W.status[p0] != ST_LIVE) return;  // consumed earlier in this step
+    W.tp1[i] = p0;
+  }
+  const int p = W.tp1[i];
   const int ns = A.n_spec;
   const Weights &w = B->w;
-  const Env env = B->env;
-  const double n_spt = P.k_max * del_t * (double)P.n_s;
-  const double accept_factor = 1.0 / P.k_max;
-  Rec target, source, pi;
+  const uint32_t ordinal = (uint32_t)(p - B->off) + 1;
+  Rec target, source;
   load_rec(A, p, target);
-  const int64_t target_id = target.id;
-  Stream aps, smp, fin;
-  aps.open(seed, P.box, step, P.site, (uint32_t)P.e, acc_kind(RNG_ACC_POISSON, round));
-  smp.open(seed, P.box, step, P.site, (uint32_t)P.e, acc_kind(RNG_ACC_SAMPLE, round));
-  fin.open(seed, P.box, step, P.site, (uint32_t)P.e, acc_kind(RNG_ACC_FINISH, round));
-  const double nc_target = nc_at_radius(w, target.klass, vol2rad(rec_vol(ns, target)));
-  const double nc_src_min = nc_at_radius(w, P.cs, A.edges[(int64_t)P.box * (A.max_bin + 1) + P.bs - 1]);
-  const double prm = nc_target / nc_src_min;
-  if (!(prm <= 1.0)) {
-    atomicExch(&B->error, 653606684);
-    return;
-  }
-  const int n_remove = rand_poisson(aps, prm * n_spt);
-  const int n_extra = rand_poisson(aps, (1.0 - prm) * n_spt);
-  const int n_total = n_remove + n_extra;
-  int n_avg = 0, n_samp = 0;
-  double prob_coag_tot = 0.0;
-  double vol_sq[MAX_SPEC], vol_cv[MAX_SPEC];
-  for (int q = 0; q < ns; ++q) {
-    vol_sq[q] = 0.0;
-    vol_cv[q] = 0.0;
-    source.vol[q] = 0.0;
-  }
-  source.id = 0;
-  source.group = source.klass = source.nprim = source.ncomp = 0;
-  for (int c = 0; c < MAX_COMP; ++c) {
-    source.csrc[c] = 0;
-    source.ctime[c] = 0.0;
-  }
-  source.least = source.greatest = 0.0;
   RmChain chain{-1, -1, RM_CHUNK, 0, 0ull};
-  const int64_t sbase = base + P.ss;
-  for (int i_samp = 1; i_samp <= n_total; ++i_samp) {
-    if (chain.n >= P.n_s) break;  // every source removed
-    if (n_samp > n_remove && n_avg >= 2) {
-      double cvmax = 0.0;
-      for (int q = 0; q < ns; ++q) {
-        double vm = source.vol[q] / (double)n_avg;
-        if (vm > 0.0) vol_cv[q] = sqrt(vol_sq[q] / (double)n_avg - vm * vm) / vm;
-        if (q == 0 || vol_cv[q] > cvmax) cvmax = vol_cv[q];
-      }
-      double m95 = cvmax / sqrt((double)n_avg) * student_t_95_coeff(n_avg);
-      if (m95 < 0.1) break;
-    }
-    // a uniform draw among the sources still present
-    int ps = -1;
-    for (int a = 0; a < MAX_REDRAW; ++a) {
-      int c = rand_int(smp, P.n_s);
-      int q = W.perm[sbase + c - 1];
-      if (W.status[q] == ST_LIVE && !rm_has(W, chain, q)) {
-        ps = q;
-        break;
-      }
-    }
-    if (ps < 0) {
-      atomicAdd(&B->n_dropped, 1);
-      break;
-    }
-    n_samp++;
-    load_rec(A, ps, pi);
-    double kk;
-    int err = weighted_kernel64(A, w, env, type, ps, p, P.cs, P.ct, P.ct, &kk);
+  int n_samp_tot = 0, n_coag_tot = 0, warn = 0, flags = 0, err_code = 0;
+  int n_dup = 0, dup_head = -1, dup_tail = -1;
+  while (q >= 0) {
+    decode_pair(q, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
+    const int kind = pacc[q];
+    const int bs = kind == 1 ? b2 : b1, cs = kind == 1 ? c2 : c1;
+    const int64_t ti = table_index(A, box, nb, nc, b1, b2, c1, c2);
+    const double k_max = A.kw[ti];
+    const int cc = A.cc[ti];
+    const int ks = par_key(bs, cs, nb);
+    const int n_s = W.lcnt[lb + ks];
+    const uint32_t site = pair_site(nb, nc, b1, b2, c1, c2);
+    Stream aps, smp, fin;
+    aps.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_POISSON, round));
+    smp.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_SAMPLE, round));
+    fin.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_FINISH, round));
+    int n_samp, n_coag;
+    int err = acc_sample_source(A, W, B, type, target, ct, cs, nc_at_radius(w, cs, edges[bs - 1]),
+                                B->off + W.lstart[lb + ks], n_s, k_max * del_t * (double)n_s,
+                                1.0 / k_max, aps, smp, fin, t, chain, &n_samp, &n_coag, &warn,
+                                source);
     if (err) {
       atomicExch(&B->error, err);
       return;
     }
-    double prob_coag = kk * accept_factor;
-    prob_coag_tot += prob_coag;
-    if (smp.uniform() < prob_coag) {
-      n_avg++;
-      err = par_rec_coagulate(ns, smp, source, pi, source);
-      if (err) {
-        atomicExch(&B->error, err);
-        return;
-      }
-      for (int q = 0; q < ns; ++q) vol_sq[q] += pi.vol[q] * pi.vol[q];
-      if (i_samp <= n_remove) {
-        double nc_i = nc_at_radius(w, pi.klass, vol2rad(rec_vol(ns, pi)));
-        double prob_remove_i = nc_target / nc_i;
-        if (smp.uniform() < prob_remove_i / prm) {
-          if (!rm_push(W, chain, ps)) {
-            atomicExch(&B->error, ERR_CAPACITY);
-            return;
-          }
-          atomicMin(&W.claim[ps], t);
-        }
-      }
-    }
-  }
-  int n_coag = 0, warn = 0, n_copies = -1;
-  if (n_avg > 0) {
-    double prob_coag_mean = prob_coag_tot / (double)n_samp;
-    if (!(prob_coag_mean <= 1.0)) warn = 1;
-    n_coag = rand_binomial(fin, n_total, prob_coag_mean);
-    for (int q = 0; q < ns; ++q) source.vol[q] = source.vol[q] * ((double)n_coag / (double)n_avg);
-    source.nprim = prob_round(fin, (double)source.nprim * ((double)n_coag / (double)n_avg));
-    const int n_comp = source.ncomp;
-    const int n_comp_new = source.nprim < MAX_COMP ? source.nprim : MAX_COMP;
-    int sample[MAX_COMP];
-    int cs_src[MAX_COMP];
-    double ct_src[MAX_COMP];
-    for (int c = 0; c < MAX_COMP; ++c) {
-      cs_src[c] = source.csrc[c];
-      ct_src[c] = source.ctime[c];
-    }
-    if (n_comp_new < n_comp) {
-      sample_without_replacement(fin, n_comp_new, n_comp, sample);
-      for (int c = 0; c < n_comp_new; ++c) {
-        source.csrc[c] = cs_src[sample[c] - 1];
-        source.ctime[c] = ct_src[sample[c] - 1];
-      }
-      source.ncomp = n_comp_new;
-    } else if (n_comp < MAX_COMP) {
-      int i_comp = 1;
-      while (i_comp <= n_comp_new) {
-        if (i_comp + n_comp <= n_comp_new) {
-          for (int c = 1; c <= n_comp; ++c) {
-            source.csrc[i_comp + c - 2] = cs_src[c - 1];
-            source.ctime[i_comp + c - 2] = ct_src[c - 1];
-          }
-          i_comp += n_comp;
-        } else {
-          int sample_size = n_comp_new - i_comp + 1;
-          sample_without_replacement(fin, sample_size, n_comp, sample);
-          for (int c = 1; c <= sample_size; ++c) {
-            source.csrc[i_comp + c - 2] = cs_src[sample[c - 1] - 1];
-            source.ctime[i_comp + c - 2] = ct_src[sample[c - 1] - 1];
-          }
-          i_comp = n_comp_new + 1;
-        }
-      }
-      source.ncomp = n_comp_new;
-    }
-    if (source.nprim < source.ncomp) {
-      atomicExch(&B->error, 808144130);
-      return;
-    }
+    n_samp_tot += n_samp;
+    n_coag_tot += n_coag;
     if (n_coag > 0) {
+      // coag_target_with_source
       Stream ft;
-      ft.open(seed, P.box, step, P.site, (uint32_t)P.e, acc_kind(RNG_ACC_TARGET, round));
+      ft.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_TARGET, round));
+      const double old_nc = nc_at_radius(w, target.klass, vol2rad(rec_vol(ns, target)));
       Rec np;
-      int nbin;
-      int err = acc_merge(ns, w, ft, target, source, P.cc, P.nb, B->log_lo, B->log_hi, np,
-                          &n_copies, &nbin);
+      err = par_rec_coagulate(ns, ft, target, source, np);
       if (err) {
         atomicExch(&B->error, err);
         return;
       }
+      np.id = target.id;
+      const double r_new = vol2rad(rec_vol(ns, np));
+      np.group = rand_group_w(ft, w, cc, r_new);
+      np.klass = cc;
+      target = np;
+      flags |= RF_MERGED;
+      const int new_bin = bin_find_log(nb, B->log_lo, B->log_hi, r_new);
+      if (new_bin < 1 || new_bin > nb) {
+        err_code = 765620746;
+        break;
+      }
+      const int n_copies = prob_round(ft, old_nc / nc_at_radius(w, np.klass, r_new));
+      if (n_copies == 0) {
+        flags |= RF_DIED;
+        break;
+      }
+      for (int c = 1; c < n_copies; ++c) {
+        const int d = W.acc_cap + atomicAdd(W.dup_top, 1);
+        if (d >= W.pool_cap) {
+          atomicExch(&B->error, ERR_CAPACITY);
+          return;
+        }
+        Rec cp = target;
+        cp.group = rand_group_w(ft, w, target.klass, r_new);
+        pool_store(W, ns, d, cp);
+        W.q_next[d] = -1;
+        if (dup_tail >= 0) W.q_next[dup_tail] = d;
+        else dup_head = d;
+        dup_tail = d;
+        n_dup++;
+      }
+      b_cur = new_bin;
     }
+    q = next_acc_pair(W, pacc, nb, nc, ct, b_cur, q);
   }
-  (void)target_id;
-  if (n_coag == 0 && chain.n == 0) {
-    W.tns[i] = n_samp | (warn ? ACC_WARN : 0);
+  if (warn) flags |= RF_WARN;
+  if (n_coag_tot == 0 && chain.n == 0 && err_code == 0) {
+    W.tns[i] = n_samp_tot | (warn ? ACC_WARN : 0);
     return;
   }
-  if (n_coag > 0) rec_store_source(W, ns, rec, source);
-  W.r_nsamp[rec] = n_samp;
-  W.r_ncoag[rec] = n_coag;
-  W.r_ncopies[rec] = n_copies;
+  if (flags & RF_MERGED) pool_store(W, ns, rec, target);
+  W.r_nsamp[rec] = n_samp_tot;
+  W.r_ncoag[rec] = n_coag_tot;
   W.r_nrm[rec] = chain.n;
   W.r_head[rec] = chain.head;
-  W.r_warn[rec] = warn;
+  W.r_ndup[rec] = n_dup;
+  W.r_dhead[rec] = dup_head;
+  W.r_flags[rec] = flags;
+  W.r_err[rec] = err_code;
   W.acc[i] = 1;
   atomicMin(&W.claim[p], t);
 }
@@ -1175,9 +1295,8 @@ __global__ void k_par_resolve(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, 
     // a target event wins when it holds the target and every removed source
     bool win = W.claim[p1] == t;
     rm_walk(W, r, [&](int q) { win = win && W.claim[q] == t; });
-    int nc = W.r_ncopies[r], nrm = W.r_nrm[r];
-    int dups = (win && nc > 1) ? nc - 1 : 0;
-    int gone = win ? nrm + (nc == 0 ? 1 : 0) : 0;
+    int dups = win ? W.r_ndup[r] : 0;
+    int gone = win ? W.r_nrm[r] + ((W.r_flags[r] & RF_DIED) ? 1 : 0) : 0;
     W.ev_out[e] = win ? O_WIN : 0;
     W.f_app[e] = dups;
     W.f_id[e] = dups;
@@ -1245,24 +1364,26 @@ __device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc 
   if (want_terms) W.terms[slot] = particle_terms(type, v, m / v, B->env);
 }
 
-// coag_target_with_source and aero_state_dup_particle for a winning target
-// event (src/coagulation.F90:522-582, src/aero_state.F90:488-539): removed
-// sources become holes with a coagulation record naming the target, the
-// target is merged in place and its copies appended, or, when no copy is
-// kept, the target is removed with a weighting record.
+// a winning target event (coag_target_with_source and
+// aero_state_dup_particle, src/coagulation.F90:522-582,
+// src/aero_state.F90:488-539): removed sources become holes with a
+// coagulation record naming the target, the merged target replaces the
+// old one in place, its copies are appended with new identifiers, and a
+// target that kept no copy is removed with a weighting record.
 __device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, int n_box, int e,
-                                 uint64_t seed, uint32_t step, int round, int type,
-                                 int want_terms, int64_t hole_base) {
+                                 int type, int want_terms, int64_t hole_base) {
   const uint32_t t = W.ev_t[e];
-  const int64_t k = find_pair(W.pair_exc, (int64_t)n_box * W.P_stride, t);
-  const AccPair P = acc_pair(boxes, A, W, k, t);
-  BoxDesc *B = &boxes[P.box];
   const int r = W.ev_rec[e];
   const int64_t p = W.ev_p1[e];
-  int *live = W.live + (int64_t)P.box * W.K_stride;
-  const int first = W.box_first[P.box];
+  const int box = W.ev_box[e];
+  BoxDesc *B = &boxes[box];
+  (void)n_box;
+  int *live = W.live + (int64_t)box * W.K_stride;
+  const int first = W.box_first[box];
   const int ns = A.n_spec;
+  const int nb = B->n_bin;
   const int64_t target_id = A.id[p];
+  const int flags = W.r_flags[r];
   int64_t hrank = hole_base + W.s_hole[e];
   int64_t irank = B->n_info + (W.s_info[e] - W.s_info[first]);
   auto info = [&](int64_t id, int action, int64_t other) {
@@ -1277,7 +1398,7 @@ __device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, i
     W.dead_t[q] = t;
     atomicSub(&live[W.key[q]], 1);
     W.status[q] = ST_HOLE;
-    W.holes[hrank++] = ((uint64_t)P.box << 32) | (uint64_t)(q - B->off);
+    W.holes[hrank++] = ((uint64_t)box << 32) | (uint64_t)(q - B->off);
     info(A.id[q], 2, target_id);
   });
   const unsigned long long n_samp = (unsigned long long)W.r_nsamp[r];
@@ -1286,51 +1407,37 @@ __device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, i
   atomicAdd((unsigned long long *)&B->step_samp, n_samp);
   atomicAdd((unsigned long long *)&B->n_coag, n_coag);
   atomicAdd((unsigned long long *)&B->step_coag, n_coag);
-  if (W.r_warn[r]) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
-  if (n_coag == 0) return;
-  const int n_copies = W.r_ncopies[r];
-  Rec tp, src, np;
-  load_rec(A, p, tp);
-  rec_load_source(W, ns, r, src);
-  Stream ft;
-  ft.open(seed, P.box, step, P.site, (uint32_t)P.e, acc_kind(RNG_ACC_TARGET, round));
-  int n_check, nbin;
-  int err = acc_merge(ns, B->w, ft, tp, src, P.cc, P.nb, B->log_lo, B->log_hi, np, &n_check,
-                      &nbin);
-  if (err) {
-    atomicExch(&B->error, err);
-    return;
-  }
-  if (nbin < 1 || nbin > P.nb) {
-    atomicExch(&B->error, 765620746);
-    return;
-  }
+  if (flags & RF_WARN) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
+  if (W.r_err[r]) atomicExch(&B->error, W.r_err[r]);
+  if (!(flags & RF_MERGED)) return;
+  const int klass = A.klass[p];
+  Rec np;
+  pool_load(W, ns, r, klass, target_id, np);
   W.dead_t[p] = t;
   atomicSub(&live[W.key[p]], 1);
-  write_new_particle(A, W, B, p, np, P.nb, type, want_terms);
-  if (n_copies == 0) {
+  write_new_particle(A, W, B, p, np, nb, type, want_terms);
+  if (flags & RF_DIED) {
     W.status[p] = ST_HOLE;
-    W.holes[hrank++] = ((uint64_t)P.box << 32) | (uint64_t)(p - B->off);
-    info(np.id, 4, 0);
-    return;
+    W.holes[hrank++] = ((uint64_t)box << 32) | (uint64_t)(p - B->off);
+  } else {
+    atomicAdd(&live[W.key[p]], 1);
   }
-  atomicAdd(&live[W.key[p]], 1);
-  const double r_np = vol2rad(A.vtot[p]);
   int64_t id_rank = W.s_id[e] - W.s_id[first];
   int64_t app_rank = W.s_app[e] - W.s_app[first];
-  for (int c = 1; c < n_copies; ++c) {
-    np.id = (B->next_id + id_rank - 1) * B->id_stride + B->id_offset + 1;
+  for (int d = W.r_dhead[r]; d >= 0; d = W.q_next[d]) {
+    const int64_t id = (B->next_id + id_rank - 1) * B->id_stride + B->id_offset + 1;
     id_rank++;
-    np.group = rand_group_w(ft, B->w, np.klass, r_np);
+    pool_load(W, ns, d, klass, id, np);
     if (B->n_part + app_rank >= B->cap) {
       atomicExch(&B->error, ERR_CAPACITY);
       return;
     }
     const int64_t slot = B->off + B->n_part + app_rank;
     app_rank++;
-    write_new_particle(A, W, B, slot, np, P.nb, type, want_terms);
+    write_new_particle(A, W, B, slot, np, nb, type, want_terms);
     atomicAdd(&live[W.key[slot]], 1);
   }
+  if (flags & RF_DIED) info(target_id, 4, 0);
 }
 
 // apply the winners' merges (coagulate, src/coagulation.F90:928-985)
@@ -1342,7 +1449,7 @@ __global__ void k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, ui
   int o = W.ev_out[e];
   if (!(o & O_WIN)) return;
   if (W.ev_rec[e] >= 0) {
-    acc_apply(boxes, A, W, n_box, e, seed, step, round, type, want_terms, hole_base);
+    acc_apply(boxes, A, W, n_box, e, type, want_terms, hole_base);
     return;
   }
   uint32_t t = W.ev_t[e];
@@ -1618,34 +1725,36 @@ inline int parallel_init(ParallelWork &W, int n_box, int max_bin) {
 // the accelerated path's records, allocated when a step first has target slots
 inline void parallel_free_records(ParallelWork &W) {
   PW &d = W.d;
-  void *ptrs[] = {d.r_vol, d.r_nprim, d.r_ncomp, d.r_csrc, d.r_ctime, d.r_least, d.r_greatest,
-                  d.r_nsamp, d.r_ncoag, d.r_ncopies, d.r_nrm, d.r_head, d.r_warn, d.rm_buf,
-                  d.rm_top};
+  void *ptrs[] = {d.r_nsamp, d.r_ncoag, d.r_nrm, d.r_head, d.r_ndup, d.r_dhead, d.r_flags,
+                  d.r_err, d.q_vol, d.q_group, d.q_nprim, d.q_ncomp, d.q_next, d.q_csrc,
+                  d.q_ctime, d.q_least, d.q_greatest, d.dup_top, d.rm_buf, d.rm_top};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
-  d.r_vol = nullptr;
-  d.r_nprim = d.r_ncomp = d.r_csrc = nullptr;
-  d.r_ctime = d.r_least = d.r_greatest = nullptr;
-  d.r_nsamp = d.r_ncoag = d.r_ncopies = d.r_nrm = d.r_head = d.r_warn = nullptr;
-  d.rm_buf = d.rm_top = nullptr;
-  d.acc_cap = 0;
-  d.rm_cap = 0;
+  d.r_nsamp = d.r_ncoag = d.r_nrm = d.r_head = d.r_ndup = d.r_dhead = d.r_flags = d.r_err = nullptr;
+  d.q_vol = d.q_ctime = d.q_least = d.q_greatest = nullptr;
+  d.q_group = d.q_nprim = d.q_ncomp = d.q_next = d.q_csrc = nullptr;
+  d.dup_top = d.rm_buf = d.rm_top = nullptr;
+  d.acc_cap = d.pool_cap = d.rm_cap = 0;
 }
 
 inline int parallel_alloc_records(ParallelWork &W, int n_spec) {
   PW &d = W.d;
-  const int64_t c = ACC_CAP;
-  PCHECK(pmalloc(&d.r_vol, (size_t)n_spec * c));
-  PCHECK(pmalloc(&d.r_csrc, (size_t)MAX_COMP * c));
-  PCHECK(pmalloc(&d.r_ctime, (size_t)MAX_COMP * c));
-  int **iv[] = {&d.r_nprim, &d.r_ncomp, &d.r_nsamp, &d.r_ncoag, &d.r_ncopies, &d.r_nrm,
-                &d.r_head, &d.r_warn};
-  for (int **q : iv) PCHECK(pmalloc(q, c));
-  PCHECK(pmalloc(&d.r_least, c));
-  PCHECK(pmalloc(&d.r_greatest, c));
+  const int64_t c = ACC_CAP, pc = (int64_t)ACC_CAP + DUP_CAP;
+  int **rv[] = {&d.r_nsamp, &d.r_ncoag, &d.r_nrm, &d.r_head, &d.r_ndup, &d.r_dhead, &d.r_flags,
+                &d.r_err};
+  for (int **q : rv) PCHECK(pmalloc(q, c));
+  PCHECK(pmalloc(&d.q_vol, (size_t)n_spec * pc));
+  PCHECK(pmalloc(&d.q_csrc, (size_t)MAX_COMP * pc));
+  PCHECK(pmalloc(&d.q_ctime, (size_t)MAX_COMP * pc));
+  int **qv[] = {&d.q_group, &d.q_nprim, &d.q_ncomp, &d.q_next};
+  for (int **q : qv) PCHECK(pmalloc(q, pc));
+  PCHECK(pmalloc(&d.q_least, pc));
+  PCHECK(pmalloc(&d.q_greatest, pc));
+  PCHECK(pmalloc(&d.dup_top, 1));
   PCHECK(pmalloc(&d.rm_buf, RM_CAP));
   PCHECK(pmalloc(&d.rm_top, 1));
-  d.acc_cap = ACC_CAP;
+  d.acc_cap = (int)c;
+  d.pool_cap = (int)pc;
   d.rm_cap = RM_CAP;
   return 0;
 }
@@ -1839,6 +1948,7 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
   // 4. Poisson test counts per bin pair, laid out in reference order
   PLAUNCH(W, "k_par_pairs", k_par_pairs, dim3((unsigned)((d.P_stride + 255) / 256), n_box),
           dim3(256), 0, st, d_box, A, d, del_t, seed, step);
+  hipLaunchKernelGGL(k_par_acc_first, dim3((d.K_stride + 63) / 64, n_box), dim3(64), 0, st, d_box, d);
   PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.pair_n, d.pair_exc,
                                           (int)(W.NP + 1), st));
   PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.pair_accn, d.acc_exc,
@@ -1848,7 +1958,7 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
   PCHECK(hipStreamSynchronize(st));
   const uint32_t T = W.h_u32[0];
   const uint32_t T_acc = W.h_u32[1];
-  if (T_acc > 0 && d.r_vol == nullptr) {
+  if (T_acc > 0 && d.r_nsamp == nullptr) {
     int e = parallel_alloc_records(W, A.n_spec);
     if (e) return e;
   }
@@ -1877,7 +1987,10 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
     for (int round = 0; n > 0; ++round) {
       if (round >= MAX_ROUNDS) break;
       W.n_rounds++;
-      if (n_acc > 0) hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.rm_top, 1, 0);
+      if (n_acc > 0) {
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.rm_top, 1, 0);
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.dup_top, 1, 0);
+      }
       PLAUNCH(W, "k_par_tests", k_par_tests, dim3((n + 255) / 256), dim3(256), 0, st, d_box, A, d,
               n_box, type, seed, step, round, batch_t0, n, pend, precision, band, validation);
       if (n_acc > 0) {
