@@ -78,11 +78,13 @@ struct PW {  // device view of the parallel work arrays
   int *ev_rec;                 // [T_cap] record of an event, -1 for per-set events
   int acc_cap, pool_cap;
   // event records [acc_cap]
-  int *r_nsamp, *r_ncoag, *r_nrm, *r_head, *r_ndup, *r_dhead, *r_flags, *r_err;
+  int *r_nsamp, *r_ncoag, *r_nrm, *r_nchain, *r_head, *r_ndup, *r_ndead, *r_dhead, *r_flags,
+      *r_err;
   // particle pool [pool_cap]: entry r < acc_cap is the final target of
   // record r; entries from acc_cap on are the copies (aero_state_dup_particle)
   double *q_vol;               // [n_spec][pool_cap]
   int *q_group, *q_nprim, *q_ncomp, *q_next;
+  int *q_pos, *q_bin, *q_dead;  // a copy's walk start (pair, bin) and whether it died
   int *q_csrc;                 // [MAX_COMP][pool_cap]
   double *q_ctime;             // [MAX_COMP][pool_cap]
   double *q_least, *q_greatest;
@@ -841,12 +843,12 @@ __device__ inline bool rm_push(const PW &W, RmChain &c, int slot) {
   W.rm_buf[c.cur + c.fill] = slot;
   c.fill++;
   c.n++;
-  c.bloom |= rm_bit(slot);
+  if (slot >= 0) c.bloom |= rm_bit(slot);
   return true;
 }
 
 __device__ inline bool rm_has(const PW &W, const RmChain &c, int slot) {
-  if (!(c.bloom & rm_bit(slot))) return false;
+  if (slot < 0 || !(c.bloom & rm_bit(slot))) return false;
   int left = c.n;
   for (int ch = c.head; ch >= 0 && left > 0; ch = W.rm_buf[ch]) {
     int m = left < RM_CHUNK - 1 ? left : RM_CHUNK - 1;
@@ -857,15 +859,24 @@ __device__ inline bool rm_has(const PW &W, const RmChain &c, int slot) {
   return false;
 }
 
-// visit the removed sources of record r in removal order
+// visit the chain of record r in order: removed sources (>= 0) and the
+// owner markers (-1 - k: the removals that follow belong to copy k)
 template <class F>
-__device__ inline void rm_walk(const PW &W, int r, F f) {
-  int left = W.r_nrm[r];
+__device__ inline void chain_walk(const PW &W, int r, F f) {
+  int left = W.r_nchain[r];
   for (int ch = W.r_head[r]; ch >= 0 && left > 0; ch = W.rm_buf[ch]) {
     int m = left < RM_CHUNK - 1 ? left : RM_CHUNK - 1;
     for (int q = 1; q <= m; ++q) f(W.rm_buf[ch + q]);
     left -= m;
   }
+}
+
+// visit the removed sources of record r in removal order
+template <class F>
+__device__ inline void rm_walk(const PW &W, int r, F f) {
+  chain_walk(W, r, [&](int q) {
+    if (q >= 0) f(q);
+  });
 }
 
 __device__ inline void pool_store(const PW &W, int n_spec, int i, const Rec &s) {
@@ -1088,93 +1099,140 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
   const int ns = A.n_spec;
   const Weights &w = B->w;
   const uint32_t ordinal = (uint32_t)(p - B->off) + 1;
-  Rec target, source;
+  Rec target, cur, source;
   load_rec(A, p, target);
   RmChain chain{-1, -1, RM_CHUNK, 0, 0ull};
-  int n_samp_tot = 0, n_coag_tot = 0, warn = 0, flags = 0, err_code = 0;
-  int n_dup = 0, dup_head = -1, dup_tail = -1;
-  while (q >= 0) {
-    decode_pair(q, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
-    const int kind = pacc[q];
-    const int bs = kind == 1 ? b2 : b1, cs = kind == 1 ? c2 : c1;
-    const int64_t ti = table_index(A, box, nb, nc, b1, b2, c1, c2);
-    const double k_max = A.kw[ti];
-    const int cc = A.cc[ti];
-    const int ks = par_key(bs, cs, nb);
-    const int n_s = W.lcnt[lb + ks];
-    const uint32_t site = pair_site(nb, nc, b1, b2, c1, c2);
-    Stream aps, smp, fin;
-    aps.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_POISSON, round));
-    smp.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_SAMPLE, round));
-    fin.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_FINISH, round));
-    int n_samp, n_coag;
-    int err = acc_sample_source(A, W, B, type, target, ct, cs, nc_at_radius(w, cs, edges[bs - 1]),
-                                B->off + W.lstart[lb + ks], n_s, k_max * del_t * (double)n_s,
-                                1.0 / k_max, aps, smp, fin, t, chain, &n_samp, &n_coag, &warn,
-                                source);
-    if (err) {
-      atomicExch(&B->error, err);
-      return;
-    }
-    n_samp_tot += n_samp;
-    n_coag_tot += n_coag;
-    if (n_coag > 0) {
-      // coag_target_with_source
-      Stream ft;
-      ft.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_TARGET, round));
-      const double old_nc = nc_at_radius(w, target.klass, vol2rad(rec_vol(ns, target)));
-      Rec np;
-      err = par_rec_coagulate(ns, ft, target, source, np);
+  int n_samp_tot = 0, n_coag_tot = 0, warn = 0, flags = 0, err_code = 0, n_rm = 0;
+  int n_dup = 0, n_dead = 0, dup_head = -1, dup_tail = -1;
+  // The target walks first; each copy made on the way (in the reference it
+  // joins the list of the target's new bin) then walks on from the pair
+  // that made it, in creation order.  Removals are preceded in the chain
+  // by a marker -1 - k when they belong to copy k.
+  int walker = -1;   // -1: the target; else the copy's pool entry
+  uint32_t tag = 0;  // walker number: copies use sites beyond the pair sites
+  cur = target;
+  for (;;) {
+    bool died = false;
+    bool marked = false;
+    while (q >= 0 && !err_code) {
+      decode_pair(q, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
+      const int kind = pacc[q];
+      const int bs = kind == 1 ? b2 : b1, cs = kind == 1 ? c2 : c1;
+      const int64_t ti = table_index(A, box, nb, nc, b1, b2, c1, c2);
+      const double k_max = A.kw[ti];
+      const int cc = A.cc[ti];
+      const int ks = par_key(bs, cs, nb);
+      const int n_s = W.lcnt[lb + ks];
+      // copies draw from their own sites, above the box's nc^2 nb^2 pair sites
+      const uint32_t site = pair_site(nb, nc, b1, b2, c1, c2) + tag * (uint32_t)(nc * nc * nb * nb);
+      Stream aps, smp, fin;
+      aps.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_POISSON, round));
+      smp.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_SAMPLE, round));
+      fin.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_FINISH, round));
+      if (walker >= 0 && !marked) {
+        // owner marker for this copy's removals (pushed before any of them)
+        if (!rm_push(W, chain, -1 - (int)tag)) {
+          atomicExch(&B->error, ERR_CAPACITY);
+          return;
+        }
+        marked = true;
+      }
+      const int n_before = chain.n;
+      int n_samp, n_coag;
+      int err = acc_sample_source(A, W, B, type, cur, ct, cs, nc_at_radius(w, cs, edges[bs - 1]),
+                                  B->off + W.lstart[lb + ks], n_s, k_max * del_t * (double)n_s,
+                                  1.0 / k_max, aps, smp, fin, t, chain, &n_samp, &n_coag, &warn,
+                                  source);
       if (err) {
         atomicExch(&B->error, err);
         return;
       }
-      np.id = target.id;
-      const double r_new = vol2rad(rec_vol(ns, np));
-      np.group = rand_group_w(ft, w, cc, r_new);
-      np.klass = cc;
-      target = np;
-      flags |= RF_MERGED;
-      const int new_bin = bin_find_log(nb, B->log_lo, B->log_hi, r_new);
-      if (new_bin < 1 || new_bin > nb) {
-        err_code = 765620746;
-        break;
-      }
-      const int n_copies = prob_round(ft, old_nc / nc_at_radius(w, np.klass, r_new));
-      if (n_copies == 0) {
-        flags |= RF_DIED;
-        break;
-      }
-      for (int c = 1; c < n_copies; ++c) {
-        const int d = W.acc_cap + atomicAdd(W.dup_top, 1);
-        if (d >= W.pool_cap) {
-          atomicExch(&B->error, ERR_CAPACITY);
+      n_rm += chain.n - n_before;
+      n_samp_tot += n_samp;
+      n_coag_tot += n_coag;
+      if (n_coag > 0) {
+        // coag_target_with_source
+        Stream ft;
+        ft.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_TARGET, round));
+        const double old_nc = nc_at_radius(w, cur.klass, vol2rad(rec_vol(ns, cur)));
+        Rec np;
+        err = par_rec_coagulate(ns, ft, cur, source, np);
+        if (err) {
+          atomicExch(&B->error, err);
           return;
         }
-        Rec cp = target;
-        cp.group = rand_group_w(ft, w, target.klass, r_new);
-        pool_store(W, ns, d, cp);
-        W.q_next[d] = -1;
-        if (dup_tail >= 0) W.q_next[dup_tail] = d;
-        else dup_head = d;
-        dup_tail = d;
-        n_dup++;
+        np.id = cur.id;
+        const double r_new = vol2rad(rec_vol(ns, np));
+        np.group = rand_group_w(ft, w, cc, r_new);
+        np.klass = cc;
+        cur = np;
+        if (walker < 0) flags |= RF_MERGED;
+        const int new_bin = bin_find_log(nb, B->log_lo, B->log_hi, r_new);
+        if (new_bin < 1 || new_bin > nb) {
+          err_code = 765620746;
+          break;
+        }
+        const int n_copies = prob_round(ft, old_nc / nc_at_radius(w, np.klass, r_new));
+        if (n_copies == 0) {
+          died = true;
+          break;
+        }
+        const int64_t q_next = next_acc_pair(W, pacc, nb, nc, ct, new_bin, q);
+        for (int c = 1; c < n_copies; ++c) {
+          const int d = W.acc_cap + atomicAdd(W.dup_top, 1);
+          if (d >= W.pool_cap) {
+            atomicExch(&B->error, ERR_CAPACITY);
+            return;
+          }
+          Rec cp = cur;
+          cp.group = rand_group_w(ft, w, cur.klass, r_new);
+          pool_store(W, ns, d, cp);
+          W.q_next[d] = -1;
+          W.q_pos[d] = (int)q_next;
+          W.q_bin[d] = new_bin;
+          W.q_dead[d] = 0;
+          if (dup_tail >= 0) W.q_next[dup_tail] = d;
+          else dup_head = d;
+          dup_tail = d;
+          n_dup++;
+        }
+        b_cur = new_bin;
+        q = q_next;
+      } else {
+        q = next_acc_pair(W, pacc, nb, nc, ct, b_cur, q);
       }
-      b_cur = new_bin;
     }
-    q = next_acc_pair(W, pacc, nb, nc, ct, b_cur, q);
+    if (walker < 0) {
+      target = cur;
+      if (died) flags |= RF_DIED;
+    } else {
+      pool_store(W, ns, walker, cur);
+      if (died) {
+        W.q_dead[walker] = 1;
+        n_dead++;
+      }
+    }
+    if (err_code) break;
+    walker = walker < 0 ? dup_head : W.q_next[walker];
+    if (walker < 0) break;
+    tag++;
+    pool_load(W, ns, walker, ct, 0, cur);
+    q = W.q_pos[walker];
+    b_cur = W.q_bin[walker];
   }
   if (warn) flags |= RF_WARN;
-  if (n_coag_tot == 0 && chain.n == 0 && err_code == 0) {
+  if (n_coag_tot == 0 && n_rm == 0 && err_code == 0) {
     W.tns[i] = n_samp_tot | (warn ? ACC_WARN : 0);
     return;
   }
   if (flags & RF_MERGED) pool_store(W, ns, rec, target);
   W.r_nsamp[rec] = n_samp_tot;
   W.r_ncoag[rec] = n_coag_tot;
-  W.r_nrm[rec] = chain.n;
+  W.r_nrm[rec] = n_rm;
+  W.r_nchain[rec] = chain.n;
   W.r_head[rec] = chain.head;
   W.r_ndup[rec] = n_dup;
+  W.r_ndead[rec] = n_dead;
   W.r_dhead[rec] = dup_head;
   W.r_flags[rec] = flags;
   W.r_err[rec] = err_code;
@@ -1295,13 +1353,12 @@ __global__ void k_par_resolve(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, 
     // a target event wins when it holds the target and every removed source
     bool win = W.claim[p1] == t;
     rm_walk(W, r, [&](int q) { win = win && W.claim[q] == t; });
-    int dups = win ? W.r_ndup[r] : 0;
-    int gone = win ? W.r_nrm[r] + ((W.r_flags[r] & RF_DIED) ? 1 : 0) : 0;
+    const int died = (W.r_flags[r] & RF_DIED) ? 1 : 0;
     W.ev_out[e] = win ? O_WIN : 0;
-    W.f_app[e] = dups;
-    W.f_id[e] = dups;
-    W.f_info[e] = gone;
-    W.f_hole[e] = gone;
+    W.f_app[e] = win ? W.r_ndup[r] - W.r_ndead[r] : 0;
+    W.f_id[e] = win ? W.r_ndup[r] : 0;
+    W.f_info[e] = win ? W.r_nrm[r] + died + W.r_ndead[r] : 0;
+    W.f_hole[e] = win ? W.r_nrm[r] + died : 0;
     W.f_lose[e] = win ? 0 : 1;
     return;
   }
@@ -1367,23 +1424,28 @@ __device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc 
 // a winning target event (coag_target_with_source and
 // aero_state_dup_particle, src/coagulation.F90:522-582,
 // src/aero_state.F90:488-539): removed sources become holes with a
-// coagulation record naming the target, the merged target replaces the
-// old one in place, its copies are appended with new identifiers, and a
-// target that kept no copy is removed with a weighting record.
-__device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, int n_box, int e,
-                                 int type, int want_terms, int64_t hole_base) {
+// coagulation record naming the particle that took them (the target or one
+// of its copies), the merged target replaces the old one in place, copies
+// are appended with new identifiers in creation order, and a target or
+// copy that kept no copy of itself is removed with a weighting record.
+__device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, int e, int type,
+                                 int want_terms, int64_t hole_base) {
   const uint32_t t = W.ev_t[e];
   const int r = W.ev_rec[e];
   const int64_t p = W.ev_p1[e];
   const int box = W.ev_box[e];
   BoxDesc *B = &boxes[box];
-  (void)n_box;
   int *live = W.live + (int64_t)box * W.K_stride;
   const int first = W.box_first[box];
   const int ns = A.n_spec;
   const int nb = B->n_bin;
   const int64_t target_id = A.id[p];
+  const int klass = A.klass[p];
   const int flags = W.r_flags[r];
+  const int64_t id_base = W.s_id[e] - W.s_id[first];
+  auto copy_id = [&](int k) {  // identifier of copy k (1-based, creation order)
+    return (B->next_id + id_base + k - 2) * B->id_stride + B->id_offset + 1;
+  };
   int64_t hrank = hole_base + W.s_hole[e];
   int64_t irank = B->n_info + (W.s_info[e] - W.s_info[first]);
   auto info = [&](int64_t id, int action, int64_t other) {
@@ -1394,12 +1456,17 @@ __device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, i
     }
     irank++;
   };
-  rm_walk(W, r, [&](int q) {
+  int64_t owner = target_id;
+  chain_walk(W, r, [&](int q) {
+    if (q < 0) {
+      owner = copy_id(-1 - q);
+      return;
+    }
     W.dead_t[q] = t;
     atomicSub(&live[W.key[q]], 1);
     W.status[q] = ST_HOLE;
     W.holes[hrank++] = ((uint64_t)box << 32) | (uint64_t)(q - B->off);
-    info(A.id[q], 2, target_id);
+    info(A.id[q], 2, owner);
   });
   const unsigned long long n_samp = (unsigned long long)W.r_nsamp[r];
   const unsigned long long n_coag = (unsigned long long)W.r_ncoag[r];
@@ -1410,7 +1477,6 @@ __device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, i
   if (flags & RF_WARN) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
   if (W.r_err[r]) atomicExch(&B->error, W.r_err[r]);
   if (!(flags & RF_MERGED)) return;
-  const int klass = A.klass[p];
   Rec np;
   pool_load(W, ns, r, klass, target_id, np);
   W.dead_t[p] = t;
@@ -1419,15 +1485,19 @@ __device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, i
   if (flags & RF_DIED) {
     W.status[p] = ST_HOLE;
     W.holes[hrank++] = ((uint64_t)box << 32) | (uint64_t)(p - B->off);
+    info(target_id, 4, 0);
   } else {
     atomicAdd(&live[W.key[p]], 1);
   }
-  int64_t id_rank = W.s_id[e] - W.s_id[first];
   int64_t app_rank = W.s_app[e] - W.s_app[first];
+  int k = 0;
   for (int d = W.r_dhead[r]; d >= 0; d = W.q_next[d]) {
-    const int64_t id = (B->next_id + id_rank - 1) * B->id_stride + B->id_offset + 1;
-    id_rank++;
-    pool_load(W, ns, d, klass, id, np);
+    ++k;
+    if (W.q_dead[d]) {
+      info(copy_id(k), 4, 0);
+      continue;
+    }
+    pool_load(W, ns, d, klass, copy_id(k), np);
     if (B->n_part + app_rank >= B->cap) {
       atomicExch(&B->error, ERR_CAPACITY);
       return;
@@ -1437,7 +1507,6 @@ __device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, i
     write_new_particle(A, W, B, slot, np, nb, type, want_terms);
     atomicAdd(&live[W.key[slot]], 1);
   }
-  if (flags & RF_DIED) info(target_id, 4, 0);
 }
 
 // apply the winners' merges (coagulate, src/coagulation.F90:928-985)
@@ -1449,7 +1518,7 @@ __global__ void k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, ui
   int o = W.ev_out[e];
   if (!(o & O_WIN)) return;
   if (W.ev_rec[e] >= 0) {
-    acc_apply(boxes, A, W, n_box, e, type, want_terms, hole_base);
+    acc_apply(boxes, A, W, e, type, want_terms, hole_base);
     return;
   }
   uint32_t t = W.ev_t[e];
@@ -1725,12 +1794,15 @@ inline int parallel_init(ParallelWork &W, int n_box, int max_bin) {
 // the accelerated path's records, allocated when a step first has target slots
 inline void parallel_free_records(ParallelWork &W) {
   PW &d = W.d;
-  void *ptrs[] = {d.r_nsamp, d.r_ncoag, d.r_nrm, d.r_head, d.r_ndup, d.r_dhead, d.r_flags,
-                  d.r_err, d.q_vol, d.q_group, d.q_nprim, d.q_ncomp, d.q_next, d.q_csrc,
-                  d.q_ctime, d.q_least, d.q_greatest, d.dup_top, d.rm_buf, d.rm_top};
+  void *ptrs[] = {d.r_nsamp, d.r_ncoag, d.r_nrm, d.r_nchain, d.r_head, d.r_ndup, d.r_ndead,
+                  d.r_dhead, d.r_flags, d.r_err, d.q_vol, d.q_group, d.q_nprim, d.q_ncomp,
+                  d.q_next, d.q_pos, d.q_bin, d.q_dead, d.q_csrc, d.q_ctime, d.q_least,
+                  d.q_greatest, d.dup_top, d.rm_buf, d.rm_top};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
-  d.r_nsamp = d.r_ncoag = d.r_nrm = d.r_head = d.r_ndup = d.r_dhead = d.r_flags = d.r_err = nullptr;
+  d.r_nsamp = d.r_ncoag = d.r_nrm = d.r_nchain = d.r_head = d.r_ndup = d.r_ndead = nullptr;
+  d.r_dhead = d.r_flags = d.r_err = nullptr;
+  d.q_pos = d.q_bin = d.q_dead = nullptr;
   d.q_vol = d.q_ctime = d.q_least = d.q_greatest = nullptr;
   d.q_group = d.q_nprim = d.q_ncomp = d.q_next = d.q_csrc = nullptr;
   d.dup_top = d.rm_buf = d.rm_top = nullptr;
@@ -1740,13 +1812,13 @@ inline void parallel_free_records(ParallelWork &W) {
 inline int parallel_alloc_records(ParallelWork &W, int n_spec) {
   PW &d = W.d;
   const int64_t c = ACC_CAP, pc = (int64_t)ACC_CAP + DUP_CAP;
-  int **rv[] = {&d.r_nsamp, &d.r_ncoag, &d.r_nrm, &d.r_head, &d.r_ndup, &d.r_dhead, &d.r_flags,
-                &d.r_err};
+  int **rv[] = {&d.r_nsamp, &d.r_ncoag, &d.r_nrm, &d.r_nchain, &d.r_head, &d.r_ndup, &d.r_ndead,
+                &d.r_dhead, &d.r_flags, &d.r_err};
   for (int **q : rv) PCHECK(pmalloc(q, c));
   PCHECK(pmalloc(&d.q_vol, (size_t)n_spec * pc));
   PCHECK(pmalloc(&d.q_csrc, (size_t)MAX_COMP * pc));
   PCHECK(pmalloc(&d.q_ctime, (size_t)MAX_COMP * pc));
-  int **qv[] = {&d.q_group, &d.q_nprim, &d.q_ncomp, &d.q_next};
+  int **qv[] = {&d.q_group, &d.q_nprim, &d.q_ncomp, &d.q_next, &d.q_pos, &d.q_bin, &d.q_dead};
   for (int **q : qv) PCHECK(pmalloc(q, pc));
   PCHECK(pmalloc(&d.q_least, pc));
   PCHECK(pmalloc(&d.q_greatest, pc));
