@@ -527,27 +527,27 @@ __global__ void k_par_pairs(BoxDesc *boxes, Arrays A, PW W, double del_t, uint64
     if (c1 <= nc && c2 <= c1 && b1 <= nb && b2 <= nb && (c1 != c2 || b2 >= b1)) {
       const int *cnt = W.lcnt + (int64_t)box * W.K_stride;
       int n1 = cnt[par_key(b1, c1, nb)], n2 = cnt[par_key(b2, c2, nb)];
-      if (n1 > 0 && n2 > 0) {
-        const int64_t ti = table_index(A, box, nb, nc, b1, b2, c1, c2);
-        double k_max = A.kw[ti];
-        const double *edges = A.edges + (int64_t)box * (A.max_bin + 1);
-        double n_spt;
-        acc = accel_kind(B->w, edges, b1, b2, c1, c2, A.cc[ti], k_max, del_t, cnt, nb, &n_spt);
-        if (acc) {
-          // target slots go to the list's first accelerated pair
-          // (k_par_acc_first)
-          atomicAdd((unsigned long long *)&B->n_accel_pairs, 1ull);
-        } else {
-          bool same = (b1 == b2) && (c1 == c2);
-          double n_possible = same ? (double)n1 * ((double)n2 - 1.0) / 2.0 : (double)n1 * (double)n2;
-          double mean = k_max * del_t * n_possible;
-          Stream ps;
-          ps.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), 0, RNG_POISSON);
-          ns = (uint32_t)rand_poisson(ps, mean);
-          if (ns) {
-            atomicAdd((unsigned long long *)&B->n_samp, (unsigned long long)ns);
-            atomicAdd((unsigned long long *)&B->step_samp, (unsigned long long)ns);
-          }
+      const int64_t ti = table_index(A, box, nb, nc, b1, b2, c1, c2);
+      double k_max = A.kw[ti];
+      const double *edges = A.edges + (int64_t)box * (A.max_bin + 1);
+      double n_spt;
+      // eligibility needs only the source list: a target that grows into
+      // an empty bin during the step still visits that bin's pairs
+      acc = accel_kind(B->w, edges, b1, b2, c1, c2, A.cc[ti], k_max, del_t, cnt, nb, &n_spt);
+      if (acc) {
+        // target slots go to the list's first accelerated pair
+        // (k_par_acc_first)
+        if (n1 > 0 && n2 > 0) atomicAdd((unsigned long long *)&B->n_accel_pairs, 1ull);
+      } else if (n1 > 0 && n2 > 0) {
+        bool same = (b1 == b2) && (c1 == c2);
+        double n_possible = same ? (double)n1 * ((double)n2 - 1.0) / 2.0 : (double)n1 * (double)n2;
+        double mean = k_max * del_t * n_possible;
+        Stream ps;
+        ps.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), 0, RNG_POISSON);
+        ns = (uint32_t)rand_poisson(ps, mean);
+        if (ns) {
+          atomicAdd((unsigned long long *)&B->n_samp, (unsigned long long)ns);
+          atomicAdd((unsigned long long *)&B->step_samp, (unsigned long long)ns);
         }
       }
     }
@@ -571,6 +571,13 @@ __device__ inline void atomic_max_double(double *addr, double v) {
   unsigned long long *a = (unsigned long long *)addr;
   unsigned long long nv = (unsigned long long)__double_as_longlong(v);
   atomicMax(a, nv);  // v >= 0
+}
+
+// a slot of a step-start list can be drawn while it holds a particle of
+// that list's (bin, class): untouched, or changed in place by an earlier
+// merge without leaving the list
+__device__ inline bool drawable(const PW &W, int p, int key) {
+  return W.status[p] != ST_HOLE && W.key[p] == key;
 }
 
 // one thread per test: draw, evaluate, accept, claim
@@ -609,49 +616,39 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
   Stream ts;
   ts.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), j,
           round == 0 ? RNG_TEST : RNG_RETRY + (uint32_t)round);
+  // entries of the step's lists, skipping slots whose particle is gone or
+  // has left the list's (bin, class); with every particle present the
+  // first draw of each stands, as in find_rand_pair (:766-803)
   int p1 = -1, p2 = -1;
-  if (round == 0) {
-    int i1 = rand_int(ts, n1);
-    int i2;
-    if (same) {
-      i2 = rand_int(ts, n2 - 1);
-      if (i2 == i1) i2 = n2;
-    } else {
-      i2 = rand_int(ts, n2);
+  int i1 = 0;
+  for (int a = 0; a < MAX_REDRAW; ++a) {
+    int c = rand_int(ts, n1);
+    int p = W.perm[base + s1 + c - 1];
+    if (drawable(W, p, key1)) {
+      i1 = c;
+      p1 = p;
+      break;
     }
-    p1 = W.perm[base + s1 + i1 - 1];
-    p2 = W.perm[base + s2 + i2 - 1];
-  } else {
-    int i1 = 0;
+  }
+  if (p1 >= 0 && (!same || n2 > 1)) {
     for (int a = 0; a < MAX_REDRAW; ++a) {
-      int c = rand_int(ts, n1);
-      int p = W.perm[base + s1 + c - 1];
-      if (W.status[p] == ST_LIVE) {
-        i1 = c;
-        p1 = p;
+      int c;
+      if (same) {
+        c = rand_int(ts, n2 - 1);
+        if (c == i1) c = n2;
+      } else {
+        c = rand_int(ts, n2);
+      }
+      int p = W.perm[base + s2 + c - 1];
+      if (drawable(W, p, key2) && p != p1) {
+        p2 = p;
         break;
       }
     }
-    if (p1 >= 0 && (!same || n2 > 1)) {
-      for (int a = 0; a < MAX_REDRAW; ++a) {
-        int c;
-        if (same) {
-          c = rand_int(ts, n2 - 1);
-          if (c == i1) c = n2;
-        } else {
-          c = rand_int(ts, n2);
-        }
-        int p = W.perm[base + s2 + c - 1];
-        if (W.status[p] == ST_LIVE && p != p1) {
-          p2 = p;
-          break;
-        }
-      }
-    }
-    if (p1 < 0 || p2 < 0) {
-      atomicAdd(&B->n_dropped, 1);
-      return;
-    }
+  }
+  if (p1 < 0 || p2 < 0) {
+    atomicAdd(&B->n_dropped, 1);
+    return;
   }
   W.tp1[i] = p1;
   W.tp2[i] = p2;
@@ -915,7 +912,7 @@ __device__ inline void pool_load(const PW &W, int n_spec, int i, int klass, int6
 // and claimed with index t.
 __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B, int type,
                                         const Rec &target, int ct, int cs, double nc_src_min,
-                                        int64_t sbase, int n_s, double n_spt,
+                                        int64_t sbase, int skey, int n_s, double n_spt,
                                         double accept_factor, Stream &aps, Stream &smp,
                                         Stream &fin, uint32_t t, RmChain &chain, int *n_samp_out,
                                         int *n_coag_out, int *warn, Rec &source) {
@@ -963,7 +960,7 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
     for (int a = 0; a < MAX_REDRAW; ++a) {
       int c = rand_int(smp, n_s);
       int q = W.perm[sbase + c - 1];
-      if (W.status[q] == ST_LIVE && !rm_has(W, chain, q)) {
+      if (drawable(W, q, skey) && !rm_has(W, chain, q)) {
         ps = q;
         break;
       }
@@ -1088,14 +1085,18 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
   decode_pair(q, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
   int b_cur = pacc[q] == 1 ? b1 : b2;
   const int ct = pacc[q] == 1 ? c1 : c2;
-  {
-    const int n_t = W.lcnt[lb + par_key(b_cur, ct, nb)];
-    const int e = n_t - (int)(t - W.pair_exc[k]);
-    const int p0 = W.perm[B->off + W.lstart[lb + par_key(b_cur, ct, nb)] + e - 1];
-    if (W.status[p0] != ST_LIVE) return;  // consumed earlier in this step
-    W.tp1[i] = p0;
+  const int n_t = W.lcnt[lb + par_key(b_cur, ct, nb)];
+  const int e = n_t - (int)(t - W.pair_exc[k]);
+  const int p = W.perm[B->off + W.lstart[lb + par_key(b_cur, ct, nb)] + e - 1];
+  if (W.status[p] == ST_HOLE) return;  // removed earlier in this step
+  if (W.key[p] != par_key(b_cur, ct, nb)) {
+    // changed in place earlier in the step: walk on from its current list
+    const int kp = W.key[p];
+    if (kp / nb + 1 != ct) return;
+    b_cur = kp % nb + 1;
+    q = next_acc_pair(W, pacc, nb, nc, ct, b_cur, q - 1);
   }
-  const int p = W.tp1[i];
+  W.tp1[i] = p;
   const int ns = A.n_spec;
   const Weights &w = B->w;
   const uint32_t ordinal = (uint32_t)(p - B->off) + 1;
@@ -1140,7 +1141,7 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
       const int n_before = chain.n;
       int n_samp, n_coag;
       int err = acc_sample_source(A, W, B, type, cur, ct, cs, nc_at_radius(w, cs, edges[bs - 1]),
-                                  B->off + W.lstart[lb + ks], n_s, k_max * del_t * (double)n_s,
+                                  B->off + W.lstart[lb + ks], ks, n_s, k_max * del_t * (double)n_s,
                                   1.0 / k_max, aps, smp, fin, t, chain, &n_samp, &n_coag, &warn,
                                   source);
       if (err) {
