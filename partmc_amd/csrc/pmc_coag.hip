@@ -614,16 +614,19 @@ int pmc_gpu_download(pmc_gpu *h, int box, int max_part, int *n_part, double *vol
   if (n_primary) HIPCHECK(hipMemcpy(n_primary, A.nprim + base, n * sizeof(int), hipMemcpyDeviceToHost));
   if (n_comp) HIPCHECK(hipMemcpy(n_comp, A.ncomp + base, n * sizeof(int), hipMemcpyDeviceToHost));
   if (comp_source || comp_time) {
-    std::vector<int> ci(n);
+    // the device keeps only the first n_comp entries of a particle; the
+    // rest read as zero
+    std::vector<int> ci(n), nco(n);
     std::vector<double> cd(n);
+    HIPCHECK(hipMemcpy(nco.data(), A.ncomp + base, n * sizeof(int), hipMemcpyDeviceToHost));
     for (int c = 0; c < MAX_COMP; ++c) {
       if (comp_source) {
         HIPCHECK(hipMemcpy(ci.data(), A.csrc + c * tc + base, n * sizeof(int), hipMemcpyDeviceToHost));
-        for (int i = 0; i < n; ++i) comp_source[(size_t)i * MAX_COMP + c] = ci[i];
+        for (int i = 0; i < n; ++i) comp_source[(size_t)i * MAX_COMP + c] = c < nco[i] ? ci[i] : 0;
       }
       if (comp_time) {
         HIPCHECK(hipMemcpy(cd.data(), A.ctime + c * tc + base, n * sizeof(double), hipMemcpyDeviceToHost));
-        for (int i = 0; i < n; ++i) comp_time[(size_t)i * MAX_COMP + c] = cd[i];
+        for (int i = 0; i < n; ++i) comp_time[(size_t)i * MAX_COMP + c] = c < nco[i] ? cd[i] : 0.0;
       }
     }
   }

@@ -96,8 +96,9 @@ __device__ inline void copy_particle(const Arrays &A, int64_t from, int64_t to) 
   A.group[to] = A.group[from];
   A.klass[to] = A.klass[from];
   A.nprim[to] = A.nprim[from];
-  A.ncomp[to] = A.ncomp[from];
-  for (int c = 0; c < MAX_COMP; ++c) {
+  const int nco = A.ncomp[from];
+  A.ncomp[to] = nco;
+  for (int c = 0; c < nco; ++c) {  // entries past n_comp are never read
     A.csrc[(int64_t)c * A.total_cap + to] = A.csrc[(int64_t)c * A.total_cap + from];
     A.ctime[(int64_t)c * A.total_cap + to] = A.ctime[(int64_t)c * A.total_cap + from];
   }
@@ -476,7 +477,7 @@ struct ExactBox {
     r.klass = A->klass[p];
     r.nprim = A->nprim[p];
     r.ncomp = A->ncomp[p];
-    for (int c = 0; c < MAX_COMP; ++c) {
+    for (int c = 0; c < r.ncomp; ++c) {
       r.csrc[c] = A->csrc[(int64_t)c * A->total_cap + p];
       r.ctime[c] = A->ctime[(int64_t)c * A->total_cap + p];
     }
@@ -496,7 +497,7 @@ struct ExactBox {
     A->klass[p] = r.klass;
     A->nprim[p] = r.nprim;
     A->ncomp[p] = r.ncomp;
-    for (int c = 0; c < MAX_COMP; ++c) {
+    for (int c = 0; c < r.ncomp; ++c) {
       A->csrc[(int64_t)c * A->total_cap + p] = r.csrc[c];
       A->ctime[(int64_t)c * A->total_cap + p] = r.ctime[c];
     }

@@ -425,7 +425,7 @@ __device__ inline void load_rec(const Arrays &A, int64_t p, Rec &r) {
   r.klass = A.klass[p];
   r.nprim = A.nprim[p];
   r.ncomp = A.ncomp[p];
-  for (int c = 0; c < MAX_COMP; ++c) {
+  for (int c = 0; c < r.ncomp; ++c) {
     r.csrc[c] = A.csrc[(int64_t)c * A.total_cap + p];
     r.ctime[c] = A.ctime[(int64_t)c * A.total_cap + p];
   }
@@ -881,7 +881,7 @@ __device__ inline void pool_store(const PW &W, int n_spec, int i, const Rec &s) 
   W.q_group[i] = s.group;
   W.q_nprim[i] = s.nprim;
   W.q_ncomp[i] = s.ncomp;
-  for (int c = 0; c < MAX_COMP; ++c) {
+  for (int c = 0; c < s.ncomp; ++c) {
     W.q_csrc[(int64_t)c * W.pool_cap + i] = s.csrc[c];
     W.q_ctime[(int64_t)c * W.pool_cap + i] = s.ctime[c];
   }
@@ -897,7 +897,7 @@ __device__ inline void pool_load(const PW &W, int n_spec, int i, int klass, int6
   s.klass = klass;
   s.nprim = W.q_nprim[i];
   s.ncomp = W.q_ncomp[i];
-  for (int c = 0; c < MAX_COMP; ++c) {
+  for (int c = 0; c < s.ncomp; ++c) {
     s.csrc[c] = W.q_csrc[(int64_t)c * W.pool_cap + i];
     s.ctime[c] = W.q_ctime[(int64_t)c * W.pool_cap + i];
   }
@@ -1404,7 +1404,7 @@ __device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc 
   A.klass[slot] = r.klass;
   A.nprim[slot] = r.nprim;
   A.ncomp[slot] = r.ncomp;
-  for (int c = 0; c < MAX_COMP; ++c) {
+  for (int c = 0; c < r.ncomp; ++c) {
     A.csrc[(int64_t)c * A.total_cap + slot] = r.csrc[c];
     A.ctime[(int64_t)c * A.total_cap + slot] = r.ctime[c];
   }
