@@ -52,6 +52,7 @@ struct PW {  // device view of the parallel work arrays
   uint8_t *status;  // [total_cap]
   uint32_t *claim;  // [total_cap]
   uint32_t *dead_t; // [total_cap] test index of the winner that consumed it
+  int *dead_r;      // [total_cap] claim round (step-wide count) of that winner
   float4 *terms;    // [total_cap]
   int *perm;        // [total_cap] global slots sorted by (box, key)
   int *tile_box, *tile_start, *tiles_in_box;
@@ -1430,7 +1431,7 @@ __device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc 
 // are appended with new identifiers in creation order, and a target or
 // copy that kept no copy of itself is removed with a weighting record.
 __device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, int e, int type,
-                                 int want_terms, int64_t hole_base) {
+                                 int want_terms, int64_t hole_base, int rid) {
   const uint32_t t = W.ev_t[e];
   const int r = W.ev_rec[e];
   const int64_t p = W.ev_p1[e];
@@ -1464,6 +1465,7 @@ __device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, i
       return;
     }
     W.dead_t[q] = t;
+    W.dead_r[q] = rid;
     atomicSub(&live[W.key[q]], 1);
     W.status[q] = ST_HOLE;
     W.holes[hrank++] = ((uint64_t)box << 32) | (uint64_t)(q - B->off);
@@ -1481,6 +1483,7 @@ __device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, i
   Rec np;
   pool_load(W, ns, r, klass, target_id, np);
   W.dead_t[p] = t;
+  W.dead_r[p] = rid;
   atomicSub(&live[W.key[p]], 1);
   write_new_particle(A, W, B, p, np, nb, type, want_terms);
   if (flags & RF_DIED) {
@@ -1513,13 +1516,13 @@ __device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, i
 // apply the winners' merges (coagulate, src/coagulation.F90:928-985)
 __global__ void k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, uint64_t seed,
                             uint32_t step, int round, int type, int want_terms,
-                            int64_t hole_base) {
+                            int64_t hole_base, int rid) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   int o = W.ev_out[e];
   if (!(o & O_WIN)) return;
   if (W.ev_rec[e] >= 0) {
-    acc_apply(boxes, A, W, e, type, want_terms, hole_base);
+    acc_apply(boxes, A, W, e, type, want_terms, hole_base, rid);
     return;
   }
   uint32_t t = W.ev_t[e];
@@ -1568,6 +1571,7 @@ __global__ void k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, ui
   int64_t hrank = hole_base + W.s_hole[e];
   if (o & O_RM1) {
     W.dead_t[p1] = t;
+    W.dead_r[p1] = rid;
     atomicSub(&live[W.key[p1]], 1);
     if (!(o & O_SLOT1)) {
       W.status[p1] = ST_HOLE;
@@ -1576,6 +1580,7 @@ __global__ void k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, ui
   }
   if (o & O_RM2) {
     W.dead_t[p2] = t;
+    W.dead_r[p2] = rid;
     atomicSub(&live[W.key[p2]], 1);
     if (!(o & O_SLOT2)) {
       W.status[p2] = ST_HOLE;
@@ -1639,14 +1644,16 @@ __global__ void k_par_release(PW W, int E) {
 }
 
 // Tests of the round that must be drawn again: accepted tests that lost a
-// claim, and any other non-winning test that drew a particle an earlier
-// winner (smaller sequential index) consumed -- in the sequential chain
-// that test would have drawn from the particles still present.
-__device__ inline bool consumed_before(const PW &W, int p, uint32_t t) {
-  return p >= 0 && W.status[p] != ST_LIVE && W.dead_t[p] < t;
+// claim, and any other non-winning test that drew a particle which an
+// earlier winner (smaller sequential index) of the same round removed or
+// changed after the draw -- in the sequential chain that test would have
+// seen the particles as that winner left them.  A particle changed in an
+// earlier round was drawn in its changed state.
+__device__ inline bool consumed_before(const PW &W, int p, uint32_t t, int rid) {
+  return p >= 0 && W.status[p] != ST_LIVE && W.dead_t[p] < t && W.dead_r[p] == rid;
 }
 
-__global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n) {
+__global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t t = W.tid[i];
@@ -1656,7 +1663,7 @@ __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n) {
   } else if (W.trec[i] >= 0) {
     // a target slot without effect: final unless its target was consumed
     // by an earlier winner; its samples count once final
-    f = consumed_before(W, W.tp1[i], t) ? 1 : 0;
+    f = consumed_before(W, W.tp1[i], t, rid) ? 1 : 0;
     int v = W.tns[i];
     if (!f && v) {
       const int64_t k = find_pair(W.pair_exc, (int64_t)n_box * W.P_stride, t);
@@ -1667,7 +1674,7 @@ __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n) {
       if (v & ACC_WARN) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
     }
   } else if (W.tp1[i] >= 0) {
-    f = (consumed_before(W, W.tp1[i], t) || consumed_before(W, W.tp2[i], t)) ? 1 : 0;
+    f = (consumed_before(W, W.tp1[i], t, rid) || consumed_before(W, W.tp2[i], t, rid)) ? 1 : 0;
   }
   W.f_pend[i] = f;
 }
@@ -1838,7 +1845,7 @@ inline void parallel_free_arrays(ParallelWork &W) {
                   d.tiles_in_box, d.hist_off, d.hist, d.hist_scan, d.lstart, d.lcnt, d.live,
                   d.pair_n, d.pair_exc, d.acc, d.acc_scan, d.tp1, d.tp2, d.tid, d.pend, d.ev_t,
                   d.ev_p1, d.ev_p2, d.ev_box, d.ev_out, d.f_app, d.f_id, d.f_info, d.f_hole,
-                  d.f_lose, d.f_pend, d.s_pend, d.dead_t, d.s_app, d.s_id, d.s_info, d.s_hole,
+                  d.f_lose, d.f_pend, d.s_pend, d.dead_t, d.dead_r, d.s_app, d.s_id, d.s_info, d.s_hole,
                   d.s_lose, d.box_first,
                   d.holes, d.holes_sorted, d.tail_live, W.scan_tmp, W.sort_tmp,
                   d.pair_acc, d.pair_accn, d.acc_exc, d.trec, d.tns, d.ev_rec, d.misc};
@@ -1878,6 +1885,7 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.status, tc));
   PCHECK(pmalloc(&d.claim, tc));
   PCHECK(pmalloc(&d.dead_t, tc));
+  PCHECK(pmalloc(&d.dead_r, tc));
   PCHECK(pmalloc(&d.terms, tc));
   PCHECK(pmalloc(&d.perm, tc));
   // tiles over each box's capacity
@@ -2090,10 +2098,11 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, fs[q], ss[q], E + 1, st));
       }
       PLAUNCH(W, "k_par_apply", k_par_apply, dim3((E + 255) / 256), dim3(256), 0, st, d_box, A, d,
-              n_box, E, seed, step, round, type, want_terms, n_holes);
+              n_box, E, seed, step, round, type, want_terms, n_holes, (int)W.n_rounds);
       hipLaunchKernelGGL(k_par_box_totals, gb, bb, 0, st, d_box, d, n_box, E);
       hipLaunchKernelGGL(k_par_release, dim3((E + 255) / 256), dim3(256), 0, st, d, E);
-      hipLaunchKernelGGL(k_par_pending, dim3((n + 255) / 256), dim3(256), 0, st, d_box, d, n_box, n);
+      hipLaunchKernelGGL(k_par_pending, dim3((n + 255) / 256), dim3(256), 0, st, d_box, d, n_box, n,
+                         (int)W.n_rounds);
       hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.f_pend + n, 1, 0);
       PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.f_pend, d.s_pend, n + 1, st));
       hipLaunchKernelGGL(k_par_gather_pending, dim3((n + 255) / 256), dim3(256), 0, st, d, n);
