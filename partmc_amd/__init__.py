@@ -236,13 +236,14 @@ class CoagGPU:
 
     def path_stats(self):
         """Per box: pairs on the per-particle path and dropped draws; for
-        the handle: claim rounds run and tests cut by the round limit."""
+        the handle: claim rounds run and tests finished in the sequential
+        tail."""
         acc = np.zeros(self.n_box, dtype=np.int64)
         drop = np.zeros(self.n_box, dtype=np.int64)
         rounds = np.zeros(2, dtype=np.int64)
         _check(load().pmc_gpu_get_path_stats(self._h, _ptr(acc), _ptr(drop), _ptr(rounds)))
         return {"n_accel_pairs": acc, "n_dropped": drop, "n_rounds": int(rounds[0]),
-                "n_cut": int(rounds[1])}
+                "n_tail": int(rounds[1])}
 
     def reserve(self, capacity):
         cap = np.ascontiguousarray(capacity, dtype=np.int64)

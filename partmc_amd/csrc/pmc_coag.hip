@@ -755,7 +755,7 @@ int pmc_gpu_reset_counters(pmc_gpu *h) {
     B.part_steps = 0;
   }
   h->work.n_rounds = 0;
-  h->work.n_dropped = 0;
+  h->work.n_tail = 0;
   HIPCHECK(hipMemcpy(h->d_box, hb.data(), sizeof(BoxDesc) * h->n_box, hipMemcpyHostToDevice));
   return 0;
 }
@@ -783,7 +783,7 @@ int pmc_gpu_get_path_stats(pmc_gpu *h, int64_t *n_accel_pairs, int64_t *n_droppe
     n_dropped[b] = hb[b].n_dropped;
   }
   rounds[0] = h->work.n_rounds;
-  rounds[1] = h->work.n_dropped;
+  rounds[1] = h->work.n_tail;
   return 0;
 }
 

@@ -39,8 +39,6 @@ constexpr uint32_t RNG_ACC_POISSON = 4;
 constexpr uint32_t RNG_ACC_SAMPLE = 5;
 constexpr uint32_t RNG_ACC_FINISH = 6;
 constexpr uint32_t RNG_ACC_TARGET = 7;
-constexpr uint32_t RNG_RETRY = 16;   // + round, parallel mode re-draws
-constexpr uint32_t RNG_ACC_RETRY = 64;  // + 4 round + (kind - RNG_ACC_POISSON)
 
 constexpr double C_PI = 3.14159265358979323846;
 constexpr double C_BOLTZMANN = 1.3806505e-23;

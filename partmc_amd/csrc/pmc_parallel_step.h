@@ -41,7 +41,7 @@ constexpr int TILE_ITEMS = 16;
 constexpr int TILE = TILE_THREADS * TILE_ITEMS;
 constexpr uint32_t UNCLAIMED = 0xFFFFFFFFu;
 constexpr uint8_t ST_LIVE = 0, ST_HOLE = 1, ST_NEW = 2;
-constexpr int MAX_ROUNDS = 16;
+constexpr int PAR_ROUNDS = 8;  // claim rounds before the sequential tail
 constexpr int MAX_REDRAW = 64;
 
 struct PW {  // device view of the parallel work arrays
@@ -94,6 +94,7 @@ struct PW {  // device view of the parallel work arrays
   int *rm_top;                 // [1] chunk allocator of the round
   int rm_cap;
   uint32_t *misc;              // [4] batch bounds from k_batch_end
+  unsigned long long *hole_top;  // [1] hole counter of the sequential tail
 };
 
 constexpr int ACC_CAP = 1 << 18;
@@ -117,11 +118,12 @@ struct ParallelWork {
   std::vector<int> tiles_in_box;
   uint32_t *h_u32 = nullptr;  // pinned scalars
   int *h_i32 = nullptr;
+  unsigned long long *h_u64 = nullptr;
   // optional per-kernel timing hooks of the owning handle
   void (*tbegin)(void *, const char *, hipEvent_t *) = nullptr;
   void (*tend)(void *, const char *, hipEvent_t) = nullptr;
   void *tctx = nullptr;
-  int64_t n_dropped = 0;  // tests still pending when the round limit was reached
+  int64_t n_tail = 0;     // tests finished in the sequential tail
   int64_t n_rounds = 0;   // claim rounds run
 };
 
@@ -581,26 +583,25 @@ __device__ inline bool drawable(const PW &W, int p, int key) {
   return W.status[p] != ST_HOLE && W.key[p] == key;
 }
 
-// one thread per test: draw, evaluate, accept, claim
-__global__ void __launch_bounds__(256)
-k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, uint32_t step,
-            int round, uint32_t t0, int n, const uint32_t *pend, int precision, double band,
-            int validation) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t t = pend ? pend[i] : t0 + (uint32_t)i;
-  const int64_t NP = (int64_t)n_box * W.P_stride;
-  const int64_t k = find_pair(W.pair_exc, NP, t);
-  if (W.pair_acc[k]) return;  // a target slot: k_par_acc_tests
-  W.acc[i] = 0;
-  W.tid[i] = t;
-  W.tp1[i] = -1;
-  W.tp2[i] = -1;
-  W.trec[i] = -1;
-  W.tns[i] = 0;
-  const int box = (int)(k / W.P_stride);
-  BoxDesc *B = &boxes[box];
-  if (B->error) return;
+// find_rand_pair + maybe_coag_pair (src/coagulation.F90:712-803) for test t
+// of pair k (global pair index): the two entries, drawn from the step's
+// lists skipping slots whose particle is gone or has left the list's
+// (bin, class) -- with every particle present the first draw of each
+// stands -- and the accept decision.  A re-run uses the test's own stream:
+// on an unchanged state it repeats the earlier outcome, on a changed one it
+// is what the sequential chain would draw there.
+struct TestOut {
+  int p1, p2;
+  bool accept;
+  bool warn;
+};
+
+__device__ inline void perset_test(BoxDesc *B, const Arrays &A, const PW &W, int box, int64_t k,
+                                   uint32_t t, int type, uint64_t seed, uint32_t step,
+                                   int precision, double band, int validation, TestOut &o) {
+  o.p1 = o.p2 = -1;
+  o.accept = false;
+  o.warn = false;
   int c1, c2, b1, b2;
   decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
   const uint32_t j = t - W.pair_exc[k];
@@ -615,11 +616,7 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
   const int s1 = W.lstart[lb + key1], s2 = W.lstart[lb + key2];
   const int64_t base = B->off;
   Stream ts;
-  ts.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), j,
-          round == 0 ? RNG_TEST : RNG_RETRY + (uint32_t)round);
-  // entries of the step's lists, skipping slots whose particle is gone or
-  // has left the list's (bin, class); with every particle present the
-  // first draw of each stands, as in find_rand_pair (:766-803)
+  ts.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), j, RNG_TEST);
   int p1 = -1, p2 = -1;
   int i1 = 0;
   for (int a = 0; a < MAX_REDRAW; ++a) {
@@ -651,19 +648,18 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
     atomicAdd(&B->n_dropped, 1);
     return;
   }
-  W.tp1[i] = p1;
-  W.tp2[i] = p2;
+  o.p1 = p1;
+  o.p2 = p2;
   const Weights &w = B->w;
   const Env env = B->env;
   double u = ts.uniform();
-  bool accept;
   double pr;
   if (precision == 0) {
     double k64;
     int err = weighted_kernel64(A, w, env, type, p1, p2, c1, c2, cc, &k64);
     if (err) atomicExch(&B->error, err);
     pr = k64 * accept_factor;
-    accept = u < pr;
+    o.accept = u < pr;
   } else {
     float k32 = weighted_kernel32(type, w, env, W.terms[p1], W.terms[p2], c1, c2, cc);
     double p32 = (double)k32 * accept_factor;
@@ -676,11 +672,11 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
     if (!(fabs(u - p32) >= band)) {
       atomicAdd((unsigned long long *)&B->n_recheck, 1ull);
       pr = k64 * accept_factor;
-      accept = u < pr;
+      o.accept = u < pr;
     } else {
       pr = p32;
-      accept = u < p32;
-      if (validation && (accept != (u < k64 * accept_factor)))
+      o.accept = u < p32;
+      if (validation && (o.accept != (u < k64 * accept_factor)))
         atomicAdd((unsigned long long *)&B->n_flip, 1ull);
     }
     if (validation && k64 > 0.0) {
@@ -688,11 +684,38 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
       atomic_max_double(&B->max_p_err, fabs(p32 - k64 * accept_factor));
     }
   }
-  if (!(pr <= 1.0)) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
-  if (accept) {
+  o.warn = !(pr <= 1.0);
+}
+
+// one thread per test: draw, evaluate, accept, claim
+__global__ void __launch_bounds__(256)
+k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, uint32_t step,
+            uint32_t t0, int n, const uint32_t *pend, int precision, double band,
+            int validation) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t t = pend ? pend[i] : t0 + (uint32_t)i;
+  const int64_t NP = (int64_t)n_box * W.P_stride;
+  const int64_t k = find_pair(W.pair_exc, NP, t);
+  if (W.pair_acc[k]) return;  // a target slot: k_par_acc_tests
+  W.acc[i] = 0;
+  W.tid[i] = t;
+  W.tp1[i] = -1;
+  W.tp2[i] = -1;
+  W.trec[i] = -1;
+  W.tns[i] = 0;
+  const int box = (int)(k / W.P_stride);
+  BoxDesc *B = &boxes[box];
+  if (B->error) return;
+  TestOut o;
+  perset_test(B, A, W, box, k, t, type, seed, step, precision, band, validation, o);
+  W.tp1[i] = o.p1;
+  W.tp2[i] = o.p2;
+  W.tns[i] = o.warn ? ACC_WARN : 0;  // counted once the test is final
+  if (o.accept) {
     W.acc[i] = 1;
-    atomicMin(&W.claim[p1], t);
-    atomicMin(&W.claim[p2], t);
+    atomicMin(&W.claim[o.p1], t);
+    atomicMin(&W.claim[o.p2], t);
   }
 }
 
@@ -767,11 +790,6 @@ __global__ void k_par_acc_first(BoxDesc *boxes, PW W) {
   if (q < 0) return;
   W.pair_n[pb + q] = (uint32_t)n_t;
   W.pair_accn[pb + q] = (uint32_t)n_t;
-}
-
-// the stream kinds of a target's draws; re-draws use their own kinds
-__device__ inline uint32_t acc_kind(uint32_t kind, int round) {
-  return round == 0 ? kind : RNG_ACC_RETRY + 4u * (uint32_t)round + (kind - RNG_ACC_POISSON);
 }
 
 // aero_weight_array_rand_group (src/aero_weight_array.F90:496-514)
@@ -915,7 +933,8 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
                                         const Rec &target, int ct, int cs, double nc_src_min,
                                         int64_t sbase, int skey, int n_s, double n_spt,
                                         double accept_factor, Stream &aps, Stream &smp,
-                                        Stream &fin, uint32_t t, RmChain &chain, int *n_samp_out,
+                                        Stream &fin, uint32_t t, bool claim, RmChain &chain,
+                                        int *n_samp_out,
                                         int *n_coag_out, int *warn, Rec &source) {
   const int ns = A.n_spec;
   const Weights &w = B->w;
@@ -988,7 +1007,7 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
         if (smp.uniform() < prob_remove_i / prm) {
           if (!rm_push(W, chain, ps)) return ERR_CAPACITY;
           n_rm_here++;
-          atomicMin(&W.claim[ps], t);
+          if (claim) atomicMin(&W.claim[ps], t);
         }
       }
     }
@@ -1042,40 +1061,19 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
   return 0;
 }
 
-// Round 0 runs one thread per target slot of the batch (record a maps to
-// its slot through acc_exc); later rounds run over the pending slots.
-__global__ void __launch_bounds__(64)
-k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed,
-                uint32_t step, int round, uint32_t t0, int n, const uint32_t *pend,
-                double del_t, uint32_t acc_base, int n_acc) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t NP = (int64_t)n_box * W.P_stride;
-  uint32_t t;
-  int i;
-  int64_t k;
-  if (pend) {
-    if (idx >= n) return;
-    i = idx;
-    t = pend[i];
-    k = find_pair(W.pair_exc, NP, t);
-    if (!W.pair_acc[k]) return;
-  } else {
-    if (idx >= n_acc) return;
-    const uint32_t a = acc_base + (uint32_t)idx;
-    k = find_pair(W.acc_exc, NP, a);
-    t = W.pair_exc[k] + (a - W.acc_exc[k]);
-    i = (int)(t - t0);
-  }
-  const int rec = (int)(W.acc_exc[k] + (t - W.pair_exc[k]) - acc_base);
-  W.acc[i] = 0;
-  W.tid[i] = t;
-  W.tp1[i] = -1;
-  W.tp2[i] = -1;
-  W.trec[i] = rec;
-  W.tns[i] = 0;
+// The walk of target slot t (pair k, record rec): returns 1 when the event
+// has an effect (record rec and the pool filled; the removed sources are
+// claimed with index t when `claim`), 0 when it has none (*tns = its
+// samples + ACC_WARN), -1 on a box error.  *p_out is the target's slot, or
+// -1 when the target is gone.
+__device__ inline int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &W, int64_t k,
+                               uint32_t t, int rec, int type, uint64_t seed, uint32_t step,
+                               double del_t, bool claim, int *p_out, int *tns) {
+  *p_out = -1;
+  *tns = 0;
   const int box = (int)(k / W.P_stride);
   BoxDesc *B = &boxes[box];
-  if (B->error) return;
+  if (B->error) return -1;
   const int nb = B->n_bin, nc = B->w.n_class;
   const int64_t lb = (int64_t)box * W.K_stride;
   const int64_t pb = (int64_t)box * W.P_stride;
@@ -1089,15 +1087,15 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
   const int n_t = W.lcnt[lb + par_key(b_cur, ct, nb)];
   const int e = n_t - (int)(t - W.pair_exc[k]);
   const int p = W.perm[B->off + W.lstart[lb + par_key(b_cur, ct, nb)] + e - 1];
-  if (W.status[p] == ST_HOLE) return;  // removed earlier in this step
+  if (W.status[p] == ST_HOLE) return 0;  // removed earlier in this step
   if (W.key[p] != par_key(b_cur, ct, nb)) {
     // changed in place earlier in the step: walk on from its current list
     const int kp = W.key[p];
-    if (kp / nb + 1 != ct) return;
+    if (kp / nb + 1 != ct) return 0;
     b_cur = kp % nb + 1;
     q = next_acc_pair(W, pacc, nb, nc, ct, b_cur, q - 1);
   }
-  W.tp1[i] = p;
+  *p_out = p;
   const int ns = A.n_spec;
   const Weights &w = B->w;
   const uint32_t ordinal = (uint32_t)(p - B->off) + 1;
@@ -1128,14 +1126,14 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
       // copies draw from their own sites, above the box's nc^2 nb^2 pair sites
       const uint32_t site = pair_site(nb, nc, b1, b2, c1, c2) + tag * (uint32_t)(nc * nc * nb * nb);
       Stream aps, smp, fin;
-      aps.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_POISSON, round));
-      smp.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_SAMPLE, round));
-      fin.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_FINISH, round));
+      aps.open(seed, box, step, site, ordinal, RNG_ACC_POISSON);
+      smp.open(seed, box, step, site, ordinal, RNG_ACC_SAMPLE);
+      fin.open(seed, box, step, site, ordinal, RNG_ACC_FINISH);
       if (walker >= 0 && !marked) {
         // owner marker for this copy's removals (pushed before any of them)
         if (!rm_push(W, chain, -1 - (int)tag)) {
           atomicExch(&B->error, ERR_CAPACITY);
-          return;
+          return -1;
         }
         marked = true;
       }
@@ -1143,11 +1141,12 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
       int n_samp, n_coag;
       int err = acc_sample_source(A, W, B, type, cur, ct, cs, nc_at_radius(w, cs, edges[bs - 1]),
                                   B->off + W.lstart[lb + ks], ks, n_s, k_max * del_t * (double)n_s,
-                                  1.0 / k_max, aps, smp, fin, t, chain, &n_samp, &n_coag, &warn,
+                                  1.0 / k_max, aps, smp, fin, t, claim, chain, &n_samp, &n_coag,
+                                  &warn,
                                   source);
       if (err) {
         atomicExch(&B->error, err);
-        return;
+        return -1;
       }
       n_rm += chain.n - n_before;
       n_samp_tot += n_samp;
@@ -1155,13 +1154,13 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
       if (n_coag > 0) {
         // coag_target_with_source
         Stream ft;
-        ft.open(seed, box, step, site, ordinal, acc_kind(RNG_ACC_TARGET, round));
+        ft.open(seed, box, step, site, ordinal, RNG_ACC_TARGET);
         const double old_nc = nc_at_radius(w, cur.klass, vol2rad(rec_vol(ns, cur)));
         Rec np;
         err = par_rec_coagulate(ns, ft, cur, source, np);
         if (err) {
           atomicExch(&B->error, err);
-          return;
+          return -1;
         }
         np.id = cur.id;
         const double r_new = vol2rad(rec_vol(ns, np));
@@ -1184,7 +1183,7 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
           const int d = W.acc_cap + atomicAdd(W.dup_top, 1);
           if (d >= W.pool_cap) {
             atomicExch(&B->error, ERR_CAPACITY);
-            return;
+            return -1;
           }
           Rec cp = cur;
           cp.group = rand_group_w(ft, w, cur.klass, r_new);
@@ -1224,8 +1223,8 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
   }
   if (warn) flags |= RF_WARN;
   if (n_coag_tot == 0 && n_rm == 0 && err_code == 0) {
-    W.tns[i] = n_samp_tot | (warn ? ACC_WARN : 0);
-    return;
+    *tns = n_samp_tot | (warn ? ACC_WARN : 0);
+    return 0;
   }
   if (flags & RF_MERGED) pool_store(W, ns, rec, target);
   W.r_nsamp[rec] = n_samp_tot;
@@ -1238,8 +1237,44 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
   W.r_dhead[rec] = dup_head;
   W.r_flags[rec] = flags;
   W.r_err[rec] = err_code;
-  W.acc[i] = 1;
-  atomicMin(&W.claim[p], t);
+  if (claim) atomicMin(&W.claim[p], t);
+  return 1;
+}
+
+// Round 0 runs one thread per target slot of the batch (record a maps to
+// its slot through acc_exc); later rounds run over the pending slots.
+__global__ void __launch_bounds__(64)
+k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed,
+                uint32_t step, uint32_t t0, int n, const uint32_t *pend, double del_t,
+                uint32_t acc_base, int n_acc) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t NP = (int64_t)n_box * W.P_stride;
+  uint32_t t;
+  int i;
+  int64_t k;
+  if (pend) {
+    if (idx >= n) return;
+    i = idx;
+    t = pend[i];
+    k = find_pair(W.pair_exc, NP, t);
+    if (!W.pair_acc[k]) return;
+  } else {
+    if (idx >= n_acc) return;
+    const uint32_t a = acc_base + (uint32_t)idx;
+    k = find_pair(W.acc_exc, NP, a);
+    t = W.pair_exc[k] + (a - W.acc_exc[k]);
+    i = (int)(t - t0);
+  }
+  const int rec = (int)(W.acc_exc[k] + (t - W.pair_exc[k]) - acc_base);
+  W.acc[i] = 0;
+  W.tid[i] = t;
+  W.tp2[i] = -1;
+  W.trec[i] = rec;
+  int p, tns;
+  const int r = acc_walk(boxes, A, W, k, t, rec, type, seed, step, del_t, true, &p, &tns);
+  W.tp1[i] = p;
+  W.tns[i] = tns;
+  if (r == 1) W.acc[i] = 1;
 }
 
 // the largest batch end t1 <= min(T, t0 + T_cap) whose target slots fit the
@@ -1423,6 +1458,120 @@ __device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc 
   if (want_terms) W.terms[slot] = particle_terms(type, v, m / v, B->env);
 }
 
+// Where a merge puts what it creates.  In a claim round the positions come
+// from the round's scans over the events (RankAlloc, one per event); in the
+// sequential tail they come straight from the box's counters (SeqAlloc).
+struct RankAlloc {
+  BoxDesc *B;
+  int64_t app, id, info, hole;  // this event's next ranks
+  __device__ int64_t append_slot() {
+    int64_t r = app++;
+    return B->n_part + r < B->cap ? B->off + B->n_part + r : -1;
+  }
+  __device__ int64_t new_id() {
+    return (B->next_id + id++ - 1) * B->id_stride + B->id_offset + 1;
+  }
+  __device__ int64_t info_index() { return B->n_info + info++; }
+  __device__ int64_t hole_index() { return hole++; }
+};
+
+struct SeqAlloc {
+  BoxDesc *B;
+  unsigned long long *hole_top;
+  __device__ int64_t append_slot() {
+    if (B->n_part >= B->cap) return -1;
+    return B->off + B->n_part++;
+  }
+  __device__ int64_t new_id() {
+    int64_t i = (B->next_id - 1) * B->id_stride + B->id_offset + 1;
+    B->next_id += 1;
+    return i;
+  }
+  __device__ int64_t info_index() { return B->n_info++; }
+  __device__ int64_t hole_index() { return (int64_t)atomicAdd(hole_top, 1ull); }
+};
+
+template <class Alloc>
+__device__ inline void put_info(const Arrays &A, BoxDesc *B, Alloc &al, int64_t id, int action,
+                                int64_t other) {
+  int64_t k = al.info_index();
+  if (k < B->info_cap) {
+    A.info_id[B->info_off + k] = id;
+    A.info_action[B->info_off + k] = action;
+    A.info_other[B->info_off + k] = other;
+  }
+}
+
+// coagulate_weighting + coagulate (src/coagulation.F90:810-985) of an
+// accepted per-set test: removals become holes (or the slot the merged
+// particle takes), the merged particle is written, removal records follow
+// the reference's order
+template <class Alloc>
+__device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, int box, int64_t k,
+                                    uint32_t t, int64_t p1, int64_t p2, uint64_t seed,
+                                    uint32_t step, int type, int want_terms, int rid, Alloc &al) {
+  int c1, c2, b1, b2;
+  decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
+  const uint32_t j = t - W.pair_exc[k];
+  atomicAdd((unsigned long long *)&B->n_coag, 1ull);
+  atomicAdd((unsigned long long *)&B->step_coag, 1ull);
+  const int nb = B->n_bin, nc = B->w.n_class;
+  const int cc = A.cc[table_index(A, box, nb, nc, b1, b2, c1, c2)];
+  int *live = W.live + (int64_t)box * W.K_stride;
+  Stream s;
+  s.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), j, RNG_MERGE);
+  int new_group;
+  const int o = merge_decide(A, B->w, s, p1, p2, c1, c2, cc, &new_group);
+  Rec pt1, pt2, ptc;
+  load_rec(A, p1, pt1);
+  load_rec(A, p2, pt2);
+  int64_t info_other = 0;
+  int64_t slot = -1;
+  if (o & O_NEW) {
+    int err = par_rec_coagulate(A.n_spec, s, pt1, pt2, ptc);
+    if (err) atomicExch(&B->error, err);
+    ptc.group = new_group;
+    ptc.klass = cc;
+    if (o & O_ID1) ptc.id = pt1.id;
+    else if (o & O_ID2) ptc.id = pt2.id;
+    else ptc.id = al.new_id();
+    info_other = ptc.id;
+    if (o & O_SLOT1) slot = p1;
+    else if (o & O_SLOT2) slot = p2;
+    else {
+      slot = al.append_slot();
+      if (slot < 0) {
+        atomicExch(&B->error, ERR_CAPACITY);
+        return;
+      }
+    }
+  }
+  if (o & O_RM1) {
+    W.dead_t[p1] = t;
+    W.dead_r[p1] = rid;
+    atomicSub(&live[W.key[p1]], 1);
+    if (!(o & O_SLOT1)) {
+      W.status[p1] = ST_HOLE;
+      W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(p1 - B->off);
+    }
+  }
+  if (o & O_RM2) {
+    W.dead_t[p2] = t;
+    W.dead_r[p2] = rid;
+    atomicSub(&live[W.key[p2]], 1);
+    if (!(o & O_SLOT2)) {
+      W.status[p2] = ST_HOLE;
+      W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(p2 - B->off);
+    }
+  }
+  if (slot >= 0) {
+    write_new_particle(A, W, B, slot, ptc, nb, type, want_terms);
+    atomicAdd(&live[W.key[slot]], 1);
+  }
+  if (o & O_INFO1) put_info(A, B, al, pt1.id, 2, info_other);
+  if (o & O_INFO2) put_info(A, B, al, pt2.id, 2, info_other);
+}
+
 // a winning target event (coag_target_with_source and
 // aero_state_dup_particle, src/coagulation.F90:522-582,
 // src/aero_state.F90:488-539): removed sources become holes with a
@@ -1430,34 +1579,22 @@ __device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc 
 // of its copies), the merged target replaces the old one in place, copies
 // are appended with new identifiers in creation order, and a target or
 // copy that kept no copy of itself is removed with a weighting record.
-__device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, int e, int type,
-                                 int want_terms, int64_t hole_base, int rid) {
-  const uint32_t t = W.ev_t[e];
-  const int r = W.ev_rec[e];
-  const int64_t p = W.ev_p1[e];
-  const int box = W.ev_box[e];
-  BoxDesc *B = &boxes[box];
+template <class Alloc>
+__device__ inline void acc_apply(BoxDesc *B, const Arrays &A, const PW &W, int box, uint32_t t,
+                                 int r, int64_t p, int type, int want_terms, int rid, Alloc &al) {
   int *live = W.live + (int64_t)box * W.K_stride;
-  const int first = W.box_first[box];
   const int ns = A.n_spec;
   const int nb = B->n_bin;
   const int64_t target_id = A.id[p];
   const int klass = A.klass[p];
   const int flags = W.r_flags[r];
-  const int64_t id_base = W.s_id[e] - W.s_id[first];
-  auto copy_id = [&](int k) {  // identifier of copy k (1-based, creation order)
-    return (B->next_id + id_base + k - 2) * B->id_stride + B->id_offset + 1;
-  };
-  int64_t hrank = hole_base + W.s_hole[e];
-  int64_t irank = B->n_info + (W.s_info[e] - W.s_info[first]);
-  auto info = [&](int64_t id, int action, int64_t other) {
-    if (irank < B->info_cap) {
-      A.info_id[B->info_off + irank] = id;
-      A.info_action[B->info_off + irank] = action;
-      A.info_other[B->info_off + irank] = other;
-    }
-    irank++;
-  };
+  // copy identifiers in creation order, drawn before any record names them
+  int64_t id_first = 0;
+  for (int kk = 0; kk < W.r_ndup[r]; ++kk) {
+    int64_t v = al.new_id();
+    if (kk == 0) id_first = v;
+  }
+  auto copy_id = [&](int kk) { return id_first + (int64_t)(kk - 1) * B->id_stride; };
   int64_t owner = target_id;
   chain_walk(W, r, [&](int q) {
     if (q < 0) {
@@ -1468,8 +1605,8 @@ __device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, i
     W.dead_r[q] = rid;
     atomicSub(&live[W.key[q]], 1);
     W.status[q] = ST_HOLE;
-    W.holes[hrank++] = ((uint64_t)box << 32) | (uint64_t)(q - B->off);
-    info(A.id[q], 2, owner);
+    W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(q - B->off);
+    put_info(A, B, al, A.id[q], 2, owner);
   });
   const unsigned long long n_samp = (unsigned long long)W.r_nsamp[r];
   const unsigned long long n_coag = (unsigned long long)W.r_ncoag[r];
@@ -1488,126 +1625,47 @@ __device__ inline void acc_apply(BoxDesc *boxes, const Arrays &A, const PW &W, i
   write_new_particle(A, W, B, p, np, nb, type, want_terms);
   if (flags & RF_DIED) {
     W.status[p] = ST_HOLE;
-    W.holes[hrank++] = ((uint64_t)box << 32) | (uint64_t)(p - B->off);
-    info(target_id, 4, 0);
+    W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(p - B->off);
+    put_info(A, B, al, target_id, 4, 0);
   } else {
     atomicAdd(&live[W.key[p]], 1);
   }
-  int64_t app_rank = W.s_app[e] - W.s_app[first];
-  int k = 0;
+  int kk = 0;
   for (int d = W.r_dhead[r]; d >= 0; d = W.q_next[d]) {
-    ++k;
+    ++kk;
     if (W.q_dead[d]) {
-      info(copy_id(k), 4, 0);
+      put_info(A, B, al, copy_id(kk), 4, 0);
       continue;
     }
-    pool_load(W, ns, d, klass, copy_id(k), np);
-    if (B->n_part + app_rank >= B->cap) {
+    pool_load(W, ns, d, klass, copy_id(kk), np);
+    const int64_t slot = al.append_slot();
+    if (slot < 0) {
       atomicExch(&B->error, ERR_CAPACITY);
       return;
     }
-    const int64_t slot = B->off + B->n_part + app_rank;
-    app_rank++;
     write_new_particle(A, W, B, slot, np, nb, type, want_terms);
     atomicAdd(&live[W.key[slot]], 1);
   }
 }
 
-// apply the winners' merges (coagulate, src/coagulation.F90:928-985)
+// apply the round's winners, each at the positions its ranks give
 __global__ void k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, uint64_t seed,
-                            uint32_t step, int round, int type, int want_terms,
-                            int64_t hole_base, int rid) {
+                            uint32_t step, int type, int want_terms, int64_t hole_base, int rid) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
-  int o = W.ev_out[e];
-  if (!(o & O_WIN)) return;
+  if (!(W.ev_out[e] & O_WIN)) return;
+  const uint32_t t = W.ev_t[e];
+  const int box = W.ev_box[e];
+  BoxDesc *B = &boxes[box];
+  const int first = W.box_first[box];
+  RankAlloc al{B, W.s_app[e] - W.s_app[first], W.s_id[e] - W.s_id[first],
+               W.s_info[e] - W.s_info[first], hole_base + W.s_hole[e]};
   if (W.ev_rec[e] >= 0) {
-    acc_apply(boxes, A, W, e, type, want_terms, hole_base, rid);
+    acc_apply(B, A, W, box, t, W.ev_rec[e], W.ev_p1[e], type, want_terms, rid, al);
     return;
   }
-  uint32_t t = W.ev_t[e];
-  int64_t p1 = W.ev_p1[e], p2 = W.ev_p2[e];
-  MergeCtx m = merge_ctx(boxes, W, n_box, t);
-  BoxDesc *B = &boxes[m.box];
-  atomicAdd((unsigned long long *)&B->n_coag, 1ull);
-  atomicAdd((unsigned long long *)&B->step_coag, 1ull);
-  const int nb = B->n_bin, nc = B->w.n_class;
-  const int cc = A.cc[table_index(A, m.box, nb, nc, m.b1, m.b2, m.c1, m.c2)];
-  int *live = W.live + (int64_t)m.box * W.K_stride;
-  const int first = W.box_first[m.box];
-  Stream s;
-  s.open(seed, m.box, step, pair_site(nb, nc, m.b1, m.b2, m.c1, m.c2), m.j, RNG_MERGE);
-  int new_group;
-  (void)merge_decide(A, B->w, s, p1, p2, m.c1, m.c2, cc, &new_group);
-  Rec pt1, pt2, ptc;
-  load_rec(A, p1, pt1);
-  load_rec(A, p2, pt2);
-  int64_t info_other = 0;
-  int64_t slot = -1;
-  if (o & O_NEW) {
-    int err = par_rec_coagulate(A.n_spec, s, pt1, pt2, ptc);
-    if (err) atomicExch(&B->error, err);
-    ptc.group = new_group;
-    ptc.klass = cc;
-    if (o & O_ID1) ptc.id = pt1.id;
-    else if (o & O_ID2) ptc.id = pt2.id;
-    else {
-      int64_t rank = W.s_id[e] - W.s_id[first];
-      ptc.id = (B->next_id + rank - 1) * B->id_stride + B->id_offset + 1;
-    }
-    info_other = ptc.id;
-    if (o & O_SLOT1) slot = p1;
-    else if (o & O_SLOT2) slot = p2;
-    else {
-      int64_t rank = W.s_app[e] - W.s_app[first];
-      if (B->n_part + rank >= B->cap) {
-        atomicExch(&B->error, ERR_CAPACITY);
-        return;
-      }
-      slot = B->off + B->n_part + rank;
-    }
-  }
-  // removals: live counts and holes
-  int64_t hrank = hole_base + W.s_hole[e];
-  if (o & O_RM1) {
-    W.dead_t[p1] = t;
-    W.dead_r[p1] = rid;
-    atomicSub(&live[W.key[p1]], 1);
-    if (!(o & O_SLOT1)) {
-      W.status[p1] = ST_HOLE;
-      W.holes[hrank++] = ((uint64_t)m.box << 32) | (uint64_t)(p1 - B->off);
-    }
-  }
-  if (o & O_RM2) {
-    W.dead_t[p2] = t;
-    W.dead_r[p2] = rid;
-    atomicSub(&live[W.key[p2]], 1);
-    if (!(o & O_SLOT2)) {
-      W.status[p2] = ST_HOLE;
-      W.holes[hrank++] = ((uint64_t)m.box << 32) | (uint64_t)(p2 - B->off);
-    }
-  }
-  if (slot >= 0) {
-    write_new_particle(A, W, B, slot, ptc, nb, type, want_terms);
-    atomicAdd(&live[W.key[slot]], 1);
-  }
-  // removal records (aero_info), in event order within the box
-  int64_t irank = B->n_info + (W.s_info[e] - W.s_info[first]);
-  if (o & O_INFO1) {
-    if (irank < B->info_cap) {
-      A.info_id[B->info_off + irank] = pt1.id;
-      A.info_action[B->info_off + irank] = 2;
-      A.info_other[B->info_off + irank] = info_other;
-    }
-    irank++;
-  }
-  if (o & O_INFO2) {
-    if (irank < B->info_cap) {
-      A.info_id[B->info_off + irank] = pt2.id;
-      A.info_action[B->info_off + irank] = 2;
-      A.info_other[B->info_off + irank] = info_other;
-    }
-  }
+  const int64_t k = find_pair(W.pair_exc, (int64_t)n_box * W.P_stride, t);
+  perset_merge(B, A, W, box, k, t, W.ev_p1[e], W.ev_p2[e], seed, step, type, want_terms, rid, al);
 }
 
 // per-box totals of a round: appended slots, new ids, records, winners
@@ -1661,8 +1719,8 @@ __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
   if (W.acc[i]) {
     f = (W.ev_out[W.acc_scan[i]] & O_WIN) ? 0 : 1;
   } else if (W.trec[i] >= 0) {
-    // a target slot without effect: final unless its target was consumed
-    // by an earlier winner; its samples count once final
+    // a target slot without effect: final unless an earlier winner of the
+    // round changed its target; its samples count once final
     f = consumed_before(W, W.tp1[i], t, rid) ? 1 : 0;
     int v = W.tns[i];
     if (!f && v) {
@@ -1676,6 +1734,11 @@ __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
   } else if (W.tp1[i] >= 0) {
     f = (consumed_before(W, W.tp1[i], t, rid) || consumed_before(W, W.tp2[i], t, rid)) ? 1 : 0;
   }
+  if (!f && W.trec[i] < 0 && (W.tns[i] & ACC_WARN)) {
+    // a per-set test's k_max warning (src/coagulation.F90:749-751), once final
+    const int64_t k = find_pair(W.pair_exc, (int64_t)n_box * W.P_stride, t);
+    atomicAdd((unsigned long long *)&boxes[k / W.P_stride].n_warn, 1ull);
+  }
   W.f_pend[i] = f;
 }
 
@@ -1683,6 +1746,59 @@ __global__ void k_par_gather_pending(PW W, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (W.f_pend[i]) W.pend[W.s_pend[i]] = W.tid[i];
+}
+
+// first index of the sorted list v[0..n) holding a value >= x
+__device__ inline int lower_bound_u32(const uint32_t *v, int n, uint32_t x) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (v[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// The sequential tail of a batch: the tests still pending after the claim
+// rounds run per box in sequential order on the current state, which is
+// the reference's own order, so they need no claims.  One thread per box.
+__global__ void k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n,
+                           const uint32_t *pend, int type, uint64_t seed, uint32_t step,
+                           int precision, double band, double del_t, uint32_t acc_base,
+                           int want_terms, int rid) {
+  const int box = blockIdx.x * blockDim.x + threadIdx.x;
+  if (box >= n_box) return;
+  BoxDesc *B = &boxes[box];
+  const int64_t NP = (int64_t)n_box * W.P_stride;
+  const int lo = lower_bound_u32(pend, n, W.pair_exc[(int64_t)box * W.P_stride]);
+  const int hi = lower_bound_u32(pend, n, W.pair_exc[(int64_t)(box + 1) * W.P_stride]);
+  SeqAlloc al{B, W.hole_top};
+  for (int q = lo; q < hi && !B->error; ++q) {
+    const uint32_t t = pend[q];
+    const int64_t k = find_pair(W.pair_exc, NP, t);
+    if (W.pair_acc[k]) {
+      const int rec = (int)(W.acc_exc[k] + (t - W.pair_exc[k]) - acc_base);
+      int p, tns;
+      const int r = acc_walk(boxes, A, W, k, t, rec, type, seed, step, del_t, false, &p, &tns);
+      if (r == 1) {
+        acc_apply(B, A, W, box, t, rec, p, type, want_terms, rid, al);
+      } else if (r == 0 && tns) {
+        unsigned long long s = (unsigned long long)(tns & (ACC_WARN - 1));
+        B->n_samp += (int64_t)s;
+        B->step_samp += (int64_t)s;
+        if (tns & ACC_WARN) B->n_warn += 1;
+      }
+    } else {
+      TestOut o;
+      perset_test(B, A, W, box, k, t, type, seed, step, precision, band, 0, o);
+      if (o.warn) B->n_warn += 1;
+      if (o.accept)
+        perset_merge(B, A, W, box, k, t, o.p1, o.p2, seed, step, type, want_terms, rid, al);
+    }
+  }
+}
+
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 __global__ void k_fill_int(int *p, int64_t n, int v) {
@@ -1796,6 +1912,7 @@ inline int parallel_init(ParallelWork &W, int n_box, int max_bin) {
   W.d.tab_classes = 1;
   if (hipHostMalloc((void **)&W.h_u32, 64 * sizeof(uint32_t)) != hipSuccess) return ERR_HIP;
   if (hipHostMalloc((void **)&W.h_i32, 64 * sizeof(int)) != hipSuccess) return ERR_HIP;
+  if (hipHostMalloc((void **)&W.h_u64, 8 * sizeof(unsigned long long)) != hipSuccess) return ERR_HIP;
   return 0;
 }
 
@@ -1848,7 +1965,8 @@ inline void parallel_free_arrays(ParallelWork &W) {
                   d.f_lose, d.f_pend, d.s_pend, d.dead_t, d.dead_r, d.s_app, d.s_id, d.s_info, d.s_hole,
                   d.s_lose, d.box_first,
                   d.holes, d.holes_sorted, d.tail_live, W.scan_tmp, W.sort_tmp,
-                  d.pair_acc, d.pair_accn, d.acc_exc, d.trec, d.tns, d.ev_rec, d.misc};
+                  d.pair_acc, d.pair_accn, d.acc_exc, d.trec, d.tns, d.ev_rec, d.misc,
+                  d.hole_top};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   parallel_free_records(W);
@@ -1868,6 +1986,8 @@ inline void parallel_free(ParallelWork &W) {
   parallel_free_arrays(W);
   if (W.h_u32) (void)hipHostFree(W.h_u32);
   if (W.h_i32) (void)hipHostFree(W.h_i32);
+  if (W.h_u64) (void)hipHostFree(W.h_u64);
+  W.h_u64 = nullptr;
   W.h_u32 = nullptr;
   W.h_i32 = nullptr;
 }
@@ -1929,6 +2049,7 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.acc_exc, W.NP + 1));
   PCHECK(hipMemset(d.pair_accn, 0, sizeof(uint32_t) * (W.NP + 1)));
   PCHECK(pmalloc(&d.misc, 4));
+  PCHECK(pmalloc(&d.hole_top, 1));
   const int64_t T = W.T_cap;
   PCHECK(pmalloc(&d.acc, T + 1));
   PCHECK(pmalloc(&d.acc_scan, T + 1));
@@ -2066,18 +2187,35 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
     t0 = t1;
     const uint32_t *pend = nullptr;
     for (int round = 0; n > 0; ++round) {
-      if (round >= MAX_ROUNDS) break;
+      if (round >= PAR_ROUNDS) {
+        // the rest in sequential order, per box
+        if (n_acc > 0) {
+          hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.rm_top, 1, 0);
+          hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.dup_top, 1, 0);
+        }
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, d.hole_top,
+                           (unsigned long long)n_holes);
+        PLAUNCH(W, "k_par_tail", k_par_tail, dim3((n_box + 63) / 64), dim3(64), 0, st, d_box, A,
+                d, n_box, n, pend, type, seed, step, precision, band, del_t, a0, want_terms,
+                (int)W.n_rounds);
+        PCHECK(hipMemcpyAsync(W.h_u64, d.hole_top, sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, st));
+        PCHECK(hipStreamSynchronize(st));
+        n_holes = (int64_t)W.h_u64[0];
+        W.n_tail += n;
+        break;
+      }
       W.n_rounds++;
       if (n_acc > 0) {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.rm_top, 1, 0);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.dup_top, 1, 0);
       }
       PLAUNCH(W, "k_par_tests", k_par_tests, dim3((n + 255) / 256), dim3(256), 0, st, d_box, A, d,
-              n_box, type, seed, step, round, batch_t0, n, pend, precision, band, validation);
+              n_box, type, seed, step, batch_t0, n, pend, precision, band, validation);
       if (n_acc > 0) {
         const int na = pend ? n : n_acc;
         PLAUNCH(W, "k_par_acc_tests", k_par_acc_tests, dim3((na + 63) / 64), dim3(64), 0, st,
-                d_box, A, d, n_box, type, seed, step, round, batch_t0, n, pend, del_t, a0, n_acc);
+                d_box, A, d, n_box, type, seed, step, batch_t0, n, pend, del_t, a0, n_acc);
       }
       hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.acc + n, 1, 0);
       PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.acc, d.acc_scan, n + 1, st));
@@ -2098,7 +2236,7 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, fs[q], ss[q], E + 1, st));
       }
       PLAUNCH(W, "k_par_apply", k_par_apply, dim3((E + 255) / 256), dim3(256), 0, st, d_box, A, d,
-              n_box, E, seed, step, round, type, want_terms, n_holes, (int)W.n_rounds);
+              n_box, E, seed, step, type, want_terms, n_holes, (int)W.n_rounds);
       hipLaunchKernelGGL(k_par_box_totals, gb, bb, 0, st, d_box, d, n_box, E);
       hipLaunchKernelGGL(k_par_release, dim3((E + 255) / 256), dim3(256), 0, st, d, E);
       hipLaunchKernelGGL(k_par_pending, dim3((n + 255) / 256), dim3(256), 0, st, d_box, d, n_box, n,
@@ -2112,7 +2250,6 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
       n_holes += W.h_i32[2];
       n = W.h_i32[1];
       pend = d.pend;
-      if (round + 1 >= MAX_ROUNDS) W.n_dropped += n;
       if (n > 0) {
         // pending tests must not alias the array the next round reads
         PCHECK(hipMemcpyAsync(d.tid, d.pend, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
