@@ -67,7 +67,7 @@ def kernel_bytes(name, n_spec, precision, part_steps_per_launch, tests_per_launc
 
 
 TIMED = ("k_prepare", "k_exact_sweep", "k_par_hist", "k_par_scatter", "k_par_pairs",
-         "k_par_tests", "k_par_apply", "k_par_compact")
+         "k_par_tests", "k_par_acc_tests", "k_par_apply", "k_par_tail", "k_par_compact")
 
 
 def run_gpu(pops, kernel, dt, mode, precision, steps, warmup, seed=20261202):
@@ -91,7 +91,12 @@ def run_gpu(pops, kernel, dt, mode, precision, steps, warmup, seed=20261202):
     g.sync()
     elapsed = time.perf_counter() - t0
     c = g.counters()
+    st = g.path_stats()
     out = {
+        "claim_rounds": st["n_rounds"],
+        "tail_tests": st["n_tail"],
+        "tail_rounds": st["n_tail_rounds"],
+        "accel_pairs": int(st["n_accel_pairs"].sum()),
         "elapsed": elapsed,
         "n_samp": int(c["n_samp"].sum()),
         "n_coag": int(c["n_coag"].sum()),
@@ -211,6 +216,10 @@ def main():
         "test_pairs": n_samp,
         "coag_events": n_coag,
         "kmax_warnings": r["n_warn"],
+        "claim_rounds_per_step": r["claim_rounds"] / args.steps,
+        "tail_tests_per_step": r["tail_tests"] / args.steps,
+        "tail_rounds_per_step": r["tail_rounds"] / args.steps,
+        "accel_pairs_per_step": r["accel_pairs"] / args.steps,
         "fp64_rechecks": r["n_recheck"],
         "kernel_ms": {k: v[0] / v[1] for k, v in timings.items()},
         "kernel_total_ms": {k: v[0] for k, v in timings.items()},

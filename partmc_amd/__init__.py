@@ -240,10 +240,10 @@ class CoagGPU:
         tail."""
         acc = np.zeros(self.n_box, dtype=np.int64)
         drop = np.zeros(self.n_box, dtype=np.int64)
-        rounds = np.zeros(2, dtype=np.int64)
+        rounds = np.zeros(3, dtype=np.int64)
         _check(load().pmc_gpu_get_path_stats(self._h, _ptr(acc), _ptr(drop), _ptr(rounds)))
         return {"n_accel_pairs": acc, "n_dropped": drop, "n_rounds": int(rounds[0]),
-                "n_tail": int(rounds[1])}
+                "n_tail": int(rounds[1]), "n_tail_rounds": int(rounds[2])}
 
     def reserve(self, capacity):
         cap = np.ascontiguousarray(capacity, dtype=np.int64)

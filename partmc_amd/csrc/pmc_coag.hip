@@ -756,6 +756,7 @@ int pmc_gpu_reset_counters(pmc_gpu *h) {
   }
   h->work.n_rounds = 0;
   h->work.n_tail = 0;
+  h->work.n_tail_rounds = 0;
   HIPCHECK(hipMemcpy(h->d_box, hb.data(), sizeof(BoxDesc) * h->n_box, hipMemcpyHostToDevice));
   return 0;
 }
@@ -784,6 +785,7 @@ int pmc_gpu_get_path_stats(pmc_gpu *h, int64_t *n_accel_pairs, int64_t *n_droppe
   }
   rounds[0] = h->work.n_rounds;
   rounds[1] = h->work.n_tail;
+  rounds[2] = h->work.n_tail_rounds;
   return 0;
 }
 

@@ -41,7 +41,8 @@ constexpr int TILE_ITEMS = 16;
 constexpr int TILE = TILE_THREADS * TILE_ITEMS;
 constexpr uint32_t UNCLAIMED = 0xFFFFFFFFu;
 constexpr uint8_t ST_LIVE = 0, ST_HOLE = 1, ST_NEW = 2;
-constexpr int PAR_ROUNDS = 8;  // claim rounds before the sequential tail
+constexpr int PAR_ROUNDS = 3;  // device-wide claim rounds before the per-box tail
+static_assert(PAR_ROUNDS >= 1, "the tail runs on the pending list of a round");
 constexpr int MAX_REDRAW = 64;
 
 struct PW {  // device view of the parallel work arrays
@@ -94,7 +95,8 @@ struct PW {  // device view of the parallel work arrays
   int *rm_top;                 // [1] chunk allocator of the round
   int rm_cap;
   uint32_t *misc;              // [4] batch bounds from k_batch_end
-  unsigned long long *hole_top;  // [1] hole counter of the sequential tail
+  unsigned long long *hole_top;  // [1] hole counter of the tail
+  unsigned long long *tail_rounds;  // [1] claim rounds run by the tail, summed over boxes
 };
 
 constexpr int ACC_CAP = 1 << 18;
@@ -123,7 +125,8 @@ struct ParallelWork {
   void (*tbegin)(void *, const char *, hipEvent_t *) = nullptr;
   void (*tend)(void *, const char *, hipEvent_t) = nullptr;
   void *tctx = nullptr;
-  int64_t n_tail = 0;     // tests finished in the sequential tail
+  int64_t n_tail = 0;     // tests finished in the tail
+  int64_t n_tail_rounds = 0;  // the tail's claim rounds, summed over boxes
   int64_t n_rounds = 0;   // claim rounds run
 };
 
@@ -1758,43 +1761,172 @@ __device__ inline int lower_bound_u32(const uint32_t *v, int n, uint32_t x) {
   return lo;
 }
 
-// The sequential tail of a batch: the tests still pending after the claim
-// rounds run per box in sequential order on the current state, which is
-// the reference's own order, so they need no claims.  One thread per box.
-__global__ void k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n,
-                           const uint32_t *pend, int type, uint64_t seed, uint32_t step,
-                           int precision, double band, double del_t, uint32_t acc_base,
-                           int want_terms, int rid) {
-  const int box = blockIdx.x * blockDim.x + threadIdx.x;
-  if (box >= n_box) return;
+constexpr int TAIL_THREADS = 64;
+
+// a phase boundary of the tail: earlier writes and atomics of the box's
+// wavefront become visible to its next phase (device-scope fence, then the
+// barrier)
+__device__ inline void tail_barrier() {
+  __threadfence();
+  __syncthreads();
+}
+
+// The tail of a batch: the tests still pending after the device-wide claim
+// rounds, which are mostly chains of tests on one particle (a large
+// particle drawn by many tests of its bin pairs).  One wavefront per box
+// runs claim rounds over a window of the box's next TAIL_THREADS pending
+// tests, in sequential order, with wave barriers in place of launches: the
+// earliest test of the window always wins, so every round makes progress.
+// The box's pending run of `pend` serves as its queue: re-runs are put back
+// in front of the tests not yet tried, keeping sequential order.
+__global__ void __launch_bounds__(TAIL_THREADS)
+k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, uint32_t *pend, int type,
+           uint64_t seed, uint32_t step, int precision, double band, double del_t,
+           uint32_t acc_base, int want_terms, int rid0, unsigned long long *rounds_out) {
+  const int box = blockIdx.x;
+  const int tid = threadIdx.x;
   BoxDesc *B = &boxes[box];
   const int64_t NP = (int64_t)n_box * W.P_stride;
   const int lo = lower_bound_u32(pend, n, W.pair_exc[(int64_t)box * W.P_stride]);
   const int hi = lower_bound_u32(pend, n, W.pair_exc[(int64_t)(box + 1) * W.P_stride]);
-  SeqAlloc al{B, W.hole_top};
-  for (int q = lo; q < hi && !B->error; ++q) {
-    const uint32_t t = pend[q];
-    const int64_t k = find_pair(W.pair_exc, NP, t);
-    if (W.pair_acc[k]) {
-      const int rec = (int)(W.acc_exc[k] + (t - W.pair_exc[k]) - acc_base);
-      int p, tns;
-      const int r = acc_walk(boxes, A, W, k, t, rec, type, seed, step, del_t, false, &p, &tns);
-      if (r == 1) {
-        acc_apply(B, A, W, box, t, rec, p, type, want_terms, rid, al);
-      } else if (r == 0 && tns) {
-        unsigned long long s = (unsigned long long)(tns & (ACC_WARN - 1));
-        B->n_samp += (int64_t)s;
-        B->step_samp += (int64_t)s;
-        if (tns & ACC_WARN) B->n_warn += 1;
+  if (lo >= hi) return;
+  using BS = hipcub::BlockScan<int, TAIL_THREADS>;
+  __shared__ typename BS::TempStorage scan_tmp;
+  __shared__ int s_tot[5];
+  __shared__ long long s_hole_base;
+  __shared__ uint32_t s_keep[TAIL_THREADS];
+  __shared__ int s_err;
+  int head = lo;
+  int rid = rid0;
+  unsigned long long n_rounds = 0;
+  while (head < hi) {
+    if (tid == 0) s_err = B->error;
+    tail_barrier();
+    if (s_err) break;
+    const int wn = min(TAIL_THREADS, hi - head);
+    const bool active = tid < wn;
+    uint32_t t = active ? pend[head + tid] : 0u;
+    int64_t k = active ? find_pair(W.pair_exc, NP, t) : 0;
+    const bool is_acc = active && W.pair_acc[k];
+    int p1 = -1, p2 = -1, rec = -1, tns = 0;
+    bool effect = false, warn = false;
+    // 1. run the tests of the window (claims as in a device-wide round)
+    if (active) {
+      if (is_acc) {
+        rec = (int)(W.acc_exc[k] + (t - W.pair_exc[k]) - acc_base);
+        const int r = acc_walk(boxes, A, W, k, t, rec, type, seed, step, del_t, true, &p1, &tns);
+        effect = (r == 1);
+      } else {
+        TestOut o;
+        perset_test(B, A, W, box, k, t, type, seed, step, precision, band, 0, o);
+        p1 = o.p1;
+        p2 = o.p2;
+        warn = o.warn;
+        effect = o.accept;
+        if (effect) {
+          atomicMin(&W.claim[p1], t);
+          atomicMin(&W.claim[p2], t);
+        }
       }
-    } else {
-      TestOut o;
-      perset_test(B, A, W, box, k, t, type, seed, step, precision, band, 0, o);
-      if (o.warn) B->n_warn += 1;
-      if (o.accept)
-        perset_merge(B, A, W, box, k, t, o.p1, o.p2, seed, step, type, want_terms, rid, al);
     }
+    tail_barrier();
+    // 2. resolve, and what each winner creates
+    bool win = false;
+    int o = 0;
+    int c_app = 0, c_id = 0, c_info = 0, c_hole = 0;
+    if (effect) {
+      if (is_acc) {
+        win = W.claim[p1] == t;
+        rm_walk(W, rec, [&](int q) { win = win && W.claim[q] == t; });
+        if (win) {
+          const int died = (W.r_flags[rec] & RF_DIED) ? 1 : 0;
+          c_app = W.r_ndup[rec] - W.r_ndead[rec];
+          c_id = W.r_ndup[rec];
+          c_info = W.r_nrm[rec] + died + W.r_ndead[rec];
+          c_hole = W.r_nrm[rec] + died;
+        }
+      } else {
+        win = W.claim[p1] == t && W.claim[p2] == t;
+        if (win) {
+          int c1, c2, b1, b2;
+          decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
+          const int nb = B->n_bin, nc = B->w.n_class;
+          const int cc = A.cc[table_index(A, box, nb, nc, b1, b2, c1, c2)];
+          Stream s;
+          s.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), t - W.pair_exc[k],
+                 RNG_MERGE);
+          int g;
+          o = merge_decide(A, B->w, s, p1, p2, c1, c2, cc, &g);
+          c_app = (o & O_APPEND) ? 1 : 0;
+          c_id = (o & O_NEWID) ? 1 : 0;
+          c_info = ((o & O_INFO1) ? 1 : 0) + ((o & O_INFO2) ? 1 : 0);
+          c_hole = ((o & O_RM1) && !(o & O_SLOT1) ? 1 : 0) + ((o & O_RM2) && !(o & O_SLOT2) ? 1 : 0);
+        }
+      }
+    }
+    int r_app, r_id, r_info, r_hole, tot;
+    BS(scan_tmp).ExclusiveSum(c_app, r_app, tot);
+    if (tid == 0) s_tot[0] = tot;
+    tail_barrier();
+    BS(scan_tmp).ExclusiveSum(c_id, r_id, tot);
+    if (tid == 0) s_tot[1] = tot;
+    tail_barrier();
+    BS(scan_tmp).ExclusiveSum(c_info, r_info, tot);
+    if (tid == 0) s_tot[2] = tot;
+    tail_barrier();
+    BS(scan_tmp).ExclusiveSum(c_hole, r_hole, tot);
+    if (tid == 0) {
+      s_tot[3] = tot;
+      s_hole_base = (long long)atomicAdd(W.hole_top, (unsigned long long)tot);
+    }
+    tail_barrier();
+    // 3. apply the winners at their ranks
+    if (win) {
+      RankAlloc al{B, r_app, r_id, r_info, (int64_t)s_hole_base + r_hole};
+      if (is_acc) acc_apply(B, A, W, box, t, rec, p1, type, want_terms, rid, al);
+      else perset_merge(B, A, W, box, k, t, p1, p2, seed, step, type, want_terms, rid, al);
+    }
+    tail_barrier();
+    if (tid == 0) {
+      B->n_part += s_tot[0];
+      B->next_id += s_tot[1];
+      B->n_info += s_tot[2];
+    }
+    // 4. release the round's claims
+    if (effect) {
+      W.claim[p1] = UNCLAIMED;
+      if (is_acc) rm_walk(W, rec, [&](int q) { W.claim[q] = UNCLAIMED; });
+      else W.claim[p2] = UNCLAIMED;
+    }
+    // 5. re-runs: losers, and tests whose particles an earlier winner of
+    // this round changed; the rest are final
+    int f = 0;
+    if (active) {
+      if (effect) f = win ? 0 : 1;
+      else f = (consumed_before(W, p1, t, rid) || (!is_acc && consumed_before(W, p2, t, rid))) ? 1 : 0;
+      if (!f) {
+        if (is_acc && !effect && tns) {
+          unsigned long long s = (unsigned long long)(tns & (ACC_WARN - 1));
+          atomicAdd((unsigned long long *)&B->n_samp, s);
+          atomicAdd((unsigned long long *)&B->step_samp, s);
+          if (tns & ACC_WARN) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
+        }
+        if (!is_acc && warn) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
+      }
+    }
+    int r_keep;
+    BS(scan_tmp).ExclusiveSum(f, r_keep, tot);
+    if (f) s_keep[r_keep] = t;
+    tail_barrier();
+    // the window's re-runs go back right before the untried tests
+    const int new_head = head + wn - tot;
+    if (tid < tot) pend[new_head + tid] = s_keep[tid];
+    tail_barrier();
+    head = new_head;
+    rid++;
+    n_rounds++;
   }
+  if (tid == 0 && n_rounds) atomicAdd(rounds_out, n_rounds);
 }
 
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) {
@@ -1966,7 +2098,7 @@ inline void parallel_free_arrays(ParallelWork &W) {
                   d.s_lose, d.box_first,
                   d.holes, d.holes_sorted, d.tail_live, W.scan_tmp, W.sort_tmp,
                   d.pair_acc, d.pair_accn, d.acc_exc, d.trec, d.tns, d.ev_rec, d.misc,
-                  d.hole_top};
+                  d.hole_top, d.tail_rounds};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   parallel_free_records(W);
@@ -2050,6 +2182,7 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(hipMemset(d.pair_accn, 0, sizeof(uint32_t) * (W.NP + 1)));
   PCHECK(pmalloc(&d.misc, 4));
   PCHECK(pmalloc(&d.hole_top, 1));
+  PCHECK(pmalloc(&d.tail_rounds, 1));
   const int64_t T = W.T_cap;
   PCHECK(pmalloc(&d.acc, T + 1));
   PCHECK(pmalloc(&d.acc_scan, T + 1));
@@ -2188,20 +2321,24 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
     const uint32_t *pend = nullptr;
     for (int round = 0; n > 0; ++round) {
       if (round >= PAR_ROUNDS) {
-        // the rest in sequential order, per box
+        // the rest per box, one wavefront each
         if (n_acc > 0) {
           hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.rm_top, 1, 0);
           hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.dup_top, 1, 0);
         }
         hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, d.hole_top,
                            (unsigned long long)n_holes);
-        PLAUNCH(W, "k_par_tail", k_par_tail, dim3((n_box + 63) / 64), dim3(64), 0, st, d_box, A,
-                d, n_box, n, pend, type, seed, step, precision, band, del_t, a0, want_terms,
-                (int)W.n_rounds);
+        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, d.tail_rounds, 0ull);
+        PLAUNCH(W, "k_par_tail", k_par_tail, dim3(n_box), dim3(TAIL_THREADS), 0, st, d_box, A, d,
+                n_box, n, (uint32_t *)pend, type, seed, step, precision, band, del_t, a0,
+                want_terms, (int)W.n_rounds + 1, d.tail_rounds);
         PCHECK(hipMemcpyAsync(W.h_u64, d.hole_top, sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, st));
+        PCHECK(hipMemcpyAsync(W.h_u64 + 1, d.tail_rounds, sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, st));
         PCHECK(hipStreamSynchronize(st));
         n_holes = (int64_t)W.h_u64[0];
+        W.n_tail_rounds += (int64_t)W.h_u64[1];
         W.n_tail += n;
         break;
       }
