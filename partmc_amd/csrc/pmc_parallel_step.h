@@ -127,6 +127,7 @@ struct ParallelWork {
   void *tctx = nullptr;
   int64_t n_tail = 0;     // tests finished in the tail
   int64_t n_tail_rounds = 0;  // the tail's claim rounds, summed over boxes
+  int64_t rid = 1;        // round identifiers: a fresh one for every claim round
   int64_t n_rounds = 0;   // claim rounds run
 };
 
@@ -1762,6 +1763,7 @@ __device__ inline int lower_bound_u32(const uint32_t *v, int n, uint32_t x) {
 }
 
 constexpr int TAIL_THREADS = 64;
+constexpr int TF_EFFECT = 1, TF_ACC = 2, TF_RUN = 4, TF_WARN = 8;
 
 // a phase boundary of the tail: earlier writes and atomics of the box's
 // wavefront become visible to its next phase (device-scope fence, then the
@@ -1772,67 +1774,96 @@ __device__ inline void tail_barrier() {
 }
 
 // The tail of a batch: the tests still pending after the device-wide claim
-// rounds, which are mostly chains of tests on one particle (a large
-// particle drawn by many tests of its bin pairs).  One wavefront per box
-// runs claim rounds over a window of the box's next TAIL_THREADS pending
-// tests, in sequential order, with wave barriers in place of launches: the
-// earliest test of the window always wins, so every round makes progress.
-// The box's pending run of `pend` serves as its queue: re-runs are put back
-// in front of the tests not yet tried, keeping sequential order.
+// rounds, mostly chains of tests on one particle (a large particle drawn by
+// many tests of its bin pairs).  One wavefront per box runs claim rounds
+// over a window of the box's next TAIL_THREADS pending tests, in sequential
+// order, with barriers in place of launches.  A test is run again only when
+// a particle it read or claimed was changed by an earlier winner; a test
+// that lost with its inputs unchanged keeps its outcome and claims again.
+// The window's earliest test always wins or is final, so every round makes
+// progress.
 __global__ void __launch_bounds__(TAIL_THREADS)
-k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, uint32_t *pend, int type,
+k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pend, int type,
            uint64_t seed, uint32_t step, int precision, double band, double del_t,
            uint32_t acc_base, int want_terms, int rid0, unsigned long long *rounds_out) {
+  (void)n_box;
   const int box = blockIdx.x;
   const int tid = threadIdx.x;
   BoxDesc *B = &boxes[box];
-  const int64_t NP = (int64_t)n_box * W.P_stride;
-  const int lo = lower_bound_u32(pend, n, W.pair_exc[(int64_t)box * W.P_stride]);
-  const int hi = lower_bound_u32(pend, n, W.pair_exc[(int64_t)(box + 1) * W.P_stride]);
+  const int64_t pb = (int64_t)box * W.P_stride;
+  const int lo = lower_bound_u32(pend, n, W.pair_exc[pb]);
+  const int hi = lower_bound_u32(pend, n, W.pair_exc[pb + W.P_stride]);
   if (lo >= hi) return;
   using BS = hipcub::BlockScan<int, TAIL_THREADS>;
   __shared__ typename BS::TempStorage scan_tmp;
-  __shared__ int s_tot[5];
+  __shared__ uint32_t w_t[TAIL_THREADS];
+  __shared__ int w_k[TAIL_THREADS], w_p1[TAIL_THREADS], w_p2[TAIL_THREADS];
+  __shared__ int w_tns[TAIL_THREADS], w_fl[TAIL_THREADS];
+  __shared__ int s_tot[4];
   __shared__ long long s_hole_base;
-  __shared__ uint32_t s_keep[TAIL_THREADS];
   __shared__ int s_err;
-  int head = lo;
+  int wn = 0, head = lo;
   int rid = rid0;
   unsigned long long n_rounds = 0;
-  while (head < hi) {
+  for (;;) {
     if (tid == 0) s_err = B->error;
     tail_barrier();
-    if (s_err) break;
-    const int wn = min(TAIL_THREADS, hi - head);
+    if (s_err || (wn == 0 && head >= hi)) break;
+    // refill the window from the queue, in order
+    const int add = min(TAIL_THREADS - wn, hi - head);
+    if (tid < add) {
+      const uint32_t t = pend[head + tid];
+      const int kl = (int)find_pair(W.pair_exc + pb, W.P_stride, t);
+      w_t[wn + tid] = t;
+      w_k[wn + tid] = kl;
+      w_fl[wn + tid] = TF_RUN | (W.pair_acc[pb + kl] ? TF_ACC : 0);
+      w_p1[wn + tid] = -1;
+      w_p2[wn + tid] = -1;
+      w_tns[wn + tid] = 0;
+    }
+    tail_barrier();
+    wn += add;
+    head += add;
     const bool active = tid < wn;
-    uint32_t t = active ? pend[head + tid] : 0u;
-    int64_t k = active ? find_pair(W.pair_exc, NP, t) : 0;
-    const bool is_acc = active && W.pair_acc[k];
-    int p1 = -1, p2 = -1, rec = -1, tns = 0;
-    bool effect = false, warn = false;
-    // 1. run the tests of the window (claims as in a device-wide round)
+    uint32_t t = 0;
+    int64_t k = pb;
+    int fl = 0, p1 = -1, p2 = -1, tns = 0;
     if (active) {
+      t = w_t[tid];
+      k = pb + w_k[tid];
+      fl = w_fl[tid];
+      p1 = w_p1[tid];
+      p2 = w_p2[tid];
+      tns = w_tns[tid];
+    }
+    const bool is_acc = fl & TF_ACC;
+    const int rec = is_acc ? (int)(W.acc_exc[k] + (t - W.pair_exc[k]) - acc_base) : -1;
+    // 1. run the tests whose inputs are new or changed
+    if (active && (fl & TF_RUN)) {
+      fl &= ~(TF_RUN | TF_EFFECT | TF_WARN);
       if (is_acc) {
-        rec = (int)(W.acc_exc[k] + (t - W.pair_exc[k]) - acc_base);
-        const int r = acc_walk(boxes, A, W, k, t, rec, type, seed, step, del_t, true, &p1, &tns);
-        effect = (r == 1);
+        const int r = acc_walk(boxes, A, W, k, t, rec, type, seed, step, del_t, false, &p1, &tns);
+        if (r == 1) fl |= TF_EFFECT;
+        p2 = -1;
       } else {
         TestOut o;
         perset_test(B, A, W, box, k, t, type, seed, step, precision, band, 0, o);
         p1 = o.p1;
         p2 = o.p2;
-        warn = o.warn;
-        effect = o.accept;
-        if (effect) {
-          atomicMin(&W.claim[p1], t);
-          atomicMin(&W.claim[p2], t);
-        }
+        if (o.accept) fl |= TF_EFFECT;
+        if (o.warn) fl |= TF_WARN;
       }
     }
+    const bool effect = fl & TF_EFFECT;
+    // 2. claims
+    if (effect) {
+      atomicMin(&W.claim[p1], t);
+      if (is_acc) rm_walk(W, rec, [&](int q) { atomicMin(&W.claim[q], t); });
+      else atomicMin(&W.claim[p2], t);
+    }
     tail_barrier();
-    // 2. resolve, and what each winner creates
+    // 3. resolve, and what each winner creates
     bool win = false;
-    int o = 0;
     int c_app = 0, c_id = 0, c_info = 0, c_hole = 0;
     if (effect) {
       if (is_acc) {
@@ -1856,7 +1887,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, uint32_t *pend, int
           s.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), t - W.pair_exc[k],
                  RNG_MERGE);
           int g;
-          o = merge_decide(A, B->w, s, p1, p2, c1, c2, cc, &g);
+          const int o = merge_decide(A, B->w, s, p1, p2, c1, c2, cc, &g);
           c_app = (o & O_APPEND) ? 1 : 0;
           c_id = (o & O_NEWID) ? 1 : 0;
           c_info = ((o & O_INFO1) ? 1 : 0) + ((o & O_INFO2) ? 1 : 0);
@@ -1867,20 +1898,20 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, uint32_t *pend, int
     int r_app, r_id, r_info, r_hole, tot;
     BS(scan_tmp).ExclusiveSum(c_app, r_app, tot);
     if (tid == 0) s_tot[0] = tot;
-    tail_barrier();
+    __syncthreads();
     BS(scan_tmp).ExclusiveSum(c_id, r_id, tot);
     if (tid == 0) s_tot[1] = tot;
-    tail_barrier();
+    __syncthreads();
     BS(scan_tmp).ExclusiveSum(c_info, r_info, tot);
     if (tid == 0) s_tot[2] = tot;
-    tail_barrier();
+    __syncthreads();
     BS(scan_tmp).ExclusiveSum(c_hole, r_hole, tot);
     if (tid == 0) {
       s_tot[3] = tot;
-      s_hole_base = (long long)atomicAdd(W.hole_top, (unsigned long long)tot);
+      s_hole_base = tot ? (long long)atomicAdd(W.hole_top, (unsigned long long)tot) : 0ll;
     }
     tail_barrier();
-    // 3. apply the winners at their ranks
+    // 4. apply the winners at their ranks
     if (win) {
       RankAlloc al{B, r_app, r_id, r_info, (int64_t)s_hole_base + r_hole};
       if (is_acc) acc_apply(B, A, W, box, t, rec, p1, type, want_terms, rid, al);
@@ -1892,37 +1923,47 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, uint32_t *pend, int
       B->next_id += s_tot[1];
       B->n_info += s_tot[2];
     }
-    // 4. release the round's claims
+    // 5. release the claims
     if (effect) {
       W.claim[p1] = UNCLAIMED;
       if (is_acc) rm_walk(W, rec, [&](int q) { W.claim[q] = UNCLAIMED; });
       else W.claim[p2] = UNCLAIMED;
     }
-    // 5. re-runs: losers, and tests whose particles an earlier winner of
-    // this round changed; the rest are final
-    int f = 0;
-    if (active) {
-      if (effect) f = win ? 0 : 1;
-      else f = (consumed_before(W, p1, t, rid) || (!is_acc && consumed_before(W, p2, t, rid))) ? 1 : 0;
-      if (!f) {
-        if (is_acc && !effect && tns) {
+    tail_barrier();
+    // 6. what stays in the window: losers (run again only if an earlier
+    // winner changed what they read) and tests whose inputs changed
+    int keep = 0;
+    if (active && !win) {
+      bool changed = consumed_before(W, p1, t, rid) || consumed_before(W, p2, t, rid);
+      if (is_acc && effect && !changed)
+        rm_walk(W, rec, [&](int q) { changed = changed || consumed_before(W, q, t, rid); });
+      if (effect || changed) {
+        keep = 1;
+        if (changed) fl |= TF_RUN;
+      } else {
+        // final without effect: its counts
+        if (is_acc && tns) {
           unsigned long long s = (unsigned long long)(tns & (ACC_WARN - 1));
           atomicAdd((unsigned long long *)&B->n_samp, s);
           atomicAdd((unsigned long long *)&B->step_samp, s);
           if (tns & ACC_WARN) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
         }
-        if (!is_acc && warn) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
+        if (!is_acc && (fl & TF_WARN)) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
       }
     }
+    if (win && !is_acc && (fl & TF_WARN)) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
     int r_keep;
-    BS(scan_tmp).ExclusiveSum(f, r_keep, tot);
-    if (f) s_keep[r_keep] = t;
-    tail_barrier();
-    // the window's re-runs go back right before the untried tests
-    const int new_head = head + wn - tot;
-    if (tid < tot) pend[new_head + tid] = s_keep[tid];
-    tail_barrier();
-    head = new_head;
+    BS(scan_tmp).ExclusiveSum(keep, r_keep, tot);
+    __syncthreads();
+    if (keep) {
+      w_t[r_keep] = t;
+      w_k[r_keep] = (int)(k - pb);
+      w_fl[r_keep] = fl;
+      w_p1[r_keep] = p1;
+      w_p2[r_keep] = p2;
+      w_tns[r_keep] = tns;
+    }
+    wn = tot;
     rid++;
     n_rounds++;
   }
@@ -2330,8 +2371,8 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
                            (unsigned long long)n_holes);
         hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, d.tail_rounds, 0ull);
         PLAUNCH(W, "k_par_tail", k_par_tail, dim3(n_box), dim3(TAIL_THREADS), 0, st, d_box, A, d,
-                n_box, n, (uint32_t *)pend, type, seed, step, precision, band, del_t, a0,
-                want_terms, (int)W.n_rounds + 1, d.tail_rounds);
+                n_box, n, pend, type, seed, step, precision, band, del_t, a0,
+                want_terms, (int)W.rid + 1, d.tail_rounds);
         PCHECK(hipMemcpyAsync(W.h_u64, d.hole_top, sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, st));
         PCHECK(hipMemcpyAsync(W.h_u64 + 1, d.tail_rounds, sizeof(unsigned long long),
@@ -2339,10 +2380,12 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         PCHECK(hipStreamSynchronize(st));
         n_holes = (int64_t)W.h_u64[0];
         W.n_tail_rounds += (int64_t)W.h_u64[1];
+        W.rid += (int64_t)W.h_u64[1] + 2;  // past every round id the tail used
         W.n_tail += n;
         break;
       }
       W.n_rounds++;
+      W.rid++;
       if (n_acc > 0) {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.rm_top, 1, 0);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.dup_top, 1, 0);
@@ -2373,11 +2416,11 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, fs[q], ss[q], E + 1, st));
       }
       PLAUNCH(W, "k_par_apply", k_par_apply, dim3((E + 255) / 256), dim3(256), 0, st, d_box, A, d,
-              n_box, E, seed, step, type, want_terms, n_holes, (int)W.n_rounds);
+              n_box, E, seed, step, type, want_terms, n_holes, (int)W.rid);
       hipLaunchKernelGGL(k_par_box_totals, gb, bb, 0, st, d_box, d, n_box, E);
       hipLaunchKernelGGL(k_par_release, dim3((E + 255) / 256), dim3(256), 0, st, d, E);
       hipLaunchKernelGGL(k_par_pending, dim3((n + 255) / 256), dim3(256), 0, st, d_box, d, n_box, n,
-                         (int)W.n_rounds);
+                         (int)W.rid);
       hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.f_pend + n, 1, 0);
       PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.f_pend, d.s_pend, n + 1, st));
       hipLaunchKernelGGL(k_par_gather_pending, dim3((n + 255) / 256), dim3(256), 0, st, d, n);
