@@ -96,6 +96,7 @@ struct PW {  // device view of the parallel work arrays
   int rm_cap;
   uint32_t *misc;              // [4] batch bounds from k_batch_end
   unsigned long long *hole_top;  // [1] hole counter of the tail
+  int *pend_k;                 // [T_cap] pair of each pending test entering the tail
   unsigned long long *tail_rounds;  // [1] claim rounds run by the tail, summed over boxes
 };
 
@@ -1765,6 +1766,15 @@ __device__ inline int lower_bound_u32(const uint32_t *v, int n, uint32_t x) {
 constexpr int TAIL_THREADS = 64;
 constexpr int TF_EFFECT = 1, TF_ACC = 2, TF_RUN = 4, TF_WARN = 8;
 
+// claims inside one kernel are read and released with device-scope atomics,
+// so they need no cache maintenance between the phases of a round
+__device__ inline uint32_t claim_load(const PW &W, int p) {
+  return __hip_atomic_load(&W.claim[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void claim_release(const PW &W, int p) {
+  __hip_atomic_store(&W.claim[p], UNCLAIMED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // a phase boundary of the tail: earlier writes and atomics of the box's
 // wavefront become visible to its next phase (device-scope fence, then the
 // barrier)
@@ -1813,7 +1823,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     const int add = min(TAIL_THREADS - wn, hi - head);
     if (tid < add) {
       const uint32_t t = pend[head + tid];
-      const int kl = (int)find_pair(W.pair_exc + pb, W.P_stride, t);
+      const int kl = (int)(W.pend_k[head + tid] - pb);
       w_t[wn + tid] = t;
       w_k[wn + tid] = kl;
       w_fl[wn + tid] = TF_RUN | (W.pair_acc[pb + kl] ? TF_ACC : 0);
@@ -1821,7 +1831,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
       w_p2[wn + tid] = -1;
       w_tns[wn + tid] = 0;
     }
-    tail_barrier();
+    __syncthreads();
     wn += add;
     head += add;
     const bool active = tid < wn;
@@ -1861,14 +1871,14 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
       if (is_acc) rm_walk(W, rec, [&](int q) { atomicMin(&W.claim[q], t); });
       else atomicMin(&W.claim[p2], t);
     }
-    tail_barrier();
+    __syncthreads();
     // 3. resolve, and what each winner creates
     bool win = false;
     int c_app = 0, c_id = 0, c_info = 0, c_hole = 0;
     if (effect) {
       if (is_acc) {
-        win = W.claim[p1] == t;
-        rm_walk(W, rec, [&](int q) { win = win && W.claim[q] == t; });
+        win = claim_load(W, p1) == t;
+        rm_walk(W, rec, [&](int q) { win = win && claim_load(W, q) == t; });
         if (win) {
           const int died = (W.r_flags[rec] & RF_DIED) ? 1 : 0;
           c_app = W.r_ndup[rec] - W.r_ndead[rec];
@@ -1877,7 +1887,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
           c_hole = W.r_nrm[rec] + died;
         }
       } else {
-        win = W.claim[p1] == t && W.claim[p2] == t;
+        win = claim_load(W, p1) == t && claim_load(W, p2) == t;
         if (win) {
           int c1, c2, b1, b2;
           decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
@@ -1910,7 +1920,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
       s_tot[3] = tot;
       s_hole_base = tot ? (long long)atomicAdd(W.hole_top, (unsigned long long)tot) : 0ll;
     }
-    tail_barrier();
+    __syncthreads();
     // 4. apply the winners at their ranks
     if (win) {
       RankAlloc al{B, r_app, r_id, r_info, (int64_t)s_hole_base + r_hole};
@@ -1925,11 +1935,10 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     }
     // 5. release the claims
     if (effect) {
-      W.claim[p1] = UNCLAIMED;
-      if (is_acc) rm_walk(W, rec, [&](int q) { W.claim[q] = UNCLAIMED; });
-      else W.claim[p2] = UNCLAIMED;
+      claim_release(W, p1);
+      if (is_acc) rm_walk(W, rec, [&](int q) { claim_release(W, q); });
+      else claim_release(W, p2);
     }
-    tail_barrier();
     // 6. what stays in the window: losers (run again only if an earlier
     // winner changed what they read) and tests whose inputs changed
     int keep = 0;
@@ -1968,6 +1977,12 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     n_rounds++;
   }
   if (tid == 0 && n_rounds) atomicAdd(rounds_out, n_rounds);
+}
+
+// the pair of every pending test, found in parallel before the tail
+__global__ void k_pend_pairs(PW W, int64_t NP, const uint32_t *pend, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) W.pend_k[i] = (int)find_pair(W.pair_exc, NP, pend[i]);
 }
 
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) {
@@ -2139,7 +2154,7 @@ inline void parallel_free_arrays(ParallelWork &W) {
                   d.s_lose, d.box_first,
                   d.holes, d.holes_sorted, d.tail_live, W.scan_tmp, W.sort_tmp,
                   d.pair_acc, d.pair_accn, d.acc_exc, d.trec, d.tns, d.ev_rec, d.misc,
-                  d.hole_top, d.tail_rounds};
+                  d.hole_top, d.tail_rounds, d.pend_k};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   parallel_free_records(W);
@@ -2237,6 +2252,7 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.ev_box, T));
   PCHECK(pmalloc(&d.ev_out, T));
   PCHECK(pmalloc(&d.trec, T));
+  PCHECK(pmalloc(&d.pend_k, T));
   PCHECK(pmalloc(&d.tns, T));
   PCHECK(pmalloc(&d.ev_rec, T));
   int **fl[] = {&d.f_app, &d.f_id, &d.f_info, &d.f_hole, &d.f_lose, &d.f_pend,
@@ -2370,6 +2386,7 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, d.hole_top,
                            (unsigned long long)n_holes);
         hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, d.tail_rounds, 0ull);
+        hipLaunchKernelGGL(k_pend_pairs, dim3((n + 255) / 256), dim3(256), 0, st, d, W.NP, pend, n);
         PLAUNCH(W, "k_par_tail", k_par_tail, dim3(n_box), dim3(TAIL_THREADS), 0, st, d_box, A, d,
                 n_box, n, pend, type, seed, step, precision, band, del_t, a0,
                 want_terms, (int)W.rid + 1, d.tail_rounds);
