@@ -82,6 +82,7 @@ struct PW {  // device view of the parallel work arrays
   // event records [acc_cap]
   int *r_nsamp, *r_ncoag, *r_nrm, *r_nchain, *r_head, *r_ndup, *r_ndead, *r_dhead, *r_flags,
       *r_err;
+  int *r_rdhead, *r_nrd;       // distinct sources read from small source lists
   // particle pool [pool_cap]: entry r < acc_cap is the final target of
   // record r; entries from acc_cap on are the copies (aero_state_dup_particle)
   double *q_vol;               // [n_spec][pool_cap]
@@ -105,6 +106,7 @@ constexpr int DUP_CAP = 1 << 18;
 constexpr int RM_CHUNK = 16;
 constexpr int RM_CAP = 1 << 24;
 constexpr int ACC_WARN = 1 << 30;
+constexpr int READ_TRACK = 128;  // source lists up to this size have their reads tracked
 constexpr int RF_WARN = 1, RF_DIED = 2, RF_MERGED = 4;
 
 struct ParallelWork {
@@ -900,6 +902,17 @@ __device__ inline void rm_walk(const PW &W, int r, F f) {
   });
 }
 
+// visit the sources record r read from small lists
+template <class F>
+__device__ inline void rd_walk(const PW &W, int r, F f) {
+  int left = W.r_nrd[r];
+  for (int ch = W.r_rdhead[r]; ch >= 0 && left > 0; ch = W.rm_buf[ch]) {
+    int m = left < RM_CHUNK - 1 ? left : RM_CHUNK - 1;
+    for (int q = 1; q <= m; ++q) f(W.rm_buf[ch + q]);
+    left -= m;
+  }
+}
+
 __device__ inline void pool_store(const PW &W, int n_spec, int i, const Rec &s) {
   for (int k = 0; k < n_spec; ++k) W.q_vol[(int64_t)k * W.pool_cap + i] = s.vol[k];
   W.q_group[i] = s.group;
@@ -939,7 +952,7 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
                                         int64_t sbase, int skey, int n_s, double n_spt,
                                         double accept_factor, Stream &aps, Stream &smp,
                                         Stream &fin, uint32_t t, bool claim, RmChain &chain,
-                                        int *n_samp_out,
+                                        RmChain &reads, int *n_samp_out,
                                         int *n_coag_out, int *warn, Rec &source) {
   const int ns = A.n_spec;
   const Weights &w = B->w;
@@ -994,6 +1007,9 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
       atomicAdd(&B->n_dropped, 1);
       break;
     }
+    // in a small list a source another target removes changes this
+    // target's draws: remember what was read
+    if (n_s <= READ_TRACK && !rm_has(W, reads, ps) && !rm_push(W, reads, ps)) return ERR_CAPACITY;
     n_samp++;
     load_rec(A, ps, pi);
     double kk;
@@ -1101,12 +1117,20 @@ __device__ inline int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &W, int
     q = next_acc_pair(W, pacc, nb, nc, ct, b_cur, q - 1);
   }
   *p_out = p;
+  // the record's chains start empty, so an attempt cut short by an error
+  // leaves nothing to walk
+  W.r_nrd[rec] = 0;
+  W.r_rdhead[rec] = -1;
+  W.r_nchain[rec] = 0;
+  W.r_nrm[rec] = 0;
+  W.r_head[rec] = -1;
   const int ns = A.n_spec;
   const Weights &w = B->w;
   const uint32_t ordinal = (uint32_t)(p - B->off) + 1;
   Rec target, cur, source;
   load_rec(A, p, target);
   RmChain chain{-1, -1, RM_CHUNK, 0, 0ull};
+  RmChain reads{-1, -1, RM_CHUNK, 0, 0ull};
   int n_samp_tot = 0, n_coag_tot = 0, warn = 0, flags = 0, err_code = 0, n_rm = 0;
   int n_dup = 0, n_dead = 0, dup_head = -1, dup_tail = -1;
   // The target walks first; each copy made on the way (in the reference it
@@ -1146,7 +1170,7 @@ __device__ inline int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &W, int
       int n_samp, n_coag;
       int err = acc_sample_source(A, W, B, type, cur, ct, cs, nc_at_radius(w, cs, edges[bs - 1]),
                                   B->off + W.lstart[lb + ks], ks, n_s, k_max * del_t * (double)n_s,
-                                  1.0 / k_max, aps, smp, fin, t, claim, chain, &n_samp, &n_coag,
+                                  1.0 / k_max, aps, smp, fin, t, claim, chain, reads, &n_samp, &n_coag,
                                   &warn,
                                   source);
       if (err) {
@@ -1227,6 +1251,8 @@ __device__ inline int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &W, int
     b_cur = W.q_bin[walker];
   }
   if (warn) flags |= RF_WARN;
+  W.r_rdhead[rec] = reads.head;
+  W.r_nrd[rec] = reads.n;
   if (n_coag_tot == 0 && n_rm == 0 && err_code == 0) {
     *tns = n_samp_tot | (warn ? ACC_WARN : 0);
     return 0;
@@ -1393,8 +1419,10 @@ __global__ void k_par_resolve(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, 
   const int r = W.ev_rec[e];
   if (r >= 0) {
     // a target event wins when it holds the target and every removed source
+    // ... and no earlier event claimed a source it read
     bool win = W.claim[p1] == t;
     rm_walk(W, r, [&](int q) { win = win && W.claim[q] == t; });
+    rd_walk(W, r, [&](int q) { win = win && W.claim[q] >= t; });
     const int died = (W.r_flags[r] & RF_DIED) ? 1 : 0;
     W.ev_out[e] = win ? O_WIN : 0;
     W.f_app[e] = win ? W.r_ndup[r] - W.r_ndead[r] : 0;
@@ -1726,7 +1754,10 @@ __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
   } else if (W.trec[i] >= 0) {
     // a target slot without effect: final unless an earlier winner of the
     // round changed its target; its samples count once final
-    f = consumed_before(W, W.tp1[i], t, rid) ? 1 : 0;
+    bool changed = consumed_before(W, W.tp1[i], t, rid);
+    if (W.tp1[i] >= 0)
+      rd_walk(W, W.trec[i], [&](int q) { changed = changed || consumed_before(W, q, t, rid); });
+    f = changed ? 1 : 0;
     int v = W.tns[i];
     if (!f && v) {
       const int64_t k = find_pair(W.pair_exc, (int64_t)n_box * W.P_stride, t);
@@ -1763,7 +1794,7 @@ __device__ inline int lower_bound_u32(const uint32_t *v, int n, uint32_t x) {
   return lo;
 }
 
-constexpr int TAIL_THREADS = 64;
+constexpr int TAIL_THREADS = 256;
 constexpr int TF_EFFECT = 1, TF_ACC = 2, TF_RUN = 4, TF_WARN = 8;
 
 // claims inside one kernel are read and released with device-scope atomics,
@@ -1879,6 +1910,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
       if (is_acc) {
         win = claim_load(W, p1) == t;
         rm_walk(W, rec, [&](int q) { win = win && claim_load(W, q) == t; });
+        rd_walk(W, rec, [&](int q) { win = win && claim_load(W, q) >= t; });
         if (win) {
           const int died = (W.r_flags[rec] & RF_DIED) ? 1 : 0;
           c_app = W.r_ndup[rec] - W.r_ndead[rec];
@@ -1946,6 +1978,8 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
       bool changed = consumed_before(W, p1, t, rid) || consumed_before(W, p2, t, rid);
       if (is_acc && effect && !changed)
         rm_walk(W, rec, [&](int q) { changed = changed || consumed_before(W, q, t, rid); });
+      if (is_acc && p1 >= 0 && !changed)
+        rd_walk(W, rec, [&](int q) { changed = changed || consumed_before(W, q, t, rid); });
       if (effect || changed) {
         keep = 1;
         if (changed) fl |= TF_RUN;
@@ -2107,14 +2141,14 @@ inline int parallel_init(ParallelWork &W, int n_box, int max_bin) {
 // the accelerated path's records, allocated when a step first has target slots
 inline void parallel_free_records(ParallelWork &W) {
   PW &d = W.d;
-  void *ptrs[] = {d.r_nsamp, d.r_ncoag, d.r_nrm, d.r_nchain, d.r_head, d.r_ndup, d.r_ndead,
+  void *ptrs[] = {d.r_rdhead, d.r_nrd, d.r_nsamp, d.r_ncoag, d.r_nrm, d.r_nchain, d.r_head, d.r_ndup, d.r_ndead,
                   d.r_dhead, d.r_flags, d.r_err, d.q_vol, d.q_group, d.q_nprim, d.q_ncomp,
                   d.q_next, d.q_pos, d.q_bin, d.q_dead, d.q_csrc, d.q_ctime, d.q_least,
                   d.q_greatest, d.dup_top, d.rm_buf, d.rm_top};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   d.r_nsamp = d.r_ncoag = d.r_nrm = d.r_nchain = d.r_head = d.r_ndup = d.r_ndead = nullptr;
-  d.r_dhead = d.r_flags = d.r_err = nullptr;
+  d.r_dhead = d.r_flags = d.r_err = d.r_rdhead = d.r_nrd = nullptr;
   d.q_pos = d.q_bin = d.q_dead = nullptr;
   d.q_vol = d.q_ctime = d.q_least = d.q_greatest = nullptr;
   d.q_group = d.q_nprim = d.q_ncomp = d.q_next = d.q_csrc = nullptr;
@@ -2126,7 +2160,7 @@ inline int parallel_alloc_records(ParallelWork &W, int n_spec) {
   PW &d = W.d;
   const int64_t c = ACC_CAP, pc = (int64_t)ACC_CAP + DUP_CAP;
   int **rv[] = {&d.r_nsamp, &d.r_ncoag, &d.r_nrm, &d.r_nchain, &d.r_head, &d.r_ndup, &d.r_ndead,
-                &d.r_dhead, &d.r_flags, &d.r_err};
+                &d.r_dhead, &d.r_flags, &d.r_err, &d.r_rdhead, &d.r_nrd};
   for (int **q : rv) PCHECK(pmalloc(q, c));
   PCHECK(pmalloc(&d.q_vol, (size_t)n_spec * pc));
   PCHECK(pmalloc(&d.q_csrc, (size_t)MAX_COMP * pc));
