@@ -11,7 +11,7 @@ import pytest
 
 import oracle
 from helpers import gpu_boxes, oracle_box, radius, small
-from partmc_amd import MODE_PARALLEL, PREC_FP32_RECHECK, PREC_FP64
+from partmc_amd import MODE_EXACT, MODE_PARALLEL, PREC_FP32_RECHECK, PREC_FP64
 from partmc_amd import populations as P
 
 pytestmark = pytest.mark.gpu
@@ -173,44 +173,49 @@ def test_brownian_ensemble_matches_oracle():
 def conc_metrics(vol, pop):
     """Particle count and the number concentration the particles stand for
     (aero_weight num_conc = magnitude * r**exponent, src/aero_weight.F90:
-    127-146), in total and above 0.2 um radius."""
+    127-146): in total, above 0.2 um and below 50 nm radius."""
     r = radius(vol)
     nc = pop.weight_magnitude * r ** pop.weight_exponent
-    return np.array([len(r), nc.sum(), nc[r > 2e-7].sum()])
+    return np.array([len(r), nc.sum(), nc[r > 2e-7].sum(), nc[r < 5e-8].sum()])
 
 
 def test_accelerated_ensemble_matches_oracle():
     """Power weighting (trimodal, exponent -3): bin pairs with a target bin of
     lower weight take the per-particle path (try_per_particle_coag,
-    src/coagulation.F90:150-241).  Over independent runs, the mean
-    coagulation count, particle count and number concentrations of the
-    parallel GPU chain and the sequential oracle agree within their 95%
-    confidence intervals."""
+    src/coagulation.F90:150-241).  Over 64 independent runs, the number
+    concentration in total, of the large and of the small particles, the
+    particle count and the coagulation count of the parallel chain agree
+    with the sequential chain's within 3 standard errors.  The sequential
+    ensemble comes from the exact mode, which reproduces the oracle bit for
+    bit (tests/test_gpu_exact.py); one run is also checked against the oracle
+    itself for the path's use."""
     base = small(P.config_c2(), 6000)
     base.weight_magnitude *= 30.0  # ~30% of the number concentration coagulates
-    n_rep, n_steps, dt = 16, 10, 60.0
-    g = par([base] * n_rep, 4321)
-    _, coag = step_n(g, P.KERNEL_BROWN, dt, n_steps)
-    st = g.path_stats()
-    assert st["n_accel_pairs"].min() > 0
+    n_rep, n_steps, dt = 64, 10, 60.0
+
+    def ensemble(mode, seed):
+        g = gpu_boxes([base] * n_rep, seed)
+        g.set_mode(mode, PREC_FP64, 1e-5)
+        _, coag = step_n(g, P.KERNEL_BROWN, dt, n_steps)
+        m = np.array([conc_metrics(g.download(k)["vol"], base) for k in range(n_rep)])
+        st = g.path_stats()
+        g.close()
+        return np.column_stack([m, coag.astype(float)]), st
+
+    gm, st = ensemble(MODE_PARALLEL, 4321)
+    em, st_e = ensemble(MODE_EXACT, 1234)
+    assert st["n_accel_pairs"].min() > 0 and st_e["n_accel_pairs"].min() > 0
     assert st["n_rounds"] > 0
-    gm = np.array([conc_metrics(g.download(k)["vol"], base) for k in range(n_rep)])
-    gm = np.column_stack([gm, coag.astype(float)])
-    om = []
-    n_accel = 0
-    for k in range(n_rep):
-        ob = oracle_box(base, 9000 + k)
-        c = 0
-        for s in range(n_steps):
-            c += ob.mc_coag(P.KERNEL_BROWN, dt, s)[1]
-        n_accel += ob.n_accel
-        om.append(np.append(conc_metrics(ob.particles()["vol"], base), c))
-    om = np.array(om)
-    assert n_accel > 0
-    diff = gm.mean(axis=0) - om.mean(axis=0)
-    se = np.sqrt(gm.var(axis=0, ddof=1) / n_rep + om.var(axis=0, ddof=1) / n_rep)
+    ob = oracle_box(base, 77)
+    for s in range(2):
+        ob.mc_coag(P.KERNEL_BROWN, dt, s)
+    assert ob.n_accel > 0
+    diff = gm.mean(axis=0) - em.mean(axis=0)
+    se = np.sqrt(gm.var(axis=0, ddof=1) / n_rep + em.var(axis=0, ddof=1) / n_rep)
+    z = diff / np.maximum(se, 1e-300)
     # the count is conserved to a few particles per run: allow one particle
-    assert np.all(np.abs(diff) <= 2.0 * se + np.array([1.0, 0, 0, 0])), (gm.mean(0), om.mean(0), se)
+    ok = np.abs(diff) <= 3.0 * se + np.array([1.0, 0, 0, 0, 0])
+    assert ok.all(), (gm.mean(0), em.mean(0), z)
 
 
 def test_accelerated_invariants():
