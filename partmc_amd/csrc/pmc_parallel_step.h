@@ -1362,8 +1362,10 @@ __device__ inline MergeCtx merge_ctx(BoxDesc *boxes, const PW &W, int n_box, uin
 }
 
 // coagulate_weighting (src/coagulation.F90:810-922): decisions only
-__device__ inline int merge_decide(const Arrays &A, const Weights &w, Stream &s, int64_t p1,
-                                   int64_t p2, int c1, int c2, int cc, int *new_group) {
+__device__ inline int merge_decide(const Arrays &A, const PW &W, const BoxDesc *B, Stream &s,
+                                   int64_t p1, int64_t p2, int c1, int c2, int cc,
+                                   int *new_group) {
+  const Weights &w = B->w;
   double r1 = vol2rad(particle_volume(A, p1));
   double r2 = vol2rad(particle_volume(A, p2));
   double rc = vol2rad(rad2vol(r1) + rad2vol(r2));
@@ -1395,8 +1397,15 @@ __device__ inline int merge_decide(const Arrays &A, const Weights &w, Stream &s,
     if (rm1 && !id1_lost) o |= O_ID1;
     else if (rm2 && !id2_lost) o |= O_ID2;
     else o |= O_NEWID;
-    // the new particle takes a freed slot when there is one
-    if (rm1) o |= O_SLOT1;
+    // the new particle takes a freed slot when there is one: the slot of
+    // the parent whose list it stays in, if any, so that it can still be
+    // drawn from that list in this step (in the reference it joins its
+    // bin's list)
+    const int nb = B->n_bin;
+    const int b_new = bin_find_log(nb, B->log_lo, B->log_hi, rc);
+    const int k_new = (b_new >= 1 && b_new <= nb) ? par_key(b_new, cc, nb) : -1;
+    if (rm1 && rm2) o |= (W.key[p2] == k_new && W.key[p1] != k_new) ? O_SLOT2 : O_SLOT1;
+    else if (rm1) o |= O_SLOT1;
     else if (rm2) o |= O_SLOT2;
     else o |= O_APPEND;
   } else {
@@ -1441,7 +1450,7 @@ __global__ void k_par_resolve(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, 
     Stream s;
     s.open(seed, m.box, step, pair_site(nb, nc, m.b1, m.b2, m.c1, m.c2), m.j, RNG_MERGE);
     int g;
-    o = merge_decide(A, B->w, s, p1, p2, m.c1, m.c2, cc, &g) | O_WIN;
+    o = merge_decide(A, W, B, s, p1, p2, m.c1, m.c2, cc, &g) | O_WIN;
   }
   W.ev_out[e] = o;
   W.f_app[e] = (o & O_APPEND) ? 1 : 0;
@@ -1554,7 +1563,7 @@ __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, in
   Stream s;
   s.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), j, RNG_MERGE);
   int new_group;
-  const int o = merge_decide(A, B->w, s, p1, p2, c1, c2, cc, &new_group);
+  const int o = merge_decide(A, W, B, s, p1, p2, c1, c2, cc, &new_group);
   Rec pt1, pt2, ptc;
   load_rec(A, p1, pt1);
   load_rec(A, p2, pt2);
@@ -1929,7 +1938,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
           s.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), t - W.pair_exc[k],
                  RNG_MERGE);
           int g;
-          const int o = merge_decide(A, B->w, s, p1, p2, c1, c2, cc, &g);
+          const int o = merge_decide(A, W, B, s, p1, p2, c1, c2, cc, &g);
           c_app = (o & O_APPEND) ? 1 : 0;
           c_id = (o & O_NEWID) ? 1 : 0;
           c_info = ((o & O_INFO1) ? 1 : 0) + ((o & O_INFO2) ? 1 : 0);
