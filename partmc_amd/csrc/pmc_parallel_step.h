@@ -1803,8 +1803,7 @@ __device__ inline int lower_bound_u32(const uint32_t *v, int n, uint32_t x) {
   return lo;
 }
 
-constexpr int TAIL_THREADS = 64;
-constexpr int TAIL_CHG = 256;  // slots the serial pass remembers as changed
+constexpr int TAIL_THREADS = 256;
 constexpr int TF_EFFECT = 1, TF_ACC = 2, TF_RUN = 4, TF_WARN = 8;
 
 // claims inside one kernel are read and released with device-scope atomics,
@@ -1816,9 +1815,9 @@ __device__ inline void claim_release(const PW &W, int p) {
   __hip_atomic_store(&W.claim[p], UNCLAIMED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-
-// a phase boundary of the tail: earlier writes of the box's workgroup
-// become visible to its next phase (device-scope fence, then the barrier)
+// a phase boundary of the tail: earlier writes and atomics of the box's
+// wavefront become visible to its next phase (device-scope fence, then the
+// barrier)
 __device__ inline void tail_barrier() {
   __threadfence();
   __syncthreads();
@@ -1826,17 +1825,17 @@ __device__ inline void tail_barrier() {
 
 // The tail of a batch: the tests still pending after the device-wide claim
 // rounds, mostly chains of tests on one particle (a large particle drawn by
-// many tests of its bin pairs).  One workgroup per box takes them in
-// sequential order, TAIL_THREADS at a time: every lane first runs its test
-// on the current state (draws, kernel, accept), which also brings the
-// particles into cache; then lane 0 commits the window in order -- a test
-// whose particles an earlier commit of the window changed is run again
-// first -- and target slots are walked and applied there.  This is the
-// reference's own order, so no claims are needed.
+// many tests of its bin pairs).  One wavefront per box runs claim rounds
+// over a window of the box's next TAIL_THREADS pending tests, in sequential
+// order, with barriers in place of launches.  A test is run again only when
+// a particle it read or claimed was changed by an earlier winner; a test
+// that lost with its inputs unchanged keeps its outcome and claims again.
+// The window's earliest test always wins or is final, so every round makes
+// progress.
 __global__ void __launch_bounds__(TAIL_THREADS)
 k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pend, int type,
            uint64_t seed, uint32_t step, int precision, double band, double del_t,
-           uint32_t acc_base, int want_terms, int rid, unsigned long long *rounds_out) {
+           uint32_t acc_base, int want_terms, int rid0, unsigned long long *rounds_out) {
   (void)n_box;
   const int box = blockIdx.x;
   const int tid = threadIdx.x;
@@ -1845,90 +1844,182 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
   const int lo = lower_bound_u32(pend, n, W.pair_exc[pb]);
   const int hi = lower_bound_u32(pend, n, W.pair_exc[pb + W.P_stride]);
   if (lo >= hi) return;
-  __shared__ int w_p1[TAIL_THREADS], w_p2[TAIL_THREADS];
-  __shared__ int w_fl[TAIL_THREADS];
+  using BS = hipcub::BlockScan<int, TAIL_THREADS>;
+  __shared__ typename BS::TempStorage scan_tmp;
+  __shared__ uint32_t w_t[TAIL_THREADS];
+  __shared__ int w_k[TAIL_THREADS], w_p1[TAIL_THREADS], w_p2[TAIL_THREADS];
+  __shared__ int w_tns[TAIL_THREADS], w_fl[TAIL_THREADS];
+  __shared__ int s_tot[4];
+  __shared__ long long s_hole_base;
   __shared__ int s_err;
-  __shared__ int chg[TAIL_CHG];
-  unsigned long long n_windows = 0;
-  for (int head = lo; head < hi; head += TAIL_THREADS) {
+  int wn = 0, head = lo;
+  int rid = rid0;
+  unsigned long long n_rounds = 0;
+  for (;;) {
     if (tid == 0) s_err = B->error;
     tail_barrier();
-    if (s_err) break;
-    const int wn = min(TAIL_THREADS, hi - head);
-    // 1. every lane runs its per-set test on the state before the window
-    if (tid < wn) {
+    if (s_err || (wn == 0 && head >= hi)) break;
+    // refill the window from the queue, in order
+    const int add = min(TAIL_THREADS - wn, hi - head);
+    if (tid < add) {
       const uint32_t t = pend[head + tid];
-      const int64_t k = W.pend_k[head + tid];
-      int fl = 0;
-      if (W.pair_acc[k]) {
-        fl = TF_ACC;
+      const int kl = (int)(W.pend_k[head + tid] - pb);
+      w_t[wn + tid] = t;
+      w_k[wn + tid] = kl;
+      w_fl[wn + tid] = TF_RUN | (W.pair_acc[pb + kl] ? TF_ACC : 0);
+      w_p1[wn + tid] = -1;
+      w_p2[wn + tid] = -1;
+      w_tns[wn + tid] = 0;
+    }
+    __syncthreads();
+    wn += add;
+    head += add;
+    const bool active = tid < wn;
+    uint32_t t = 0;
+    int64_t k = pb;
+    int fl = 0, p1 = -1, p2 = -1, tns = 0;
+    if (active) {
+      t = w_t[tid];
+      k = pb + w_k[tid];
+      fl = w_fl[tid];
+      p1 = w_p1[tid];
+      p2 = w_p2[tid];
+      tns = w_tns[tid];
+    }
+    const bool is_acc = fl & TF_ACC;
+    const int rec = is_acc ? (int)(W.acc_exc[k] + (t - W.pair_exc[k]) - acc_base) : -1;
+    // 1. run the tests whose inputs are new or changed
+    if (active && (fl & TF_RUN)) {
+      fl &= ~(TF_RUN | TF_EFFECT | TF_WARN);
+      if (is_acc) {
+        const int r = acc_walk(boxes, A, W, k, t, rec, type, seed, step, del_t, false, &p1, &tns);
+        if (r == 1) fl |= TF_EFFECT;
+        p2 = -1;
       } else {
         TestOut o;
         perset_test(B, A, W, box, k, t, type, seed, step, precision, band, 0, o);
-        w_p1[tid] = o.p1;
-        w_p2[tid] = o.p2;
-        fl = (o.accept ? TF_EFFECT : 0) | (o.warn ? TF_WARN : 0);
+        p1 = o.p1;
+        p2 = o.p2;
+        if (o.accept) fl |= TF_EFFECT;
+        if (o.warn) fl |= TF_WARN;
       }
-      w_fl[tid] = fl;
+    }
+    const bool effect = fl & TF_EFFECT;
+    // 2. claims
+    if (effect) {
+      atomicMin(&W.claim[p1], t);
+      if (is_acc) rm_walk(W, rec, [&](int q) { atomicMin(&W.claim[q], t); });
+      else atomicMin(&W.claim[p2], t);
+    }
+    __syncthreads();
+    // 3. resolve, and what each winner creates
+    bool win = false;
+    int c_app = 0, c_id = 0, c_info = 0, c_hole = 0;
+    if (effect) {
+      if (is_acc) {
+        win = claim_load(W, p1) == t;
+        rm_walk(W, rec, [&](int q) { win = win && claim_load(W, q) == t; });
+        rd_walk(W, rec, [&](int q) { win = win && claim_load(W, q) >= t; });
+        if (win) {
+          const int died = (W.r_flags[rec] & RF_DIED) ? 1 : 0;
+          c_app = W.r_ndup[rec] - W.r_ndead[rec];
+          c_id = W.r_ndup[rec];
+          c_info = W.r_nrm[rec] + died + W.r_ndead[rec];
+          c_hole = W.r_nrm[rec] + died;
+        }
+      } else {
+        win = claim_load(W, p1) == t && claim_load(W, p2) == t;
+        if (win) {
+          int c1, c2, b1, b2;
+          decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
+          const int nb = B->n_bin, nc = B->w.n_class;
+          const int cc = A.cc[table_index(A, box, nb, nc, b1, b2, c1, c2)];
+          Stream s;
+          s.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), t - W.pair_exc[k],
+                 RNG_MERGE);
+          int g;
+          const int o = merge_decide(A, W, B, s, p1, p2, c1, c2, cc, &g);
+          c_app = (o & O_APPEND) ? 1 : 0;
+          c_id = (o & O_NEWID) ? 1 : 0;
+          c_info = ((o & O_INFO1) ? 1 : 0) + ((o & O_INFO2) ? 1 : 0);
+          c_hole = ((o & O_RM1) && !(o & O_SLOT1) ? 1 : 0) + ((o & O_RM2) && !(o & O_SLOT2) ? 1 : 0);
+        }
+      }
+    }
+    int r_app, r_id, r_info, r_hole, tot;
+    BS(scan_tmp).ExclusiveSum(c_app, r_app, tot);
+    if (tid == 0) s_tot[0] = tot;
+    __syncthreads();
+    BS(scan_tmp).ExclusiveSum(c_id, r_id, tot);
+    if (tid == 0) s_tot[1] = tot;
+    __syncthreads();
+    BS(scan_tmp).ExclusiveSum(c_info, r_info, tot);
+    if (tid == 0) s_tot[2] = tot;
+    __syncthreads();
+    BS(scan_tmp).ExclusiveSum(c_hole, r_hole, tot);
+    if (tid == 0) {
+      s_tot[3] = tot;
+      s_hole_base = tot ? (long long)atomicAdd(W.hole_top, (unsigned long long)tot) : 0ll;
+    }
+    __syncthreads();
+    // 4. apply the winners at their ranks
+    if (win) {
+      RankAlloc al{B, r_app, r_id, r_info, (int64_t)s_hole_base + r_hole};
+      if (is_acc) acc_apply(B, A, W, box, t, rec, p1, type, want_terms, rid, al);
+      else perset_merge(B, A, W, box, k, t, p1, p2, seed, step, type, want_terms, rid, al);
     }
     tail_barrier();
-    // 2. lane 0 commits the window in sequential order
     if (tid == 0) {
-      SeqAlloc al{B, W.hole_top};
-      int n_chg = 0;
-      bool all_changed = false;
-      auto changed = [&](int p) {
-        if (p < 0) return false;
-        if (all_changed) return true;
-        for (int c = 0; c < n_chg; ++c)
-          if (chg[c] == p) return true;
-        return false;
-      };
-      auto mark = [&](int p) {
-        if (p < 0 || all_changed) return;
-        if (n_chg < TAIL_CHG) chg[n_chg++] = p;
-        else all_changed = true;
-      };
-      for (int i = 0; i < wn && !B->error; ++i) {
-        const uint32_t t = pend[head + i];
-        const int64_t k = W.pend_k[head + i];
-        int fl = w_fl[i];
-        if (fl & TF_ACC) {
-          const int rec = (int)(W.acc_exc[k] + (t - W.pair_exc[k]) - acc_base);
-          int p, tns;
-          const int r = acc_walk(boxes, A, W, k, t, rec, type, seed, step, del_t, false, &p, &tns);
-          if (r == 1) {
-            acc_apply(B, A, W, box, t, rec, p, type, want_terms, rid, al);
-            mark(p);
-            rm_walk(W, rec, [&](int q) { mark(q); });
-          } else if (r == 0 && tns) {
-            const unsigned long long s = (unsigned long long)(tns & (ACC_WARN - 1));
-            B->n_samp += (int64_t)s;
-            B->step_samp += (int64_t)s;
-            if (tns & ACC_WARN) B->n_warn += 1;
-          }
-          continue;
+      B->n_part += s_tot[0];
+      B->next_id += s_tot[1];
+      B->n_info += s_tot[2];
+    }
+    // 5. release the claims
+    if (effect) {
+      claim_release(W, p1);
+      if (is_acc) rm_walk(W, rec, [&](int q) { claim_release(W, q); });
+      else claim_release(W, p2);
+    }
+    // 6. what stays in the window: losers (run again only if an earlier
+    // winner changed what they read) and tests whose inputs changed
+    int keep = 0;
+    if (active && !win) {
+      bool changed = consumed_before(W, p1, t, rid) || consumed_before(W, p2, t, rid);
+      if (is_acc && effect && !changed)
+        rm_walk(W, rec, [&](int q) { changed = changed || consumed_before(W, q, t, rid); });
+      if (is_acc && p1 >= 0 && !changed)
+        rd_walk(W, rec, [&](int q) { changed = changed || consumed_before(W, q, t, rid); });
+      if (effect || changed) {
+        keep = 1;
+        if (changed) fl |= TF_RUN;
+      } else {
+        // final without effect: its counts
+        if (is_acc && tns) {
+          unsigned long long s = (unsigned long long)(tns & (ACC_WARN - 1));
+          atomicAdd((unsigned long long *)&B->n_samp, s);
+          atomicAdd((unsigned long long *)&B->step_samp, s);
+          if (tns & ACC_WARN) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
         }
-        int p1 = w_p1[i], p2 = w_p2[i];
-        if (p1 < 0 || changed(p1) || changed(p2)) {
-          // drawn on a state an earlier commit changed: run it again now
-          TestOut o;
-          perset_test(B, A, W, box, k, t, type, seed, step, precision, band, 0, o);
-          p1 = o.p1;
-          p2 = o.p2;
-          fl = (o.accept ? TF_EFFECT : 0) | (o.warn ? TF_WARN : 0);
-        }
-        if (fl & TF_WARN) B->n_warn += 1;
-        if (fl & TF_EFFECT) {
-          perset_merge(B, A, W, box, k, t, p1, p2, seed, step, type, want_terms, rid, al);
-          mark(p1);
-          mark(p2);
-        }
+        if (!is_acc && (fl & TF_WARN)) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
       }
     }
-    n_windows++;
+    if (win && !is_acc && (fl & TF_WARN)) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
+    int r_keep;
+    BS(scan_tmp).ExclusiveSum(keep, r_keep, tot);
+    __syncthreads();
+    if (keep) {
+      w_t[r_keep] = t;
+      w_k[r_keep] = (int)(k - pb);
+      w_fl[r_keep] = fl;
+      w_p1[r_keep] = p1;
+      w_p2[r_keep] = p2;
+      w_tns[r_keep] = tns;
+    }
+    wn = tot;
+    rid++;
+    n_rounds++;
   }
-  if (tid == 0 && n_windows) atomicAdd(rounds_out, n_windows);
+  if (tid == 0 && n_rounds) atomicAdd(rounds_out, n_rounds);
 }
 
 // the pair of every pending test, found in parallel before the tail
@@ -2342,7 +2433,6 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         PLAUNCH(W, "k_par_tail", k_par_tail, dim3(n_box), dim3(TAIL_THREADS), 0, st, d_box, A, d,
                 n_box, n, pend, type, seed, step, precision, band, del_t, a0,
                 want_terms, (int)W.rid + 1, d.tail_rounds);
-        // (the tail's count is of windows)
         PCHECK(hipMemcpyAsync(W.h_u64, d.hole_top, sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, st));
         PCHECK(hipMemcpyAsync(W.h_u64 + 1, d.tail_rounds, sizeof(unsigned long long),
@@ -2350,7 +2440,7 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         PCHECK(hipStreamSynchronize(st));
         n_holes = (int64_t)W.h_u64[0];
         W.n_tail_rounds += (int64_t)W.h_u64[1];
-        W.rid += 2;  // past the round id the tail used
+        W.rid += (int64_t)W.h_u64[1] + 2;  // past every round id the tail used
         W.n_tail += n;
         break;
       }
