@@ -56,7 +56,7 @@ def kernel_bytes(name, n_spec, precision, part_steps_per_launch, tests_per_launc
         return 8.0 * part_steps_per_launch                   # read key, write perm
     if name == "k_par_pairs":
         return 20.0 * pairs_per_launch                       # 2 counts, k_max, n_samp
-    if name in ("k_par_tests", "k_exact_sweep"):
+    if name in ("k_par_tests", "k_exact_sweep", "k_par_tail"):
         per = 8.0 + (32.0 if precision == "fp32" else 2 * 8.0 * n_spec)
         return per * tests_per_launch                        # 2 perm + 2 particles
     if name == "k_par_apply":
@@ -187,6 +187,10 @@ def main():
         if dom in ("k_par_tests", "k_par_apply"):
             traffic = kernel_bytes(dom, n_spec, args.precision, psteps / args.steps,
                                    n_samp / n, pairs, n_coag / n)
+        if dom == "k_par_tail":
+            # the tests the tail finished, per launch (one launch per step)
+            traffic = kernel_bytes(dom, n_spec, args.precision, psteps / args.steps,
+                                   r["tail_tests"] / n, pairs, n_coag / n)
         avg_s = ms / n / 1e3
         achieved = traffic / avg_s / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
