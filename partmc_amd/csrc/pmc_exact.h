@@ -937,6 +937,11 @@ __device__ inline uint32_t pair_site(int nb, int nc, int b1, int b2, int c1, int
   return (uint32_t)((((c1 - 1) * nc + (c2 - 1)) * nb + (b1 - 1)) * nb + (b2 - 1));
 }
 
+// Lane 0's writes become visible to the other lanes of its wavefront: the
+// box lives in one workgroup, so a workgroup-scope fence suffices (a
+// device-scope fence would write back and invalidate the whole L2).
+__device__ inline void wave_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
 // The sequential sweep: one 64-lane wavefront per box.
 __global__ void __launch_bounds__(64)
 k_exact_sweep(BoxDesc *boxes, Arrays A, int type, double del_t, uint64_t seed,
@@ -1037,7 +1042,7 @@ k_exact_sweep(BoxDesc *boxes, Arrays A, int type, double del_t, uint64_t seed,
                     }
                   }
                   err = __shfl(err, 0);
-                  __threadfence();
+                  wave_fence();
                 }
               }
             }
@@ -1112,7 +1117,7 @@ k_exact_sweep(BoxDesc *boxes, Arrays A, int type, double del_t, uint64_t seed,
               err = exact_coagulate(E, w, ms, a1, a2, c1, c2, cc);
             }
             err = __shfl(err, 0);
-            __threadfence();
+            wave_fence();
             tot_coag += 1;
             i = i + first + 1;
           }

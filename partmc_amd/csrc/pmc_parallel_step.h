@@ -1815,11 +1815,12 @@ __device__ inline void claim_release(const PW &W, int p) {
   __hip_atomic_store(&W.claim[p], UNCLAIMED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// a phase boundary of the tail: earlier writes and atomics of the box's
-// wavefront become visible to its next phase (device-scope fence, then the
-// barrier)
+// a phase boundary of the tail: earlier writes of the box's workgroup
+// become visible to its next phase.  Only this workgroup touches the box,
+// so a workgroup-scope fence suffices; claims go through device-scope
+// atomics.
 __device__ inline void tail_barrier() {
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   __syncthreads();
 }
 
