@@ -1450,7 +1450,7 @@ __global__ void k_par_resolve(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, 
     Stream s;
     s.open(seed, m.box, step, pair_site(nb, nc, m.b1, m.b2, m.c1, m.c2), m.j, RNG_MERGE);
     int g;
-    o = merge_decide(A, W, B, s, p1, p2, m.c1, m.c2, cc, &g) | O_WIN;
+    o = merge_decide(A, W, B, s, p1, p2, m.c1, m.c2, cc, &g) | O_WIN | (g << 16);
   }
   W.ev_out[e] = o;
   W.f_app[e] = (o & O_APPEND) ? 1 : 0;
@@ -1506,6 +1506,7 @@ __device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc 
 struct RankAlloc {
   BoxDesc *B;
   int64_t app, id, info, hole;  // this event's next ranks
+  int *live;                    // the box's live counts per list key
   __device__ int64_t append_slot() {
     int64_t r = app++;
     return B->n_part + r < B->cap ? B->off + B->n_part + r : -1;
@@ -1515,22 +1516,32 @@ struct RankAlloc {
   }
   __device__ int64_t info_index() { return B->n_info + info++; }
   __device__ int64_t hole_index() { return hole++; }
+  __device__ void live_add(int key, int d) { atomicAdd(&live[key], d); }
+  __device__ void count(int64_t samp, int64_t coag, int warn) {
+    if (samp) {
+      atomicAdd((unsigned long long *)&B->n_samp, (unsigned long long)samp);
+      atomicAdd((unsigned long long *)&B->step_samp, (unsigned long long)samp);
+    }
+    if (coag) {
+      atomicAdd((unsigned long long *)&B->n_coag, (unsigned long long)coag);
+      atomicAdd((unsigned long long *)&B->step_coag, (unsigned long long)coag);
+    }
+    if (warn) atomicAdd((unsigned long long *)&B->n_warn, (unsigned long long)warn);
+  }
 };
 
-struct SeqAlloc {
-  BoxDesc *B;
-  unsigned long long *hole_top;
-  __device__ int64_t append_slot() {
-    if (B->n_part >= B->cap) return -1;
-    return B->off + B->n_part++;
+// The tail's allocator: ranks as in a round, live-count changes gathered
+// in the workgroup's shared memory and counters in the lane, both flushed
+// once (many winners of one box would otherwise contend on the same words)
+struct TailAlloc : RankAlloc {
+  int *s_live;
+  int64_t n_samp, n_coag, n_warn;
+  __device__ void live_add(int key, int d) { atomicAdd(&s_live[key], d); }
+  __device__ void count(int64_t samp, int64_t coag, int warn) {
+    n_samp += samp;
+    n_coag += coag;
+    n_warn += warn;
   }
-  __device__ int64_t new_id() {
-    int64_t i = (B->next_id - 1) * B->id_stride + B->id_offset + 1;
-    B->next_id += 1;
-    return i;
-  }
-  __device__ int64_t info_index() { return B->n_info++; }
-  __device__ int64_t hole_index() { return (int64_t)atomicAdd(hole_top, 1ull); }
 };
 
 template <class Alloc>
@@ -1544,40 +1555,97 @@ __device__ inline void put_info(const Arrays &A, BoxDesc *B, Alloc &al, int64_t 
   }
 }
 
+// the merged particle of an accepted per-set test written straight from
+// its parents' slots into dst (which may be either parent's slot):
+// aero_particle_coagulate (src/aero_particle.F90:917-1033) for the case
+// without component resampling, n_comp1 + n_comp2 <= MAX_COMP
+__device__ inline int direct_merge(const Arrays &A, const PW &W, BoxDesc *B, int64_t p1,
+                                   int64_t p2, int64_t dst, int64_t id, int group, int klass,
+                                   int type, int want_terms) {
+  const int64_t tc = A.total_cap;
+  const int n1 = A.ncomp[p1], n2 = A.ncomp[p2];
+  const int np1 = A.nprim[p1], np2 = A.nprim[p2];
+  if (np1 < n1) return 465791384;
+  if (np2 < n2) return 465791385;
+  const double least = fmin(A.least[p1], A.least[p2]);
+  const double greatest = fmax(A.greatest[p1], A.greatest[p2]);
+  double v = 0.0, m = 0.0;
+  for (int s = 0; s < A.n_spec; ++s) {
+    const double x = A.vol[(int64_t)s * tc + p1] + A.vol[(int64_t)s * tc + p2];
+    A.vol[(int64_t)s * tc + dst] = x;
+    v += x;
+    m += x * A.density[s];
+  }
+  // components: the first parent's, then the second's
+  if (dst == p1) {
+    for (int c = 0; c < n2; ++c) {
+      A.csrc[(int64_t)(n1 + c) * tc + dst] = A.csrc[(int64_t)c * tc + p2];
+      A.ctime[(int64_t)(n1 + c) * tc + dst] = A.ctime[(int64_t)c * tc + p2];
+    }
+  } else {
+    for (int c = n2 - 1; c >= 0; --c) {  // moves up: safe when dst == p2
+      A.csrc[(int64_t)(n1 + c) * tc + dst] = A.csrc[(int64_t)c * tc + p2];
+      A.ctime[(int64_t)(n1 + c) * tc + dst] = A.ctime[(int64_t)c * tc + p2];
+    }
+    for (int c = 0; c < n1; ++c) {
+      A.csrc[(int64_t)c * tc + dst] = A.csrc[(int64_t)c * tc + p1];
+      A.ctime[(int64_t)c * tc + dst] = A.ctime[(int64_t)c * tc + p1];
+    }
+  }
+  A.vtot[dst] = v;
+  A.id[dst] = id;
+  A.group[dst] = group;
+  A.klass[dst] = klass;
+  A.nprim[dst] = np1 + np2;
+  A.ncomp[dst] = n1 + n2;
+  A.least[dst] = least;
+  A.greatest[dst] = greatest;
+  W.status[dst] = ST_NEW;
+  const int nb = B->n_bin;
+  const int b = bin_find_log(nb, B->log_lo, B->log_hi, vol2rad(v));
+  if (b < 1 || b > nb) {
+    W.key[dst] = 0;
+    return 134572570;
+  }
+  W.key[dst] = par_key(b, klass, nb);
+  if (want_terms) W.terms[dst] = particle_terms(type, v, m / v, B->env);
+  return 0;
+}
+
 // coagulate_weighting + coagulate (src/coagulation.F90:810-985) of an
 // accepted per-set test: removals become holes (or the slot the merged
 // particle takes), the merged particle is written, removal records follow
-// the reference's order
+// the reference's order.  o_known >= 0 passes the outcome of merge_decide
+// already taken on this test's stream (with the new group in bits 16+).
 template <class Alloc>
 __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, int box, int64_t k,
                                     uint32_t t, int64_t p1, int64_t p2, uint64_t seed,
-                                    uint32_t step, int type, int want_terms, int rid, Alloc &al) {
+                                    uint32_t step, int type, int want_terms, int rid, Alloc &al,
+                                    int o_known = -1) {
   int c1, c2, b1, b2;
   decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
   const uint32_t j = t - W.pair_exc[k];
-  atomicAdd((unsigned long long *)&B->n_coag, 1ull);
-  atomicAdd((unsigned long long *)&B->step_coag, 1ull);
+  al.count(0, 1, 0);
   const int nb = B->n_bin, nc = B->w.n_class;
   const int cc = A.cc[table_index(A, box, nb, nc, b1, b2, c1, c2)];
-  int *live = W.live + (int64_t)box * W.K_stride;
+  const int key1 = W.key[p1], key2 = W.key[p2];
+  const int64_t id1 = A.id[p1], id2 = A.id[p2];
+  int o, new_group;
   Stream s;
-  s.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), j, RNG_MERGE);
-  int new_group;
-  const int o = merge_decide(A, W, B, s, p1, p2, c1, c2, cc, &new_group);
-  Rec pt1, pt2, ptc;
-  load_rec(A, p1, pt1);
-  load_rec(A, p2, pt2);
+  const bool resample = (A.ncomp[p1] + A.ncomp[p2] > MAX_COMP);
+  if (o_known >= 0 && !resample) {
+    // no draws follow the merge decisions: the stream is not needed again
+    o = o_known & 0xFFFF;
+    new_group = o_known >> 16;
+  } else {
+    s.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), j, RNG_MERGE);
+    o = merge_decide(A, W, B, s, p1, p2, c1, c2, cc, &new_group);
+  }
   int64_t info_other = 0;
   int64_t slot = -1;
   if (o & O_NEW) {
-    int err = par_rec_coagulate(A.n_spec, s, pt1, pt2, ptc);
-    if (err) atomicExch(&B->error, err);
-    ptc.group = new_group;
-    ptc.klass = cc;
-    if (o & O_ID1) ptc.id = pt1.id;
-    else if (o & O_ID2) ptc.id = pt2.id;
-    else ptc.id = al.new_id();
-    info_other = ptc.id;
+    const int64_t nid = (o & O_ID1) ? id1 : (o & O_ID2) ? id2 : al.new_id();
+    info_other = nid;
     if (o & O_SLOT1) slot = p1;
     else if (o & O_SLOT2) slot = p2;
     else {
@@ -1587,11 +1655,25 @@ __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, in
         return;
       }
     }
+    int err;
+    if (!resample) {
+      err = direct_merge(A, W, B, p1, p2, slot, nid, new_group, cc, type, want_terms);
+    } else {
+      Rec pt1, pt2, ptc;
+      load_rec(A, p1, pt1);
+      load_rec(A, p2, pt2);
+      err = par_rec_coagulate(A.n_spec, s, pt1, pt2, ptc);
+      ptc.group = new_group;
+      ptc.klass = cc;
+      ptc.id = nid;
+      if (!err) write_new_particle(A, W, B, slot, ptc, nb, type, want_terms);
+    }
+    if (err) atomicExch(&B->error, err);
   }
   if (o & O_RM1) {
     W.dead_t[p1] = t;
     W.dead_r[p1] = rid;
-    atomicSub(&live[W.key[p1]], 1);
+    al.live_add(key1, -1);
     if (!(o & O_SLOT1)) {
       W.status[p1] = ST_HOLE;
       W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(p1 - B->off);
@@ -1600,18 +1682,15 @@ __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, in
   if (o & O_RM2) {
     W.dead_t[p2] = t;
     W.dead_r[p2] = rid;
-    atomicSub(&live[W.key[p2]], 1);
+    al.live_add(key2, -1);
     if (!(o & O_SLOT2)) {
       W.status[p2] = ST_HOLE;
       W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(p2 - B->off);
     }
   }
-  if (slot >= 0) {
-    write_new_particle(A, W, B, slot, ptc, nb, type, want_terms);
-    atomicAdd(&live[W.key[slot]], 1);
-  }
-  if (o & O_INFO1) put_info(A, B, al, pt1.id, 2, info_other);
-  if (o & O_INFO2) put_info(A, B, al, pt2.id, 2, info_other);
+  if (slot >= 0) al.live_add(W.key[slot], 1);
+  if (o & O_INFO1) put_info(A, B, al, id1, 2, info_other);
+  if (o & O_INFO2) put_info(A, B, al, id2, 2, info_other);
 }
 
 // a winning target event (coag_target_with_source and
@@ -1624,7 +1703,6 @@ __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, in
 template <class Alloc>
 __device__ inline void acc_apply(BoxDesc *B, const Arrays &A, const PW &W, int box, uint32_t t,
                                  int r, int64_t p, int type, int want_terms, int rid, Alloc &al) {
-  int *live = W.live + (int64_t)box * W.K_stride;
   const int ns = A.n_spec;
   const int nb = B->n_bin;
   const int64_t target_id = A.id[p];
@@ -1645,32 +1723,28 @@ __device__ inline void acc_apply(BoxDesc *B, const Arrays &A, const PW &W, int b
     }
     W.dead_t[q] = t;
     W.dead_r[q] = rid;
-    atomicSub(&live[W.key[q]], 1);
+    al.live_add(W.key[q], -1);
     W.status[q] = ST_HOLE;
     W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(q - B->off);
     put_info(A, B, al, A.id[q], 2, owner);
   });
   const unsigned long long n_samp = (unsigned long long)W.r_nsamp[r];
   const unsigned long long n_coag = (unsigned long long)W.r_ncoag[r];
-  atomicAdd((unsigned long long *)&B->n_samp, n_samp);
-  atomicAdd((unsigned long long *)&B->step_samp, n_samp);
-  atomicAdd((unsigned long long *)&B->n_coag, n_coag);
-  atomicAdd((unsigned long long *)&B->step_coag, n_coag);
-  if (flags & RF_WARN) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
+  al.count((int64_t)n_samp, (int64_t)n_coag, (flags & RF_WARN) ? 1 : 0);
   if (W.r_err[r]) atomicExch(&B->error, W.r_err[r]);
   if (!(flags & RF_MERGED)) return;
   Rec np;
   pool_load(W, ns, r, klass, target_id, np);
   W.dead_t[p] = t;
   W.dead_r[p] = rid;
-  atomicSub(&live[W.key[p]], 1);
+  al.live_add(W.key[p], -1);
   write_new_particle(A, W, B, p, np, nb, type, want_terms);
   if (flags & RF_DIED) {
     W.status[p] = ST_HOLE;
     W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(p - B->off);
     put_info(A, B, al, target_id, 4, 0);
   } else {
-    atomicAdd(&live[W.key[p]], 1);
+    al.live_add(W.key[p], 1);
   }
   int kk = 0;
   for (int d = W.r_dhead[r]; d >= 0; d = W.q_next[d]) {
@@ -1686,7 +1760,7 @@ __device__ inline void acc_apply(BoxDesc *B, const Arrays &A, const PW &W, int b
       return;
     }
     write_new_particle(A, W, B, slot, np, nb, type, want_terms);
-    atomicAdd(&live[W.key[slot]], 1);
+    al.live_add(W.key[slot], 1);
   }
 }
 
@@ -1701,13 +1775,15 @@ __global__ void k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, ui
   BoxDesc *B = &boxes[box];
   const int first = W.box_first[box];
   RankAlloc al{B, W.s_app[e] - W.s_app[first], W.s_id[e] - W.s_id[first],
-               W.s_info[e] - W.s_info[first], hole_base + W.s_hole[e]};
+               W.s_info[e] - W.s_info[first], hole_base + W.s_hole[e],
+               W.live + (int64_t)box * W.K_stride};
   if (W.ev_rec[e] >= 0) {
     acc_apply(B, A, W, box, t, W.ev_rec[e], W.ev_p1[e], type, want_terms, rid, al);
     return;
   }
   const int64_t k = find_pair(W.pair_exc, (int64_t)n_box * W.P_stride, t);
-  perset_merge(B, A, W, box, k, t, W.ev_p1[e], W.ev_p2[e], seed, step, type, want_terms, rid, al);
+  perset_merge(B, A, W, box, k, t, W.ev_p1[e], W.ev_p2[e], seed, step, type, want_terms, rid, al,
+               W.ev_out[e] & ~O_WIN);
 }
 
 // per-box totals of a round: appended slots, new ids, records, winners
@@ -1853,6 +1929,9 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
   __shared__ int s_tot[4];
   __shared__ long long s_hole_base;
   __shared__ int s_err;
+  extern __shared__ int s_live[];  // live-count changes of the box's lists
+  for (int q = tid; q < W.K_stride; q += TAIL_THREADS) s_live[q] = 0;
+  int64_t c_samp = 0, c_coag = 0, c_warn = 0;  // this lane's counter changes
   int wn = 0, head = lo;
   int rid = rid0;
   unsigned long long n_rounds = 0;
@@ -1915,6 +1994,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     __syncthreads();
     // 3. resolve, and what each winner creates
     bool win = false;
+    int o_win = -1;
     int c_app = 0, c_id = 0, c_info = 0, c_hole = 0;
     if (effect) {
       if (is_acc) {
@@ -1940,6 +2020,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
                  RNG_MERGE);
           int g;
           const int o = merge_decide(A, W, B, s, p1, p2, c1, c2, cc, &g);
+          o_win = o | (g << 16);
           c_app = (o & O_APPEND) ? 1 : 0;
           c_id = (o & O_NEWID) ? 1 : 0;
           c_info = ((o & O_INFO1) ? 1 : 0) + ((o & O_INFO2) ? 1 : 0);
@@ -1965,9 +2046,20 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     __syncthreads();
     // 4. apply the winners at their ranks
     if (win) {
-      RankAlloc al{B, r_app, r_id, r_info, (int64_t)s_hole_base + r_hole};
+      TailAlloc al;
+      al.B = B;
+      al.app = r_app;
+      al.id = r_id;
+      al.info = r_info;
+      al.hole = (int64_t)s_hole_base + r_hole;
+      al.live = nullptr;
+      al.s_live = s_live;
+      al.n_samp = al.n_coag = al.n_warn = 0;
       if (is_acc) acc_apply(B, A, W, box, t, rec, p1, type, want_terms, rid, al);
-      else perset_merge(B, A, W, box, k, t, p1, p2, seed, step, type, want_terms, rid, al);
+      else perset_merge(B, A, W, box, k, t, p1, p2, seed, step, type, want_terms, rid, al, o_win);
+      c_samp += al.n_samp;
+      c_coag += al.n_coag;
+      c_warn += al.n_warn;
     }
     tail_barrier();
     if (tid == 0) {
@@ -1996,15 +2088,13 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
       } else {
         // final without effect: its counts
         if (is_acc && tns) {
-          unsigned long long s = (unsigned long long)(tns & (ACC_WARN - 1));
-          atomicAdd((unsigned long long *)&B->n_samp, s);
-          atomicAdd((unsigned long long *)&B->step_samp, s);
-          if (tns & ACC_WARN) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
+          c_samp += tns & (ACC_WARN - 1);
+          if (tns & ACC_WARN) c_warn += 1;
         }
-        if (!is_acc && (fl & TF_WARN)) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
+        if (!is_acc && (fl & TF_WARN)) c_warn += 1;
       }
     }
-    if (win && !is_acc && (fl & TF_WARN)) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
+    if (win && !is_acc && (fl & TF_WARN)) c_warn += 1;
     int r_keep;
     BS(scan_tmp).ExclusiveSum(keep, r_keep, tot);
     __syncthreads();
@@ -2020,6 +2110,20 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     rid++;
     n_rounds++;
   }
+  __syncthreads();
+  // flush the gathered live-count and counter changes
+  int *live = W.live + (int64_t)box * W.K_stride;
+  for (int q = tid; q < W.K_stride; q += TAIL_THREADS)
+    if (s_live[q]) atomicAdd(&live[q], s_live[q]);
+  if (c_samp) {
+    atomicAdd((unsigned long long *)&B->n_samp, (unsigned long long)c_samp);
+    atomicAdd((unsigned long long *)&B->step_samp, (unsigned long long)c_samp);
+  }
+  if (c_coag) {
+    atomicAdd((unsigned long long *)&B->n_coag, (unsigned long long)c_coag);
+    atomicAdd((unsigned long long *)&B->step_coag, (unsigned long long)c_coag);
+  }
+  if (c_warn) atomicAdd((unsigned long long *)&B->n_warn, (unsigned long long)c_warn);
   if (tid == 0 && n_rounds) atomicAdd(rounds_out, n_rounds);
 }
 
@@ -2431,7 +2535,8 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
                            (unsigned long long)n_holes);
         hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, d.tail_rounds, 0ull);
         hipLaunchKernelGGL(k_pend_pairs, dim3((n + 255) / 256), dim3(256), 0, st, d, W.NP, pend, n);
-        PLAUNCH(W, "k_par_tail", k_par_tail, dim3(n_box), dim3(TAIL_THREADS), 0, st, d_box, A, d,
+        PLAUNCH(W, "k_par_tail", k_par_tail, dim3(n_box), dim3(TAIL_THREADS),
+                d.K_stride * sizeof(int), st, d_box, A, d,
                 n_box, n, pend, type, seed, step, precision, band, del_t, a0,
                 want_terms, (int)W.rid + 1, d.tail_rounds);
         PCHECK(hipMemcpyAsync(W.h_u64, d.hole_top, sizeof(unsigned long long),
