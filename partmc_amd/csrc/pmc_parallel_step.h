@@ -1555,13 +1555,73 @@ __device__ inline void put_info(const Arrays &A, BoxDesc *B, Alloc &al, int64_t 
   }
 }
 
+// the merged particle's components when the parents hold more than
+// MAX_COMP together (aero_particle_coagulate, src/aero_particle.F90:
+// 917-1033): prob_round of the first parent's share, then two selection
+// samples (sample_without_replacement keeps sorted order), streamed from
+// the parents into dst.  The parent that dst overwrites is read into
+// registers first; the loops are unrolled so the arrays stay in registers.
+__device__ inline void resample_components(const Arrays &A, Stream &s, int64_t p1, int64_t p2,
+                                           int64_t dst, int n1, int n2, int np1, int np2) {
+  const int64_t tc = A.total_cap;
+  const int n1_new = prob_round(s, (double)np1 / (double)(np1 + np2) * (double)MAX_COMP);
+  const int n2_new = MAX_COMP - n1_new;
+  const int64_t keep = (dst == p1) ? p1 : p2;  // the parent read ahead
+  int ks[MAX_COMP];
+  double kt[MAX_COMP];
+  const int nk = (dst == p1) ? n1 : n2;
+#pragma unroll
+  for (int c = 0; c < MAX_COMP; ++c) {
+    if (c < nk) {
+      ks[c] = A.csrc[(int64_t)c * tc + keep];
+      kt[c] = A.ctime[(int64_t)c * tc + keep];
+    }
+  }
+  const bool ahead1 = (dst == p1), ahead2 = (dst == p2);
+  // first parent: n1_new of its n1 components into positions 0 ..
+  {
+    int m = 0;
+#pragma unroll
+    for (int c = 0; c < MAX_COMP; ++c) {
+      if (c < n1 && m < n1_new) {
+        const double r = s.uniform();
+        if (!((double)(n1 - c) * r >= (double)(n1_new - m))) {
+          const int cs = ahead1 ? ks[c] : A.csrc[(int64_t)c * tc + p1];
+          const double ct = ahead1 ? kt[c] : A.ctime[(int64_t)c * tc + p1];
+          A.csrc[(int64_t)m * tc + dst] = cs;
+          A.ctime[(int64_t)m * tc + dst] = ct;
+          m++;
+        }
+      }
+    }
+  }
+  // second parent: n2_new of its n2 components after them
+  {
+    int m = 0;
+#pragma unroll
+    for (int c = 0; c < MAX_COMP; ++c) {
+      if (c < n2 && m < n2_new) {
+        const double r = s.uniform();
+        if (!((double)(n2 - c) * r >= (double)(n2_new - m))) {
+          const int cs = ahead2 ? ks[c] : A.csrc[(int64_t)c * tc + p2];
+          const double ct = ahead2 ? kt[c] : A.ctime[(int64_t)c * tc + p2];
+          A.csrc[(int64_t)(n1_new + m) * tc + dst] = cs;
+          A.ctime[(int64_t)(n1_new + m) * tc + dst] = ct;
+          m++;
+        }
+      }
+    }
+  }
+}
+
 // the merged particle of an accepted per-set test written straight from
 // its parents' slots into dst (which may be either parent's slot):
-// aero_particle_coagulate (src/aero_particle.F90:917-1033) for the case
-// without component resampling, n_comp1 + n_comp2 <= MAX_COMP
+// aero_particle_coagulate (src/aero_particle.F90:917-1033).  Components
+// are concatenated, or resampled on `rs` (the merge stream after the merge
+// decisions) when the parents hold more than MAX_COMP together.
 __device__ inline int direct_merge(const Arrays &A, const PW &W, BoxDesc *B, int64_t p1,
                                    int64_t p2, int64_t dst, int64_t id, int group, int klass,
-                                   int type, int want_terms) {
+                                   int type, int want_terms, Stream *rs) {
   const int64_t tc = A.total_cap;
   const int n1 = A.ncomp[p1], n2 = A.ncomp[p2];
   const int np1 = A.nprim[p1], np2 = A.nprim[p2];
@@ -1577,7 +1637,11 @@ __device__ inline int direct_merge(const Arrays &A, const PW &W, BoxDesc *B, int
     m += x * A.density[s];
   }
   // components: the first parent's, then the second's
-  if (dst == p1) {
+  int n_out = n1 + n2;
+  if (n1 + n2 > MAX_COMP) {
+    resample_components(A, *rs, p1, p2, dst, n1, n2, np1, np2);
+    n_out = MAX_COMP;
+  } else if (dst == p1) {
     for (int c = 0; c < n2; ++c) {
       A.csrc[(int64_t)(n1 + c) * tc + dst] = A.csrc[(int64_t)c * tc + p2];
       A.ctime[(int64_t)(n1 + c) * tc + dst] = A.ctime[(int64_t)c * tc + p2];
@@ -1597,7 +1661,7 @@ __device__ inline int direct_merge(const Arrays &A, const PW &W, BoxDesc *B, int
   A.group[dst] = group;
   A.klass[dst] = klass;
   A.nprim[dst] = np1 + np2;
-  A.ncomp[dst] = n1 + n2;
+  A.ncomp[dst] = n_out;
   A.least[dst] = least;
   A.greatest[dst] = greatest;
   W.status[dst] = ST_NEW;
@@ -1655,19 +1719,7 @@ __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, in
         return;
       }
     }
-    int err;
-    if (!resample) {
-      err = direct_merge(A, W, B, p1, p2, slot, nid, new_group, cc, type, want_terms);
-    } else {
-      Rec pt1, pt2, ptc;
-      load_rec(A, p1, pt1);
-      load_rec(A, p2, pt2);
-      err = par_rec_coagulate(A.n_spec, s, pt1, pt2, ptc);
-      ptc.group = new_group;
-      ptc.klass = cc;
-      ptc.id = nid;
-      if (!err) write_new_particle(A, W, B, slot, ptc, nb, type, want_terms);
-    }
+    const int err = direct_merge(A, W, B, p1, p2, slot, nid, new_group, cc, type, want_terms, &s);
     if (err) atomicExch(&B->error, err);
   }
   if (o & O_RM1) {
@@ -1922,7 +1974,9 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
   const int hi = lower_bound_u32(pend, n, W.pair_exc[pb + W.P_stride]);
   if (lo >= hi) return;
   using BS = hipcub::BlockScan<int, TAIL_THREADS>;
+  using BS2 = hipcub::BlockScan<long long, TAIL_THREADS>;
   __shared__ typename BS::TempStorage scan_tmp;
+  __shared__ typename BS2::TempStorage scan_tmp2;
   __shared__ uint32_t w_t[TAIL_THREADS];
   __shared__ int w_k[TAIL_THREADS], w_p1[TAIL_THREADS], w_p2[TAIL_THREADS];
   __shared__ int w_tns[TAIL_THREADS], w_fl[TAIL_THREADS];
@@ -2028,20 +2082,23 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
         }
       }
     }
-    int r_app, r_id, r_info, r_hole, tot;
-    BS(scan_tmp).ExclusiveSum(c_app, r_app, tot);
-    if (tid == 0) s_tot[0] = tot;
+    // ranks of what the winners create: two scans of packed counts
+    long long ai = ((long long)c_id << 32) | (long long)c_app;
+    long long ih = ((long long)c_hole << 32) | (long long)c_info;
+    long long r_ai, r_ih, t_ai, t_ih;
+    BS2(scan_tmp2).ExclusiveSum(ai, r_ai, t_ai);
     __syncthreads();
-    BS(scan_tmp).ExclusiveSum(c_id, r_id, tot);
-    if (tid == 0) s_tot[1] = tot;
-    __syncthreads();
-    BS(scan_tmp).ExclusiveSum(c_info, r_info, tot);
-    if (tid == 0) s_tot[2] = tot;
-    __syncthreads();
-    BS(scan_tmp).ExclusiveSum(c_hole, r_hole, tot);
+    BS2(scan_tmp2).ExclusiveSum(ih, r_ih, t_ih);
+    const int r_app = (int)(r_ai & 0xFFFFFFFFll), r_id = (int)(r_ai >> 32);
+    const int r_info = (int)(r_ih & 0xFFFFFFFFll), r_hole = (int)(r_ih >> 32);
+    int tot;
     if (tid == 0) {
-      s_tot[3] = tot;
-      s_hole_base = tot ? (long long)atomicAdd(W.hole_top, (unsigned long long)tot) : 0ll;
+      s_tot[0] = (int)(t_ai & 0xFFFFFFFFll);
+      s_tot[1] = (int)(t_ai >> 32);
+      s_tot[2] = (int)(t_ih & 0xFFFFFFFFll);
+      s_tot[3] = (int)(t_ih >> 32);
+      const int th = s_tot[3];
+      s_hole_base = th ? (long long)atomicAdd(W.hole_top, (unsigned long long)th) : 0ll;
     }
     __syncthreads();
     // 4. apply the winners at their ranks
