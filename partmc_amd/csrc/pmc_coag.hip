@@ -325,9 +325,7 @@ int pmc_gpu_create(pmc_gpu **out, int device, int n_box, int n_spec, const doubl
     B.w.type[0] = WEIGHT_NONE;
     B.w.mag[0] = 1.0;
     B.w.expo[0] = 0.0;
-    B.env.temp = 288.0;
-    B.env.pressure = 1e5;
-    B.env.beta = 1000.0;
+    B.env = make_env(288.0, 1e5, 1000.0);
   }
   HIPCHECK(dmalloc(&h->d_box, n_box));
   HIPCHECK(hipMemcpy(h->d_box, hb.data(), sizeof(BoxDesc) * n_box, hipMemcpyHostToDevice));
@@ -360,7 +358,7 @@ int pmc_gpu_destroy(pmc_gpu *h) {
 int pmc_gpu_set_env(pmc_gpu *h, int box, double temp, double pressure, double additive_coeff) {
   int e = check_box(h, box);
   if (e) return e;
-  Env env{temp, pressure, additive_coeff};
+  Env env = make_env(temp, pressure, additive_coeff);
   HIPCHECK(hipSetDevice(h->device));
   HIPCHECK(hipStreamSynchronize(h->stream));
   if ((e = SET_FIELD(h, box, env, env))) return e;
@@ -933,7 +931,7 @@ int pmc_gpu_eval_kernel(pmc_gpu *h, int kernel_type, int n, const double *vol1, 
   HIPCHECK(dmalloc(&derr, n));
   HIPCHECK(hipMemcpy(d1, vol1, (size_t)n * ns * sizeof(double), hipMemcpyHostToDevice));
   HIPCHECK(hipMemcpy(d2, vol2, (size_t)n * ns * sizeof(double), hipMemcpyHostToDevice));
-  Env env{temp, pressure, beta};
+  Env env = make_env(temp, pressure, beta);
   hipLaunchKernelGGL(k_eval_kernel, dim3((n + 127) / 128), dim3(128), 0, h->stream, h->A, kernel_type,
                      n, d1, d2, env, dk, dk32, derr);
   HIPCHECK(hipGetLastError());

@@ -389,7 +389,23 @@ __device__ inline double max_coag_num_conc_factor(const Weights &w,
 // ------------------------------------------------------------- kernels
 struct Env {
   double temp, pressure, beta;
+  // air viscosity and mean free path at (temp, pressure), for the FP32
+  // Brownian terms (the FP64 kernel derives its own, as the reference does)
+  double viscosd, gasfreepath;
 };
+
+__host__ __device__ inline Env make_env(double temp, double pressure, double beta) {
+  Env e;
+  e.temp = temp;
+  e.pressure = pressure;
+  e.beta = beta;
+  const double rhoair = (pressure * C_AIR_MW) / (C_UNIV_GAS * temp);
+  e.viscosd = 1.8325e-05 * (416.16 / (temp + 120.0)) * pow(temp / 296.16, 1.5);
+  const double viscosk = e.viscosd / rhoair;
+  const double gasspeed = sqrt((8.0 * C_BOLTZMANN * temp * C_AVOGADRO) / (C_PI * C_AIR_MW));
+  e.gasfreepath = 2.0 * viscosk / gasspeed;
+  return e;
+}
 
 // kernel_brown_helper, src/coag_kernel_brown.F90:111-197 (spherical)
 __device__ inline double kernel_brown_helper(double vol_i, double den_i,

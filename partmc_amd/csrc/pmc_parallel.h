@@ -16,19 +16,15 @@ __device__ inline float4 particle_terms(int type, double v, double den, const En
   double r = vol2rad(v);
   t.x = (float)r;
   if (type == KERNEL_BROWN) {
-    double temp = env.temp, pressure = env.pressure;
-    double rhoair = (pressure * C_AIR_MW) / (C_UNIV_GAS * temp);
-    double viscosd = 1.8325e-05 * (416.16 / (temp + 120.0)) * pow(temp / 296.16, 1.5);
-    double viscosk = viscosd / rhoair;
-    double gasspeed = sqrt((8.0 * C_BOLTZMANN * temp * C_AVOGADRO) / (C_PI * C_AIR_MW));
-    double gasfreepath = 2.0 * viscosk / gasspeed;
-    double knud = gasfreepath / r;
+    const double temp = env.temp, viscosd = env.viscosd;
+    double knud = env.gasfreepath / r;
     double cunning = 1.0 + knud * (1.249 + 0.42 * exp(-0.87 / knud));
     double diffus = (C_BOLTZMANN * temp * cunning) / (6.0 * C_PI * r * viscosd);
     double speedsq = 8.0 * C_BOLTZMANN * temp / (C_PI * den * v);
     double freepath = 8.0 * diffus / (C_PI * sqrt(speedsq));
     double tmp1 = powi(2.0 * r + freepath, 3);
-    double tmp2 = pow(4.0 * r * r + freepath * freepath, 1.5);
+    const double q = 4.0 * r * r + freepath * freepath;
+    double tmp2 = q * sqrt(q);  // q^1.5
     double d = (tmp1 - tmp2) / (6.0 * r * freepath) - 2.0 * r;
     t.y = (float)diffus;
     t.z = (float)speedsq;

@@ -1087,7 +1087,7 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
 // claimed with index t when `claim`), 0 when it has none (*tns = its
 // samples + ACC_WARN), -1 on a box error.  *p_out is the target's slot, or
 // -1 when the target is gone.
-__device__ inline int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &W, int64_t k,
+__device__ __noinline__ int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &W, int64_t k,
                                uint32_t t, int rec, int type, uint64_t seed, uint32_t step,
                                double del_t, bool claim, int *p_out, int *tns) {
   *p_out = -1;
@@ -1680,12 +1680,13 @@ __device__ inline int direct_merge(const Arrays &A, const PW &W, BoxDesc *B, int
 // accepted per-set test: removals become holes (or the slot the merged
 // particle takes), the merged particle is written, removal records follow
 // the reference's order.  o_known >= 0 passes the outcome of merge_decide
-// already taken on this test's stream (with the new group in bits 16+).
+// already taken on this test's stream (with the new group in bits 16+),
+// and post, if given, that stream after them.
 template <class Alloc>
 __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, int box, int64_t k,
                                     uint32_t t, int64_t p1, int64_t p2, uint64_t seed,
                                     uint32_t step, int type, int want_terms, int rid, Alloc &al,
-                                    int o_known = -1) {
+                                    int o_known = -1, const Stream *post = nullptr) {
   int c1, c2, b1, b2;
   decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
   const uint32_t j = t - W.pair_exc[k];
@@ -1697,10 +1698,12 @@ __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, in
   int o, new_group;
   Stream s;
   const bool resample = (A.ncomp[p1] + A.ncomp[p2] > MAX_COMP);
-  if (o_known >= 0 && !resample) {
-    // no draws follow the merge decisions: the stream is not needed again
+  if (o_known >= 0 && (!resample || post)) {
+    // the merge decisions are known; a resampling continues the stream
+    // where they left it
     o = o_known & 0xFFFF;
     new_group = o_known >> 16;
+    if (post) s = *post;
   } else {
     s.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), j, RNG_MERGE);
     o = merge_decide(A, W, B, s, p1, p2, c1, c2, cc, &new_group);
@@ -1753,7 +1756,7 @@ __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, in
 // are appended with new identifiers in creation order, and a target or
 // copy that kept no copy of itself is removed with a weighting record.
 template <class Alloc>
-__device__ inline void acc_apply(BoxDesc *B, const Arrays &A, const PW &W, int box, uint32_t t,
+__device__ __noinline__ void acc_apply(BoxDesc *B, const Arrays &A, const PW &W, int box, uint32_t t,
                                  int r, int64_t p, int type, int want_terms, int rid, Alloc &al) {
   const int ns = A.n_spec;
   const int nb = B->n_bin;
@@ -2049,6 +2052,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     // 3. resolve, and what each winner creates
     bool win = false;
     int o_win = -1;
+    Stream ms;  // a per-set winner's merge stream, after its decisions
     int c_app = 0, c_id = 0, c_info = 0, c_hole = 0;
     if (effect) {
       if (is_acc) {
@@ -2069,11 +2073,10 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
           decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
           const int nb = B->n_bin, nc = B->w.n_class;
           const int cc = A.cc[table_index(A, box, nb, nc, b1, b2, c1, c2)];
-          Stream s;
-          s.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), t - W.pair_exc[k],
-                 RNG_MERGE);
+          ms.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), t - W.pair_exc[k],
+                  RNG_MERGE);
           int g;
-          const int o = merge_decide(A, W, B, s, p1, p2, c1, c2, cc, &g);
+          const int o = merge_decide(A, W, B, ms, p1, p2, c1, c2, cc, &g);
           o_win = o | (g << 16);
           c_app = (o & O_APPEND) ? 1 : 0;
           c_id = (o & O_NEWID) ? 1 : 0;
@@ -2113,7 +2116,8 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
       al.s_live = s_live;
       al.n_samp = al.n_coag = al.n_warn = 0;
       if (is_acc) acc_apply(B, A, W, box, t, rec, p1, type, want_terms, rid, al);
-      else perset_merge(B, A, W, box, k, t, p1, p2, seed, step, type, want_terms, rid, al, o_win);
+      else perset_merge(B, A, W, box, k, t, p1, p2, seed, step, type, want_terms, rid, al, o_win,
+                        &ms);
       c_samp += al.n_samp;
       c_coag += al.n_coag;
       c_warn += al.n_warn;
