@@ -41,7 +41,7 @@ constexpr int TILE_ITEMS = 16;
 constexpr int TILE = TILE_THREADS * TILE_ITEMS;
 constexpr uint32_t UNCLAIMED = 0xFFFFFFFFu;
 constexpr uint8_t ST_LIVE = 0, ST_HOLE = 1, ST_NEW = 2;
-constexpr int PAR_ROUNDS = 3;  // device-wide claim rounds before the per-box tail
+constexpr int PAR_ROUNDS = 1;  // device-wide claim rounds before the per-box tail
 static_assert(PAR_ROUNDS >= 1, "the tail runs on the pending list of a round");
 constexpr int MAX_REDRAW = 64;
 
