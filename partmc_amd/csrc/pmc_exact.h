@@ -163,12 +163,22 @@ k_prepare(BoxDesc *boxes, Arrays A, int only_box) {
       return;
     }
     const int nb = B->n_bin;
-    for (int i = t; i < s_n; i += SORT_THREADS)
-      A.fwd_bin[base + i] = bin_find_log(nb, B->log_lo, B->log_hi, vol2rad(A.vtot[base + i]));
+    int outside = 0;
+    for (int i = t; i < s_n; i += SORT_THREADS) {
+      const int b = bin_find_log(nb, B->log_lo, B->log_hi, vol2rad(A.vtot[base + i]));
+      A.fwd_bin[base + i] = b;
+      outside += (b < 1 || b > nb) ? 1 : 0;
+    }
+    __shared__ int s_outside;
+    if (t == 0) s_outside = 0;
+    __syncthreads();
+    if (outside) atomicAdd(&s_outside, outside);
     __syncthreads();
     if (t == 0) {
       int n = s_n;
-      for (int i = n - 1; i >= 0; --i) {
+      // aero_sorted_discard_outside_grid's swap-last pass, needed only when
+      // some particle lies outside the grid
+      for (int i = s_outside ? n - 1 : -1; i >= 0; --i) {
         int b = A.fwd_bin[base + i];
         if (b < 1 || b > nb) {
           if (i < n - 1) copy_particle(A, base + n - 1, base + i);
