@@ -145,6 +145,17 @@ int pmc_gpu_reset_counters(pmc_gpu *h);
 /* Largest |p32 - p64| of the acceptance probability p = k/k_max seen in
  * validation mode, per box (the quantity the recheck band must exceed). */
 int pmc_gpu_get_error_stats(pmc_gpu *h, double *max_p_err);
+/* Size distribution of one box on a log-spaced diameter grid of n_bin bins
+ * over [d_min, d_max] (m): bin_grid_histogram_1d (src/bin_grid.F90:225-248)
+ * of the particle diameters weighted by their number concentrations, as
+ * extract_aero_size does (src/extract_aero_size.F90:132-138).  quantity:
+ * PMC_DIST_NUM (m^-3 per ln D), PMC_DIST_MASS (kg m^-3 per ln D) or
+ * PMC_DIST_VOL (m^3 m^-3 per ln D).  hist receives n_bin values. */
+#define PMC_DIST_NUM 0
+#define PMC_DIST_MASS 1
+#define PMC_DIST_VOL 2
+int pmc_gpu_size_dist(pmc_gpu *h, int box, int quantity, int n_bin, double d_min,
+                      double d_max, double *hist);
 /* Particle-steps per box since the last reset: the sum over steps of the
  * box's particle count at the start of each step. */
 int pmc_gpu_get_part_steps(pmc_gpu *h, int64_t *part_steps);

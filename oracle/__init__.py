@@ -161,6 +161,17 @@ def bin_grid_make_log(n_bin, rmin, rmax):
     return e, c, w
 
 
+def histogram_1d(edges, widths, x, w):
+    """bin_grid_histogram_1d (src/bin_grid.F90:225-248) on a log grid."""
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    wd = np.ascontiguousarray(widths, dtype=np.float64)
+    xs = np.ascontiguousarray(x, dtype=np.float64)
+    ws = np.ascontiguousarray(w, dtype=np.float64)
+    out = np.zeros(len(wd))
+    lib().orc_histogram_1d(len(wd), _ptr(e), _ptr(wd), len(xs), _ptr(xs), _ptr(ws), _ptr(out))
+    return out
+
+
 def kernel(kind, density, vol1, vol2, temp=288.0, pressure=1e5, beta=1000.0):
     dens = np.ascontiguousarray(density, dtype=np.float64)
     v1 = np.ascontiguousarray(vol1, dtype=np.float64)

@@ -41,6 +41,7 @@ ABI = {
     "pmc_gpu_reserve": (_i, [_p, _p]),
     "pmc_gpu_get_part_steps": (_i, [_p, _p]),
     "pmc_gpu_get_path_stats": (_i, [_p, _p, _p, _p]),
+    "pmc_gpu_size_dist": (_i, [_p, _i, _i, _i, _d, _d, _p]),
     "pmc_gpu_get_error_stats": (_i, [_p, _p]),
     "pmc_gpu_set_timing": (_i, [_p, _i]),
     "pmc_gpu_get_kernel_time": (_i, [_p, ctypes.c_char_p, _p, _p]),
@@ -232,6 +233,14 @@ class CoagGPU:
     def part_steps(self):
         out = np.zeros(self.n_box, dtype=np.int64)
         _check(load().pmc_gpu_get_part_steps(self._h, _ptr(out)))
+        return out
+
+    def size_dist(self, box, n_bin, d_min, d_max, quantity="num"):
+        """bin_grid_histogram_1d of the box's diameters weighted by number
+        concentration (times mass or volume), per ln D, on a log grid."""
+        q = {"num": 0, "mass": 1, "vol": 2}[quantity]
+        out = np.zeros(n_bin)
+        _check(load().pmc_gpu_size_dist(self._h, box, q, n_bin, d_min, d_max, _ptr(out)))
         return out
 
     def path_stats(self):

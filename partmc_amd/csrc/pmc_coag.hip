@@ -54,6 +54,7 @@ __constant__ double HALL_ECOLL[315] = {
 
 #include "pmc_exact.h"
 #include "pmc_parallel.h"
+#include "pmc_stats.h"
 
 using namespace pmc;
 
@@ -784,6 +785,72 @@ int pmc_gpu_get_path_stats(pmc_gpu *h, int64_t *n_accel_pairs, int64_t *n_droppe
   rounds[0] = h->work.n_rounds;
   rounds[1] = h->work.n_tail;
   rounds[2] = h->work.n_tail_rounds;
+  return 0;
+}
+
+int pmc_gpu_size_dist(pmc_gpu *h, int box, int quantity, int n_bin, double d_min, double d_max,
+                      double *hist) {
+  int e = check_box(h, box);
+  if (e) return e;
+  if (!hist || n_bin < 1 || !(d_min > 0.0 && d_min < d_max))
+    return fail(ERR_ARG, "size_dist: need n_bin >= 1 and 0 < d_min < d_max");
+  if (quantity != DIST_NUM && quantity != DIST_MASS && quantity != DIST_VOL)
+    return fail(ERR_ARG, "size_dist: unknown quantity %d", quantity);
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  for (int i = 0; i < n_bin; ++i) hist[i] = 0.0;
+  int n = 0;
+  HIPCHECK(hipMemcpy(&n, (char *)(h->d_box + box) + offsetof(BoxDesc, n_part), sizeof(int),
+                     hipMemcpyDeviceToHost));
+  if (n == 0) return 0;
+  // the grid of bin_grid_make (log) over diameters: its end points and width
+  const double log_lo = std::log(d_min), log_hi = std::log(d_max);
+  int *key = nullptr, *key_s = nullptr, *ukey = nullptr, *n_runs = nullptr;
+  double *val = nullptr, *val_s = nullptr, *agg = nullptr, *d_hist = nullptr;
+  void *tmp = nullptr;
+  size_t b_sort = 0, b_red = 0;
+  hipError_t err = hipSuccess;
+  auto step = [&](hipError_t x) {
+    if (err == hipSuccess) err = x;
+  };
+  step(hipMalloc((void **)&key, sizeof(int) * n));
+  step(hipMalloc((void **)&key_s, sizeof(int) * n));
+  step(hipMalloc((void **)&ukey, sizeof(int) * n));
+  step(hipMalloc((void **)&n_runs, sizeof(int)));
+  step(hipMalloc((void **)&val, sizeof(double) * n));
+  step(hipMalloc((void **)&val_s, sizeof(double) * n));
+  step(hipMalloc((void **)&agg, sizeof(double) * n));
+  step(hipMalloc((void **)&d_hist, sizeof(double) * n_bin));
+  if (err == hipSuccess) {
+    int key_bits = 1;
+    while ((1 << key_bits) <= n_bin) key_bits++;
+    step(hipcub::DeviceRadixSort::SortPairs(nullptr, b_sort, key, key_s, val, val_s, n, 0,
+                                            key_bits, h->stream));
+    step(hipcub::DeviceReduce::ReduceByKey(nullptr, b_red, key_s, ukey, val_s, agg, n_runs,
+                                           hipcub::Sum(), n, h->stream));
+    step(hipMalloc(&tmp, std::max(b_sort, b_red)));
+    if (err == hipSuccess) {
+      hipLaunchKernelGGL(k_dist_values, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_box,
+                         h->A, box, quantity, n_bin, log_lo, log_hi, key, val);
+      size_t bs = std::max(b_sort, b_red);
+      step(hipcub::DeviceRadixSort::SortPairs(tmp, bs, key, key_s, val, val_s, n, 0, key_bits,
+                                              h->stream));
+      bs = std::max(b_sort, b_red);
+      step(hipcub::DeviceReduce::ReduceByKey(tmp, bs, key_s, ukey, val_s, agg, n_runs,
+                                             hipcub::Sum(), n, h->stream));
+      step(hipMemsetAsync(d_hist, 0, sizeof(double) * n_bin, h->stream));
+      hipLaunchKernelGGL(k_dist_scatter, dim3((n + 255) / 256), dim3(256), 0, h->stream, ukey, agg,
+                         n_runs, n_bin, d_hist);
+      step(hipGetLastError());
+      step(hipMemcpyAsync(hist, d_hist, sizeof(double) * n_bin, hipMemcpyDeviceToHost,
+                          h->stream));
+      step(hipStreamSynchronize(h->stream));
+    }
+  }
+  void *ptrs[] = {key, key_s, ukey, n_runs, val, val_s, agg, d_hist, tmp};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (err != hipSuccess) return fail(ERR_HIP, "size_dist: %s", hipGetErrorString(err));
   return 0;
 }
 

@@ -352,3 +352,21 @@ def test_out_of_grid_merge_is_an_error():
     with pytest.raises(RuntimeError, match="134572570"):
         for s in range(50):
             ob.mc_coag(P.KERNEL_ADDITIVE, 100.0, s)
+
+
+def test_histogram_1d_matches_numpy_log_histogram():
+    """bin_grid_histogram_1d: half-open bins, the top edge in the last bin,
+    out-of-range values dropped, each weight divided by the bin width in
+    ln x; numpy.histogram over ln x has the same conventions."""
+    rng = np.random.default_rng(11)
+    n_bin, lo, hi = 37, 1e-9, 1e-5
+    e, _, wd = oracle.bin_grid_make_log(n_bin, lo, hi)
+    x = np.exp(rng.uniform(np.log(lo) - 1.0, np.log(hi) + 1.0, size=5000))
+    x[:5] = [lo, hi, e[3], e[20], hi * (1 + 1e-12)]
+    w = rng.uniform(0.1, 3.0, size=x.size)
+    got = oracle.histogram_1d(e, wd, x, w)
+    ref, _ = np.histogram(np.log(x), bins=np.log(e), weights=w)
+    np.testing.assert_allclose(got, ref / wd, rtol=1e-12)
+    # the edge values land in the bin they open; the top edge in the last bin
+    one = oracle.histogram_1d(e, wd, [e[3], hi, lo], [1.0, 1.0, 1.0])
+    assert one[3] > 0 and one[-1] > 0 and one[0] > 0 and np.count_nonzero(one) == 3
