@@ -156,6 +156,26 @@ int pmc_gpu_get_error_stats(pmc_gpu *h, double *max_p_err);
 #define PMC_DIST_VOL 2
 int pmc_gpu_size_dist(pmc_gpu *h, int box, int quantity, int n_bin, double d_min,
                       double d_max, double *hist);
+/* Population control.
+ * pmc_gpu_set_n_part_ideal: aero_state_set_n_part_ideal
+ * (src/aero_state.F90:241-258); the box's ideal count is split evenly over
+ * its (group, class) pairs when pmc_gpu_rebalance runs.
+ * pmc_gpu_rebalance: aero_state_rebalance (src/aero_state.F90:1790-1852) for
+ * every box, as run_part_timestep calls it after coagulation
+ * (src/run_part.F90:982-984), one process: aero_state_double (:1722-1750)
+ * while a (group, class) holds fewer than half its ideal count, then
+ * aero_state_halve (:1755-1786) while it holds more than twice.  Halved
+ * particles are logged as AERO_INFO_HALVED (3) with other id 0.  The
+ * halving uniforms are drawn from the handle's seed and current step.
+ * Doubling beyond a box's particle capacity fails with PMC_ERR_CAPACITY
+ * before any box is changed.
+ * pmc_gpu_get_weights: the box's aero_weight_array (magnitudes change under
+ * population control), as set by pmc_gpu_set_weights; max_n bounds the
+ * n_group * n_class entries written. */
+int pmc_gpu_set_n_part_ideal(pmc_gpu *h, int box, double n_part);
+int pmc_gpu_rebalance(pmc_gpu *h, int allow_doubling, int allow_halving);
+int pmc_gpu_get_weights(pmc_gpu *h, int box, int max_n, int *n_group, int *n_class, int *type,
+                        double *magnitude, double *exponent);
 /* Particle-steps per box since the last reset: the sum over steps of the
  * box's particle count at the start of each step. */
 int pmc_gpu_get_part_steps(pmc_gpu *h, int64_t *part_steps);
@@ -190,6 +210,9 @@ int pmc_gpu_get_kbounds(pmc_gpu *h, int box, double *kmin, double *kmax);
  * since the last upload; returns the count in *n. */
 int pmc_gpu_get_removed(pmc_gpu *h, int box, int max_n, int *n, int64_t *id,
                         int *action, int64_t *other_id);
+/* Empty the removal log of one box, as run_part_timestep does after each
+ * output (aero_info_array_zero, src/aero_info_array.F90:62-73). */
+int pmc_gpu_clear_removed(pmc_gpu *h, int box);
 
 /* Kernel evaluation on the device for n particle pairs (tests and
  * golden vectors): kernel(type, p1, p2) of src/coag_kernel.F90:83-125 in

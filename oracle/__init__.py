@@ -101,6 +101,10 @@ def lib():
             "orc_box_n_accel": (_i64, [_p]),
             "orc_box_regridded": (_i, [_p]),
             "orc_mc_coag": (_i, [_p, _i, _d, _u32, _p, _p]),
+            "orc_box_double": (_i, [_p, _i, _i]),
+            "orc_box_halve": (_i, [_p, _i, _i, _u32, _u32]),
+            "orc_box_rebalance": (_i, [_p, _i, _i, _p, _u32]),
+            "orc_box_get_weights": (None, [_p, _p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -348,6 +352,28 @@ class Box:
     @property
     def regridded(self):
         return bool(lib().orc_box_regridded(self._h))
+
+    def rebalance(self, n_part_ideal, step, allow_doubling=True, allow_halving=True):
+        """aero_state_rebalance; n_part_ideal is a scalar split evenly over
+        the (group, class) pairs, as aero_state_set_n_part_ideal does."""
+        ng, nc = self.weights.n_group, self.weights.n_class
+        ideal = np.full(ng * nc, float(n_part_ideal) / (ng * nc))
+        err = lib().orc_box_rebalance(self._h, int(allow_doubling), int(allow_halving),
+                                      _ptr(ideal), step)
+        if err:
+            raise RuntimeError(f"oracle rebalance error {err}")
+
+    def double(self, g=1, c=1):
+        lib().orc_box_double(self._h, g, c)
+
+    def halve(self, step, site, g=1, c=1):
+        lib().orc_box_halve(self._h, g, c, step, site)
+
+    def weight_magnitudes(self):
+        ng, nc = self.weights.n_group, self.weights.n_class
+        m = np.zeros(ng * nc)
+        lib().orc_box_get_weights(self._h, _ptr(m))
+        return m.reshape(nc, ng).T.copy()
 
     def mc_coag(self, kernel_type, del_t, step):
         ns = np.zeros(1, dtype=np.int64)

@@ -1797,6 +1797,100 @@ int orc_mc_coag(orc_box *b, int type, double del_t, uint32_t step,
 }
 
 /* ------------------------------------------------------------------ */
+/* population control: aero_state_rebalance and its doubling/halving   */
+/* ------------------------------------------------------------------ */
+/* aero_state_total_particles for one (group, class),
+ * src/aero_state.F90:287-310 */
+static int count_group_class(const orc_box *b, int g, int c) {
+    int n = 0;
+    for (int i = 0; i < b->n; ++i)
+        if (b->group[i] == g && b->klass[i] == c) n++;
+    return n;
+}
+
+/* aero_state_double, src/aero_state.F90:1722-1750: every particle of
+ * (g, c) present when the pass starts is appended again with a new id, the
+ * sort is invalidated and the weight magnitude halved (aero_weight_scale,
+ * src/aero_weight.F90:60-69). */
+int orc_box_double(orc_box *b, int g, int c) {
+    int n = b->n;
+    for (int i = 0; i < n; ++i) {
+        if (b->group[i] != g || b->klass[i] != c) continue;
+        prec p;
+        load_particle(b, i, &p);
+        p.id = new_id(b);
+        ensure_cap(b, b->n + 1);
+        store_particle(b, b->n, &p);
+        b->n++;
+    }
+    b->valid_sort = 0;
+    b->wmag[(g - 1) + b->n_group * (c - 1)] *= 0.5;
+    return 0;
+}
+
+/* aero_state_halve, src/aero_state.F90:1755-1786.  Particles are visited
+ * from the last to the first; a particle of (g, c) is removed when its
+ * uniform is below 1/2, recorded as AERO_INFO_HALVED (other_id 0), with the
+ * swap-last removal of aero_state_remove_particle (through the sorted lists
+ * when the sort is valid).  The uniform of the particle at 1-based index i
+ * is the first of stream (step, site, ordinal i, ORC_RNG_HALVE). */
+int orc_box_halve(orc_box *b, int g, int c, uint32_t step, uint32_t site) {
+    for (int i = b->n - 1; i >= 0; --i) {
+        if (b->group[i] != g || b->klass[i] != c) continue;
+        orc_stream s;
+        orc_stream_open(&s, b->seed, b->box_index, step, site, (uint32_t)(i + 1),
+                        ORC_RNG_HALVE);
+        if (orc_stream_uniform(&s) < 0.5) {
+            int64_t id = b->id[i];
+            if (b->valid_sort) {
+                state_remove(b, i, 1, id, ORC_INFO_HALVED, 0);
+            } else {
+                apa_remove(b, i);
+                add_info(b, id, ORC_INFO_HALVED, 0);
+            }
+        }
+    }
+    b->wmag[(g - 1) + b->n_group * (c - 1)] *= 2.0;
+    return 0;
+}
+
+/* aero_state_rebalance, src/aero_state.F90:1790-1852, one process: double
+ * each (g, c) while it holds fewer than half its ideal count (and some
+ * particles), then halve each while it holds more than twice.  ideal is
+ * n_part_ideal(g, c) at ideal[(g-1) + n_group*(c-1)].  Halving pass k of
+ * (g, c) draws from site (g-1) + n_group*((c-1) + n_class*k). */
+int orc_box_rebalance(orc_box *b, int allow_doubling, int allow_halving,
+                      const double *ideal, uint32_t step) {
+    if (allow_doubling)
+        for (int g = 1; g <= b->n_group; ++g)
+            for (int c = 1; c <= b->n_class; ++c) {
+                double id = ideal[(g - 1) + b->n_group * (c - 1)];
+                int n = count_group_class(b, g, c);
+                while ((double)n < id / 2.0 && n > 0) {
+                    orc_box_double(b, g, c);
+                    n = count_group_class(b, g, c);
+                }
+            }
+    if (allow_halving)
+        for (int g = 1; g <= b->n_group; ++g)
+            for (int c = 1; c <= b->n_class; ++c) {
+                double id = ideal[(g - 1) + b->n_group * (c - 1)];
+                uint32_t k = 0;
+                while ((double)count_group_class(b, g, c) > id * 2.0) {
+                    uint32_t site = (uint32_t)(g - 1) +
+                        (uint32_t)b->n_group * ((uint32_t)(c - 1) + (uint32_t)b->n_class * k);
+                    orc_box_halve(b, g, c, step, site);
+                    k++;
+                }
+            }
+    return 0;
+}
+
+void orc_box_get_weights(const orc_box *b, double *wmag) {
+    memcpy(wmag, b->wmag, sizeof(double) * (size_t)(b->n_group * b->n_class));
+}
+
+/* ------------------------------------------------------------------ */
 /* batch samplers for the pmf tests (test/rand protocol)               */
 /* ------------------------------------------------------------------ */
 void orc_sample_pmf(uint64_t seed, int which, double p1, double p2, int64_t n,

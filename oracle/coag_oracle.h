@@ -45,6 +45,7 @@ extern "C" {
 
 /* removal actions, src/aero_info.F90:19-29 */
 #define ORC_INFO_COAG 2
+#define ORC_INFO_HALVED 3
 #define ORC_INFO_WEIGHT 4
 
 /* RNG stream kinds (the call sites of mc_coag) */
@@ -55,6 +56,7 @@ extern "C" {
 #define ORC_RNG_ACC_SAMPLE 5   /* sample_source_particle: per-sample draws */
 #define ORC_RNG_ACC_FINISH 6   /* binomial, prob_round, resampling         */
 #define ORC_RNG_ACC_TARGET 7   /* coag_target_with_source: merge, groups, dups */
+#define ORC_RNG_HALVE 8        /* aero_state_halve: one uniform per particle */
 
 /* ---- Philox4x32-10 and the per-site uniform stream ---- */
 typedef struct {
@@ -182,6 +184,13 @@ int orc_box_regridded(const orc_box *b);
 /* mc_coag, src/coagulation.F90:43-96.  Returns 0 or a reference error code. */
 int orc_mc_coag(orc_box *b, int kernel_type, double del_t, uint32_t step,
                 int64_t *tot_n_samp, int64_t *tot_n_coag);
+
+/* population control (src/aero_state.F90:1722-1852) */
+int orc_box_double(orc_box *b, int g, int c);
+int orc_box_halve(orc_box *b, int g, int c, uint32_t step, uint32_t site);
+int orc_box_rebalance(orc_box *b, int allow_doubling, int allow_halving,
+                      const double *ideal, uint32_t step);
+void orc_box_get_weights(const orc_box *b, double *wmag);
 
 #ifdef __cplusplus
 }

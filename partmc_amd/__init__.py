@@ -42,6 +42,10 @@ ABI = {
     "pmc_gpu_get_part_steps": (_i, [_p, _p]),
     "pmc_gpu_get_path_stats": (_i, [_p, _p, _p, _p]),
     "pmc_gpu_size_dist": (_i, [_p, _i, _i, _i, _d, _d, _p]),
+    "pmc_gpu_set_n_part_ideal": (_i, [_p, _i, _d]),
+    "pmc_gpu_clear_removed": (_i, [_p, _i]),
+    "pmc_gpu_rebalance": (_i, [_p, _i, _i]),
+    "pmc_gpu_get_weights": (_i, [_p, _i, _i, _p, _p, _p, _p, _p]),
     "pmc_gpu_get_error_stats": (_i, [_p, _p]),
     "pmc_gpu_set_timing": (_i, [_p, _i]),
     "pmc_gpu_get_kernel_time": (_i, [_p, ctypes.c_char_p, _p, _p]),
@@ -234,6 +238,30 @@ class CoagGPU:
         out = np.zeros(self.n_box, dtype=np.int64)
         _check(load().pmc_gpu_get_part_steps(self._h, _ptr(out)))
         return out
+
+    def clear_removed(self, box):
+        _check(load().pmc_gpu_clear_removed(self._h, box))
+
+    def set_n_part_ideal(self, box, n_part):
+        _check(load().pmc_gpu_set_n_part_ideal(self._h, box, float(n_part)))
+
+    def rebalance(self, allow_doubling=True, allow_halving=True):
+        """aero_state_rebalance on every box."""
+        _check(load().pmc_gpu_rebalance(self._h, int(allow_doubling), int(allow_halving)))
+
+    def weights(self, box):
+        """(type, magnitude, exponent), each of shape (n_group, n_class)."""
+        n = 64
+        ng = np.zeros(1, dtype=np.int32)
+        nc = np.zeros(1, dtype=np.int32)
+        t = np.zeros(n, dtype=np.int32)
+        m = np.zeros(n)
+        e = np.zeros(n)
+        _check(load().pmc_gpu_get_weights(self._h, box, n, _ptr(ng), _ptr(nc), _ptr(t), _ptr(m),
+                                          _ptr(e)))
+        g, c = int(ng[0]), int(nc[0])
+        shape = lambda a: a[: g * c].reshape(c, g).T.copy()
+        return shape(t), shape(m), shape(e)
 
     def size_dist(self, box, n_bin, d_min, d_max, quantity="num"):
         """bin_grid_histogram_1d of the box's diameters weighted by number

@@ -55,6 +55,7 @@ __constant__ double HALL_ECOLL[315] = {
 #include "pmc_exact.h"
 #include "pmc_parallel.h"
 #include "pmc_stats.h"
+#include "pmc_rebalance.h"
 
 using namespace pmc;
 
@@ -95,6 +96,9 @@ struct pmc_gpu {
   uint64_t seed = 0;
   uint32_t step = 0;
   ParallelWork work{};
+  // population control: n_part_ideal per box (< 0: not set), counts, plans
+  std::vector<double> n_part_ideal;
+  int *d_gc = nullptr, *d_plan = nullptr;
   // per-kernel HIP-event timing
   int timing = 0;
   struct Timed {
@@ -182,7 +186,8 @@ static int alloc_tables(pmc_gpu *h, int classes) {
 }
 
 // Lay out particle storage for the given per-box capacities, keeping the
-// particles already uploaded.
+// particles already uploaded, their bin lists (the list order is state the
+// next step depends on) and the removal logs.
 static int layout(pmc_gpu *h, const std::vector<int64_t> &new_cap) {
   std::vector<BoxDesc> hb(h->n_box);
   HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
@@ -247,13 +252,37 @@ static int layout(pmc_gpu *h, const std::vector<int64_t> &new_cap) {
       MOVE(fwd_idx, int);
 #undef MOVE
     }
+    if (old.total_cap > 0) {
+      // the live half of the list pool, with list starts shifted to match
+      const int64_t o = hb[b].pool_off + (int64_t)hb[b].pool_cur * hb[b].pool_half;
+      const int64_t d = poff[b] + (int64_t)hb[b].pool_cur * phalf[b];
+      if (hb[b].pool_top > 0)
+        HIPCHECK(hipMemcpy(A.pool + d, old.pool + o, hb[b].pool_top * sizeof(int),
+                           hipMemcpyDeviceToDevice));
+      std::vector<int> ls(A.max_keys);
+      int *lsd = A.lstart + (int64_t)b * A.max_keys;
+      HIPCHECK(hipMemcpy(ls.data(), lsd, sizeof(int) * A.max_keys, hipMemcpyDeviceToHost));
+      for (int &v : ls) v = (int)(v - o + d);
+      HIPCHECK(hipMemcpy(lsd, ls.data(), sizeof(int) * A.max_keys, hipMemcpyHostToDevice));
+      // the removal log
+      const int64_t ni = std::min<int64_t>(hb[b].n_info, hb[b].info_cap);
+      if (ni > 0) {
+        const int64_t io = hb[b].info_off, id_ = ioff[b];
+        HIPCHECK(hipMemcpy(A.info_id + id_, old.info_id + io, ni * sizeof(int64_t), hipMemcpyDeviceToDevice));
+        HIPCHECK(hipMemcpy(A.info_other + id_, old.info_other + io, ni * sizeof(int64_t), hipMemcpyDeviceToDevice));
+        HIPCHECK(hipMemcpy(A.info_action + id_, old.info_action + io, ni * sizeof(int), hipMemcpyDeviceToDevice));
+      }
+    }
     hb[b].off = off[b];
     hb[b].cap = new_cap[b];
     hb[b].pool_off = poff[b];
     hb[b].pool_half = phalf[b];
     hb[b].info_off = ioff[b];
     hb[b].info_cap = new_cap[b] + 64;
-    hb[b].valid_sort = 0;  // lists are rebuilt after a re-layout
+    // parallel-mode keys and terms live in per-particle work arrays that are
+    // laid out again by the next parallel step
+    hb[b].pkeys_valid = 0;
+    hb[b].terms_kernel = 0;
   }
   h->A = old;
   free_particles(h);
@@ -347,7 +376,7 @@ int pmc_gpu_destroy(pmc_gpu *h) {
   free_particles(h);
   Arrays &A = h->A;
   void *ptrs[] = {A.density, A.edges, A.centers, A.widths, A.kmin, A.kmax, A.kw, A.cc,
-                  A.lstart,  A.llen,  A.lcap,    A.sort_scratch, h->d_box};
+                  A.lstart,  A.llen,  A.lcap,    A.sort_scratch, h->d_box, h->d_gc, h->d_plan};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   parallel_free(h->work);
@@ -854,6 +883,142 @@ int pmc_gpu_size_dist(pmc_gpu *h, int box, int quantity, int n_bin, double d_min
   return 0;
 }
 
+int pmc_gpu_set_n_part_ideal(pmc_gpu *h, int box, double n_part) {
+  int e = check_box(h, box);
+  if (e) return e;
+  if (!(n_part >= 0.0)) return fail(ERR_ARG, "set_n_part_ideal: n_part must be >= 0");
+  if ((int)h->n_part_ideal.size() != h->n_box) h->n_part_ideal.assign(h->n_box, -1.0);
+  h->n_part_ideal[box] = n_part;
+  return 0;
+}
+
+int pmc_gpu_get_weights(pmc_gpu *h, int box, int max_n, int *n_group, int *n_class,
+                        int *type, double *magnitude, double *exponent) {
+  int e = check_box(h, box);
+  if (e) return e;
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  Weights w;
+  HIPCHECK(hipMemcpy(&w, (char *)(h->d_box + box) + offsetof(BoxDesc, w), sizeof(Weights),
+                     hipMemcpyDeviceToHost));
+  const int n = w.n_group * w.n_class;
+  if (n_group) *n_group = w.n_group;
+  if (n_class) *n_class = w.n_class;
+  if (n > max_n) return fail(ERR_ARG, "get_weights: %d weights, room for %d", n, max_n);
+  for (int i = 0; i < n; ++i) {
+    if (type) type[i] = w.type[i];
+    if (magnitude) magnitude[i] = w.mag[i];
+    if (exponent) exponent[i] = w.expo[i];
+  }
+  return 0;
+}
+
+// aero_state_rebalance (src/aero_state.F90:1790-1852) for every box at once.
+// The counts of each (group, class) decide the passes: doublings follow
+// from the counts alone, so capacity is checked before any pass runs;
+// halving passes repeat, with fresh counts, while a class holds more than
+// twice its ideal number.
+int pmc_gpu_rebalance(pmc_gpu *h, int allow_doubling, int allow_halving) {
+  if (!h) return fail(ERR_ARG, "null handle");
+  if ((int)h->n_part_ideal.size() != h->n_box)
+    return fail(ERR_STATE, "rebalance: set_n_part_ideal first");
+  for (int b = 0; b < h->n_box; ++b)
+    if (h->n_part_ideal[b] < 0.0) return fail(ERR_STATE, "rebalance: box %d has no n_part_ideal", b);
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  const int nb = h->n_box, stride = GC_KEYS + 1;
+  if (!h->d_gc) HIPCHECK(hipMalloc((void **)&h->d_gc, sizeof(int) * (size_t)nb * stride));
+  if (!h->d_plan) HIPCHECK(hipMalloc((void **)&h->d_plan, sizeof(int) * (size_t)nb));
+  std::vector<BoxDesc> hb(nb);
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * nb, hipMemcpyDeviceToHost));
+  int e = check_box_errors(h, hb);
+  if (e) return e;
+  std::vector<int> gc((size_t)nb * stride), plan(nb);
+  auto count = [&]() -> hipError_t {
+    hipLaunchKernelGGL(k_count_gc, dim3(nb), dim3(REBAL_THREADS), 0, h->stream, h->d_box, h->A,
+                       h->d_gc);
+    hipError_t x = hipGetLastError();
+    if (x != hipSuccess) return x;
+    x = hipMemcpyAsync(gc.data(), h->d_gc, sizeof(int) * gc.size(), hipMemcpyDeviceToHost,
+                       h->stream);
+    if (x != hipSuccess) return x;
+    return hipStreamSynchronize(h->stream);
+  };
+  HIPCHECK(count());
+  int ng_max = 1, nc_max = 1;
+  for (auto &B : hb) {
+    ng_max = std::max(ng_max, B.w.n_group);
+    nc_max = std::max(nc_max, B.w.n_class);
+  }
+  auto ideal = [&](int b) {
+    return h->n_part_ideal[b] / (double)(hb[b].w.n_group * hb[b].w.n_class);
+  };
+  auto has = [&](int b, int g, int c) { return g <= hb[b].w.n_group && c <= hb[b].w.n_class; };
+  auto key = [&](int b, int g, int c) { return (size_t)b * stride + (g - 1) + hb[b].w.n_group * (c - 1); };
+
+  if (allow_doubling) {
+    // doublings per (box, g, c): while count < ideal / 2 and count > 0
+    std::vector<int> nd((size_t)nb * ng_max * nc_max, 0);
+    for (int b = 0; b < nb; ++b) {
+      int64_t n_tot = gc[(size_t)b * stride + GC_KEYS];
+      for (int g = 1; g <= ng_max; ++g)
+        for (int c = 1; c <= nc_max; ++c) {
+          if (!has(b, g, c)) continue;
+          const int64_t n0 = gc[key(b, g, c)];
+          int64_t n = n0;
+          int d = 0;
+          while ((double)n < ideal(b) / 2.0 && n > 0) {
+            n *= 2;
+            d++;
+          }
+          n_tot += n - n0;
+          if (n_tot > hb[b].cap)
+            return fail(ERR_CAPACITY, "rebalance: box %d needs %lld particle slots to double, capacity %lld",
+                        b, (long long)n_tot, (long long)hb[b].cap);
+          nd[((size_t)b * ng_max + (g - 1)) * nc_max + (c - 1)] = d;
+        }
+    }
+    for (int g = 1; g <= ng_max; ++g)
+      for (int c = 1; c <= nc_max; ++c) {
+        int d_max = 0;
+        for (int b = 0; b < nb; ++b) {
+          plan[b] = nd[((size_t)b * ng_max + (g - 1)) * nc_max + (c - 1)];
+          d_max = std::max(d_max, plan[b]);
+        }
+        if (!d_max) continue;
+        HIPCHECK(hipMemcpyAsync(h->d_plan, plan.data(), sizeof(int) * nb, hipMemcpyHostToDevice,
+                                h->stream));
+        for (int p = 0; p < d_max; ++p)
+          hipLaunchKernelGGL(k_double_pass, dim3(nb), dim3(REBAL_THREADS), 0, h->stream, h->d_box,
+                             h->A, h->d_plan, g, c, p);
+        HIPCHECK(hipGetLastError());
+      }
+    HIPCHECK(count());
+  }
+  if (allow_halving) {
+    for (int g = 1; g <= ng_max; ++g)
+      for (int c = 1; c <= nc_max; ++c)
+        for (int k = 0;; ++k) {
+          int any = 0;
+          for (int b = 0; b < nb; ++b) {
+            plan[b] = (has(b, g, c) && (double)gc[key(b, g, c)] > ideal(b) * 2.0) ? 1 : 0;
+            any |= plan[b];
+          }
+          if (!any) break;
+          if (k >= 64) return fail(ERR_STATE, "rebalance: class (%d, %d) still above twice its ideal after 64 halvings", g, c);
+          HIPCHECK(hipMemcpyAsync(h->d_plan, plan.data(), sizeof(int) * nb, hipMemcpyHostToDevice,
+                                  h->stream));
+          hipLaunchKernelGGL(k_halve_pass, dim3((nb + 63) / 64), dim3(64), 0, h->stream, h->d_box,
+                             h->A, nb, h->d_plan, g, c, k, h->seed, h->step);
+          HIPCHECK(hipGetLastError());
+          HIPCHECK(count());
+        }
+  }
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * nb, hipMemcpyDeviceToHost));
+  return check_box_errors(h, hb);
+}
+
 int pmc_gpu_get_part_steps(pmc_gpu *h, int64_t *part_steps) {
   if (!h || !part_steps) return fail(ERR_ARG, "get_part_steps: bad arguments");
   HIPCHECK(hipSetDevice(h->device));
@@ -978,6 +1143,15 @@ int pmc_gpu_get_removed(pmc_gpu *h, int box, int max_n, int *n, int64_t *id, int
   if (action) HIPCHECK(hipMemcpy(action, h->A.info_action + hb.info_off, cnt * sizeof(int), hipMemcpyDeviceToHost));
   if (other_id) HIPCHECK(hipMemcpy(other_id, h->A.info_other + hb.info_off, cnt * sizeof(int64_t), hipMemcpyDeviceToHost));
   return 0;
+}
+
+int pmc_gpu_clear_removed(pmc_gpu *h, int box) {
+  int e = check_box(h, box);
+  if (e) return e;
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  int64_t zero = 0;
+  return SET_FIELD(h, box, n_info, zero);
 }
 
 int pmc_gpu_eval_kernel(pmc_gpu *h, int kernel_type, int n, const double *vol1, const double *vol2,

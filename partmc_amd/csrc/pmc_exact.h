@@ -220,6 +220,19 @@ k_prepare(BoxDesc *boxes, Arrays A, int only_box) {
     if (t == 0) {
       B->regridded = 0;
       const int nb = B->n_bin;
+      if (!use_lists && B->valid_sort && nb > 0) {
+        // a valid sort without current lists: the occupied bins are those
+        // of the radius extremes
+        if (s_n == 0) {
+          rmin = edges[nb];
+          rmax = edges[0];
+        } else {
+          const int b_lo = bin_find_log(nb, B->log_lo, B->log_hi, rmin);
+          const int b_hi = bin_find_log(nb, B->log_lo, B->log_hi, rmax);
+          rmin = edges[max(b_lo, 1) - 1];
+          rmax = edges[min(b_hi, nb)];
+        }
+      }
       if (use_lists) {
         int i_min = 0, i_max = 0;
         for (int b = 1; b <= nb; ++b) {

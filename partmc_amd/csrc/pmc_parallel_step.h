@@ -296,6 +296,19 @@ __global__ void k_par_remake(BoxDesc *boxes, Arrays A, PW W, int n_box) {
       rmin = edges[i_min - 1];
       rmax = edges[i_max];
     }
+  } else if (B->valid_sort && nb > 0) {
+    // a valid sort whose keys were dropped (population control moved
+    // particles): the occupied bins are those of the radius extremes
+    rmin = edges[nb];
+    rmax = edges[0];
+    if (B->n_part > 0) {
+      const int b_lo = bin_find_log(nb, B->log_lo, B->log_hi,
+                                    __longlong_as_double((long long)B->rmin_bits));
+      const int b_hi = bin_find_log(nb, B->log_lo, B->log_hi,
+                                    __longlong_as_double((long long)B->rmax_bits));
+      rmin = edges[max(b_lo, 1) - 1];
+      rmax = edges[min(b_hi, nb)];
+    }
   } else if (B->n_part > 0) {
     rmin = __longlong_as_double((long long)B->rmin_bits);
     rmax = __longlong_as_double((long long)B->rmax_bits);

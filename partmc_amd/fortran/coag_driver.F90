@@ -3,10 +3,13 @@
 ! run_part_timestep calls mc_coag (src/run_part.F90:919-920).
 !
 ! usage: coag_driver DIR N_SPEC N_PART N_STEPS KERNEL DT SEED TEMP PRES BETA
-!                    WMAG N_BIN R_MIN R_MAX MODE
+!                    WMAG N_BIN R_MIN R_MAX MODE [N_PART_IDEAL]
 ! Reads DIR/dens.bin (n_spec doubles), DIR/vol.bin (vol(n_spec, n_part)),
 ! DIR/ids.bin (n_part int64); writes DIR/counters.txt (step, tot_n_samp,
 ! tot_n_coag per line), DIR/out_n.txt, DIR/out_vol.bin and DIR/out_ids.bin.
+! With N_PART_IDEAL > 0 each step is followed by aero_state_rebalance with
+! doubling and halving allowed (src/run_part.F90:982-984), and the final
+! weight magnitude is written to DIR/out_wmag.txt.
 program coag_driver
 
   use iso_c_binding
@@ -17,11 +20,11 @@ program coag_driver
   integer :: n_spec, n_part, n_steps, kernel, mode, n_bin, step, u, i
   integer :: tot_n_samp, tot_n_coag, n_out
   integer(c_int64_t) :: seed, next_id
-  real(c_double) :: dt, temp, pres, beta, wmag, r_min, r_max
+  real(c_double) :: dt, temp, pres, beta, wmag, r_min, r_max, n_ideal
   real(c_double), allocatable :: dens(:), vol(:, :), vol_out(:, :)
   integer(c_int64_t), allocatable :: ids(:), ids_out(:)
   integer(c_int), allocatable :: group(:), klass(:), nprim(:)
-  integer(c_int) :: wtype(1), n_got
+  integer(c_int) :: wtype(1), n_got, ng, ncl
   real(c_double) :: wm(1), we(1)
   type(c_ptr) :: h
 
@@ -41,6 +44,8 @@ program coag_driver
   call real_arg(13, r_min)
   call real_arg(14, r_max)
   call int_arg(15, mode)
+  n_ideal = 0d0
+  if (command_argument_count() >= 16) call real_arg(16, n_ideal)
 
   allocate(dens(n_spec), vol(n_spec, n_part), ids(n_part))
   allocate(group(n_part), klass(n_part), nprim(n_part))
@@ -77,10 +82,12 @@ program coag_driver
           r_min, r_max))
   end if
   call coag_gpu_check(pmc_gpu_set_mode(h, int(mode, c_int), PMC_PREC_FP64, 1d-5))
+  if (n_ideal > 0d0) call coag_gpu_check(pmc_gpu_set_n_part_ideal(h, 0_c_int, n_ideal))
 
   open(newunit=u, file=trim(dir)//'/counters.txt', form='formatted')
   do step = 1, n_steps
      call coag_gpu_mc_coag(h, kernel, dt, tot_n_samp, tot_n_coag)
+     if (n_ideal > 0d0) call coag_gpu_check(pmc_gpu_rebalance(h, 1_c_int, 1_c_int))
      write(u, '(i0,1x,i0,1x,i0)') step - 1, tot_n_samp, tot_n_coag
   end do
   close(u)
@@ -104,6 +111,13 @@ program coag_driver
        form='unformatted', status='replace')
   write(u) ids_out(1:n_got)
   close(u)
+  if (n_ideal > 0d0) then
+     call coag_gpu_check(pmc_gpu_get_weights(h, 0_c_int, 1_c_int, ng, ncl, &
+          wtype, wm, we))
+     open(newunit=u, file=trim(dir)//'/out_wmag.txt', form='formatted')
+     write(u, '(es25.17)') wm(1)
+     close(u)
+  end if
   call coag_gpu_check(pmc_gpu_destroy(h))
 
 contains

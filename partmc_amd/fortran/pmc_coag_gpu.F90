@@ -133,6 +133,46 @@ module pmc_coag_gpu
        real(c_double), value :: del_t
        integer(c_int64_t), intent(out) :: n_samp(*), n_coag(*)
      end function pmc_gpu_mc_coag
+
+     integer(c_int) function pmc_gpu_set_n_part_ideal(h, box, n_part) &
+          bind(c, name="pmc_gpu_set_n_part_ideal")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int), value :: box
+       real(c_double), value :: n_part
+     end function pmc_gpu_set_n_part_ideal
+
+     integer(c_int) function pmc_gpu_rebalance(h, allow_doubling, allow_halving) &
+          bind(c, name="pmc_gpu_rebalance")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: allow_doubling, allow_halving
+     end function pmc_gpu_rebalance
+
+     integer(c_int) function pmc_gpu_get_weights(h, box, max_n, n_group, &
+          n_class, type, magnitude, exponent) bind(c, name="pmc_gpu_get_weights")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int), value :: box, max_n
+       integer(c_int), intent(out) :: n_group, n_class, type(*)
+       real(c_double), intent(out) :: magnitude(*), exponent(*)
+     end function pmc_gpu_get_weights
+
+     integer(c_int) function pmc_gpu_size_dist(h, box, quantity, n_bin, d_min, &
+          d_max, hist) bind(c, name="pmc_gpu_size_dist")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int), value :: box, quantity, n_bin
+       real(c_double), value :: d_min, d_max
+       real(c_double), intent(out) :: hist(*)
+     end function pmc_gpu_size_dist
+
+     integer(c_int) function pmc_gpu_clear_removed(h, box) &
+          bind(c, name="pmc_gpu_clear_removed")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int), value :: box
+     end function pmc_gpu_clear_removed
   end interface
 
 contains

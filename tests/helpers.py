@@ -41,3 +41,28 @@ def small(pop, n, seed=1):
 def radius(vol):
     v = np.asarray(vol).sum(axis=1)
     return (3.0 * v / 4.0 / np.pi) ** (1.0 / 3.0)
+
+
+def compare_state(ob, g, n_class=1, box=0, lists=True):
+    """The GPU box equals the oracle's: particles, lists (when the sort is
+    valid), next id, removal log."""
+    o = ob.particles()
+    d = g.download(box)
+    assert len(o["id"]) == len(d["id"])
+    np.testing.assert_array_equal(o["id"], d["id"])
+    np.testing.assert_array_equal(o["group"], d["group"])
+    np.testing.assert_array_equal(o["klass"], d["klass"])
+    np.testing.assert_array_equal(o["n_primary"], d["n_primary"])
+    np.testing.assert_array_equal(o["n_comp"], d["n_comp"])
+    np.testing.assert_array_equal(o["comp_source"], d["comp_source"])
+    np.testing.assert_allclose(d["vol"], o["vol"], rtol=1e-12, atol=0)
+    if lists:
+        oc, oe = ob.lists()
+        gc, ge = g.lists(box, n_class)
+        np.testing.assert_array_equal(oc, gc)
+        np.testing.assert_array_equal(oe, ge)
+    assert g.next_id(box) == ob.next_id
+    oi = ob.info()
+    gi = g.removed(box)
+    for a, b in zip(oi, gi):
+        np.testing.assert_array_equal(a, b)

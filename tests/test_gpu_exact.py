@@ -9,7 +9,7 @@ since merges only add stored doubles).
 import numpy as np
 import pytest
 
-from helpers import gpu_boxes, oracle_box, small
+from helpers import compare_state, gpu_boxes, oracle_box, small
 from partmc_amd import populations as P
 
 pytestmark = pytest.mark.gpu
@@ -23,28 +23,6 @@ def run_pair(pop, kernel, dt, n_steps, seed):
         gs, gc = g.mc_coag(kernel, dt)
         assert (os_, oc) == (int(gs[0]), int(gc[0])), f"step {step}: counters differ"
     return ob, g
-
-
-def compare_state(ob, g, n_class=1):
-    o = ob.particles()
-    d = g.download(0)
-    assert len(o["id"]) == len(d["id"])
-    np.testing.assert_array_equal(o["id"], d["id"])
-    np.testing.assert_array_equal(o["group"], d["group"])
-    np.testing.assert_array_equal(o["klass"], d["klass"])
-    np.testing.assert_array_equal(o["n_primary"], d["n_primary"])
-    np.testing.assert_array_equal(o["n_comp"], d["n_comp"])
-    np.testing.assert_array_equal(o["comp_source"], d["comp_source"])
-    np.testing.assert_allclose(d["vol"], o["vol"], rtol=1e-12, atol=0)
-    oc, oe = ob.lists()
-    gc, ge = g.lists(0, n_class)
-    np.testing.assert_array_equal(oc, gc)
-    np.testing.assert_array_equal(oe, ge)
-    assert g.next_id(0) == ob.next_id
-    oi = ob.info()
-    gi = g.removed(0)
-    for a, b in zip(oi, gi):
-        np.testing.assert_array_equal(a, b)
 
 
 def test_grid_and_bins_match_oracle():

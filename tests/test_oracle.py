@@ -370,3 +370,127 @@ def test_histogram_1d_matches_numpy_log_histogram():
     # the edge values land in the bin they open; the top edge in the last bin
     one = oracle.histogram_1d(e, wd, [e[3], hi, lo], [1.0, 1.0, 1.0])
     assert one[3] > 0 and one[-1] > 0 and one[0] > 0 and np.count_nonzero(one) == 3
+
+
+# ------------------------------------------------------ population control
+def swap_last_order(n, removed):
+    """Final order of aero_state_halve's removals (visit n-1 .. 0, move the
+    last particle into each hole), restated as a deque: visiting index i,
+    a kept particle is pushed at the front; a removal while the kept tail is
+    non-empty moves the tail's last particle to its front."""
+    tail = []
+    for i in range(n - 1, -1, -1):
+        if i in removed:
+            if tail:
+                tail.insert(0, tail.pop())
+        else:
+            tail.insert(0, i)
+    return tail
+
+
+def lists_consistent(ob):
+    d = ob.particles()
+    counts, entries = ob.lists()
+    e, _, _ = ob.grid()
+    b = np.searchsorted(e, radius(d["vol"]), side="right")
+    np.testing.assert_array_equal(counts.sum(axis=0), np.bincount(b - 1, minlength=len(e) - 1))
+    assert sorted(entries.tolist()) == list(range(len(d["id"])))
+    pos = 0
+    for c in range(counts.shape[0]):
+        for k in range(counts.shape[1]):
+            sel = entries[pos:pos + counts[c, k]]
+            assert np.all(b[sel] == k + 1) and np.all(d["klass"][sel] == c + 1)
+            pos += counts[c, k]
+
+
+def two_class_box(n, seed, sort=True):
+    rng = np.random.default_rng(seed)
+    w = oracle.Weights([[1, 2]], [[1e9, 3e8]], [[0.0, -1.0]])
+    ob = oracle.Box(np.array([1800.0, 1000.0]), w, seed=seed)
+    vol = 10 ** rng.uniform(-24, -21, size=(n, 1)) * rng.dirichlet([1, 1], size=n)
+    klass = rng.integers(1, 3, size=n)
+    ob.add_particles(vol, np.arange(1, n + 1), np.ones(n), klass)
+    ob.next_id = n + 1
+    if sort:
+        ob.sort_with_grid(60, 1e-10, 1e-5)
+    return ob
+
+
+def test_double_appends_copies_with_new_ids():
+    """aero_state_double: the (g, c) particles present at the start are
+    appended once more, in order, with consecutive new ids; the magnitude
+    halves, so the class's number concentration is unchanged."""
+    ob = two_class_box(300, 4)
+    d0 = ob.particles()
+    sel = np.nonzero(d0["klass"] == 2)[0]
+    ob.double(1, 2)
+    d = ob.particles()
+    n0 = len(d0["id"])
+    assert len(d["id"]) == n0 + len(sel)
+    np.testing.assert_array_equal(d["vol"][:n0], d0["vol"])
+    np.testing.assert_array_equal(d["vol"][n0:], d0["vol"][sel])
+    np.testing.assert_array_equal(d["id"][n0:], np.arange(301, 301 + len(sel)))
+    assert ob.next_id == 301 + len(sel)
+    np.testing.assert_array_equal(ob.weight_magnitudes(), [[1e9, 1.5e8]])
+
+
+@pytest.mark.parametrize("sort", [True, False])
+def test_halve_swap_last_order_and_lists(sort):
+    """aero_state_halve removes (g, c) particles with probability 1/2 and
+    logs them as AERO_INFO_HALVED from the last index down; the survivors
+    end in the swap-last order, and with a valid sort the lists stay
+    consistent with the bins."""
+    ob = two_class_box(4000, 5, sort)
+    d0 = ob.particles()
+    ob.halve(step=3, site=11, g=1, c=1)
+    d = ob.particles()
+    ids, act, oth = ob.info()
+    idx = {int(v): i for i, v in enumerate(d0["id"])}
+    removed = [idx[int(v)] for v in ids]
+    assert np.all(act == 3) and np.all(oth == 0)
+    assert removed == sorted(removed, reverse=True)
+    assert np.all(d0["klass"][removed] == 1)
+    n1 = int((d0["klass"] == 1).sum())
+    assert abs(len(removed) - n1 / 2) < 5 * math.sqrt(n1 / 4)
+    # the keyed draw: first uniform of (step 3, site 11, ordinal i + 1)
+    for i in range(0, 4000, 97):
+        u = oracle.Stream(5, 0, 3, 11, i + 1, 8).uniform()
+        assert (i in set(removed)) == (d0["klass"][i] == 1 and u < 0.5)
+    order = swap_last_order(len(d0["id"]), set(removed))
+    np.testing.assert_array_equal(d["id"], d0["id"][order])
+    np.testing.assert_array_equal(ob.weight_magnitudes(), [[2e9, 3e8]])
+    if sort:
+        lists_consistent(ob)
+
+
+def test_rebalance_doubles_and_halves_to_the_ideal():
+    """aero_state_rebalance: double while below half the ideal count,
+    halve while above twice; the magnitude scales by the inverse of the
+    particle-count change, so number concentration is preserved (exactly
+    for doubling, in expectation for halving)."""
+    ob = two_class_box(400, 6)
+    k = ob.particles()["klass"]
+    n1, n2 = int((k == 1).sum()), int((k == 2).sum())
+    # ideal 2000 per class (4000 / 2): 200 -> 400 -> 800 -> 1600
+    ob.rebalance(4000, step=0)
+    k = ob.particles()["klass"]
+    m = ob.weight_magnitudes()
+    for c, n in ((1, n1), (2, n2)):
+        d = 0
+        while n * 2 ** d < 1000:
+            d += 1
+        assert int((k == c).sum()) == n * 2 ** d
+        assert m[0, c - 1] == [1e9, 3e8][c - 1] / 2 ** d
+    # now far above: ideal 100 per class, halve until at most 200
+    before = [(k == c).sum() / m[0, c - 1] for c in (1, 2)]
+    ob.rebalance(200, step=1)
+    k = ob.particles()["klass"]
+    m2 = ob.weight_magnitudes()
+    for c in (1, 2):
+        n = int((k == c).sum())
+        assert 0 < n <= 200
+        passes = round(math.log2(m2[0, c - 1] / m[0, c - 1]))
+        assert passes >= 2
+        # number concentration n * mag within 6 sigma of its expectation
+        exp_n = before[c - 1] * m[0, c - 1] / 2 ** passes
+        assert abs(n - exp_n) < 6 * math.sqrt(exp_n) + 1
