@@ -96,9 +96,10 @@ struct pmc_gpu {
   uint64_t seed = 0;
   uint32_t step = 0;
   ParallelWork work{};
-  // population control: n_part_ideal per box (< 0: not set), counts, plans
+  // population control: n_part_ideal per box (< 0: not set) and scratch
   std::vector<double> n_part_ideal;
-  int *d_gc = nullptr, *d_plan = nullptr;
+  RebalWork rw{};
+  int64_t rw_slots = 0;  // slots of rw.rm / rw.rp
   // per-kernel HIP-event timing
   int timing = 0;
   struct Timed {
@@ -376,7 +377,9 @@ int pmc_gpu_destroy(pmc_gpu *h) {
   free_particles(h);
   Arrays &A = h->A;
   void *ptrs[] = {A.density, A.edges, A.centers, A.widths, A.kmin, A.kmax, A.kw, A.cc,
-                  A.lstart,  A.llen,  A.lcap,    A.sort_scratch, h->d_box, h->d_gc, h->d_plan};
+                  A.lstart,  A.llen,  A.lcap,    A.sort_scratch, h->d_box,
+                  h->rw.gc,  h->rw.plan, h->rw.tile_a, h->rw.tile_off, h->rw.tile_kmax,
+                  h->rw.box_tot, h->rw.box_imax, h->rw.rm, h->rw.rp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   parallel_free(h->work);
@@ -913,6 +916,42 @@ int pmc_gpu_get_weights(pmc_gpu *h, int box, int max_n, int *n_group, int *n_cla
   return 0;
 }
 
+// Scratch of the population-control passes, grown to the current layout.
+static int rebal_ensure(pmc_gpu *h, bool slots) {
+  RebalWork &R = h->rw;
+  const int nb = h->n_box;
+  if (!R.gc) {
+    HIPCHECK(dmalloc(&R.gc, (size_t)nb * (GC_KEYS + 1)));
+    HIPCHECK(dmalloc(&R.plan, (size_t)2 * nb));
+    HIPCHECK(dmalloc(&R.box_tot, (size_t)nb));
+    HIPCHECK(dmalloc(&R.box_imax, (size_t)nb));
+  }
+  int64_t cap_max = 1;
+  for (int64_t c : h->cap) cap_max = std::max(cap_max, c);
+  const int tiles = (int)((cap_max + REBAL_TILE - 1) / REBAL_TILE);
+  if (tiles > R.max_tiles) {
+    for (int *p : {R.tile_a, R.tile_off, R.tile_kmax})
+      if (p) (void)hipFree(p);
+    R.tile_a = R.tile_off = R.tile_kmax = nullptr;
+    R.max_tiles = 0;
+    HIPCHECK(dmalloc(&R.tile_a, (size_t)nb * tiles));
+    HIPCHECK(dmalloc(&R.tile_off, (size_t)nb * tiles));
+    HIPCHECK(dmalloc(&R.tile_kmax, (size_t)nb * tiles));
+    R.max_tiles = tiles;
+  }
+  if (slots && h->rw_slots < h->A.total_cap) {
+    if (R.rm) (void)hipFree(R.rm);
+    if (R.rp) (void)hipFree(R.rp);
+    R.rm = nullptr;
+    R.rp = nullptr;
+    h->rw_slots = 0;
+    HIPCHECK(dmalloc(&R.rm, (size_t)h->A.total_cap));
+    HIPCHECK(dmalloc(&R.rp, (size_t)h->A.total_cap));
+    h->rw_slots = h->A.total_cap;
+  }
+  return 0;
+}
+
 // aero_state_rebalance (src/aero_state.F90:1790-1852) for every box at once.
 // The counts of each (group, class) decide the passes: doublings follow
 // from the counts alone, so capacity is checked before any pass runs;
@@ -926,23 +965,37 @@ int pmc_gpu_rebalance(pmc_gpu *h, int allow_doubling, int allow_halving) {
     if (h->n_part_ideal[b] < 0.0) return fail(ERR_STATE, "rebalance: box %d has no n_part_ideal", b);
   HIPCHECK(hipSetDevice(h->device));
   HIPCHECK(hipStreamSynchronize(h->stream));
+  int e = rebal_ensure(h, false);
+  if (e) return e;
+  RebalWork &R = h->rw;
   const int nb = h->n_box, stride = GC_KEYS + 1;
-  if (!h->d_gc) HIPCHECK(hipMalloc((void **)&h->d_gc, sizeof(int) * (size_t)nb * stride));
-  if (!h->d_plan) HIPCHECK(hipMalloc((void **)&h->d_plan, sizeof(int) * (size_t)nb));
   std::vector<BoxDesc> hb(nb);
   HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * nb, hipMemcpyDeviceToHost));
-  int e = check_box_errors(h, hb);
+  e = check_box_errors(h, hb);
   if (e) return e;
-  std::vector<int> gc((size_t)nb * stride), plan(nb);
+  std::vector<int> gc((size_t)nb * stride), plan(2 * nb);
+  auto tiles_for = [&](int64_t n) { return (int)std::max<int64_t>(1, (n + REBAL_TILE - 1) / REBAL_TILE); };
+  auto n_max = [&]() {
+    int64_t m = 0;
+    for (int b = 0; b < nb; ++b) m = std::max<int64_t>(m, gc[(size_t)b * stride + GC_KEYS]);
+    return m;
+  };
+  int64_t count_tiles = 1;
+  for (auto &B : hb) count_tiles = std::max<int64_t>(count_tiles, tiles_for(B.n_part));
   auto count = [&]() -> hipError_t {
-    hipLaunchKernelGGL(k_count_gc, dim3(nb), dim3(REBAL_THREADS), 0, h->stream, h->d_box, h->A,
-                       h->d_gc);
+    const int64_t ng = (int64_t)nb * stride;
+    hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, h->stream,
+                       R.gc, ng);
+    hipLaunchKernelGGL(k_count_gc, dim3((unsigned)count_tiles, nb), dim3(REBAL_THREADS), 0,
+                       h->stream, h->d_box, h->A, R.gc);
     hipError_t x = hipGetLastError();
     if (x != hipSuccess) return x;
-    x = hipMemcpyAsync(gc.data(), h->d_gc, sizeof(int) * gc.size(), hipMemcpyDeviceToHost,
+    x = hipMemcpyAsync(gc.data(), R.gc, sizeof(int) * gc.size(), hipMemcpyDeviceToHost,
                        h->stream);
     if (x != hipSuccess) return x;
-    return hipStreamSynchronize(h->stream);
+    x = hipStreamSynchronize(h->stream);
+    count_tiles = tiles_for(n_max());
+    return x;
   };
   HIPCHECK(count());
   int ng_max = 1, nc_max = 1;
@@ -955,10 +1008,12 @@ int pmc_gpu_rebalance(pmc_gpu *h, int allow_doubling, int allow_halving) {
   };
   auto has = [&](int b, int g, int c) { return g <= hb[b].w.n_group && c <= hb[b].w.n_class; };
   auto key = [&](int b, int g, int c) { return (size_t)b * stride + (g - 1) + hb[b].w.n_group * (c - 1); };
+  const dim3 bb(64), gb((nb + 63) / 64);
 
   if (allow_doubling) {
     // doublings per (box, g, c): while count < ideal / 2 and count > 0
     std::vector<int> nd((size_t)nb * ng_max * nc_max, 0);
+    std::vector<int64_t> n_box_now(nb);
     for (int b = 0; b < nb; ++b) {
       int64_t n_tot = gc[(size_t)b * stride + GC_KEYS];
       for (int g = 1; g <= ng_max; ++g)
@@ -977,7 +1032,9 @@ int pmc_gpu_rebalance(pmc_gpu *h, int allow_doubling, int allow_halving) {
                         b, (long long)n_tot, (long long)hb[b].cap);
           nd[((size_t)b * ng_max + (g - 1)) * nc_max + (c - 1)] = d;
         }
+      n_box_now[b] = gc[(size_t)b * stride + GC_KEYS];
     }
+    bool any_pass = false;
     for (int g = 1; g <= ng_max; ++g)
       for (int c = 1; c <= nc_max; ++c) {
         int d_max = 0;
@@ -986,16 +1043,33 @@ int pmc_gpu_rebalance(pmc_gpu *h, int allow_doubling, int allow_halving) {
           d_max = std::max(d_max, plan[b]);
         }
         if (!d_max) continue;
-        HIPCHECK(hipMemcpyAsync(h->d_plan, plan.data(), sizeof(int) * nb, hipMemcpyHostToDevice,
+        any_pass = true;
+        HIPCHECK(hipMemcpyAsync(R.plan, plan.data(), sizeof(int) * nb, hipMemcpyHostToDevice,
                                 h->stream));
-        for (int p = 0; p < d_max; ++p)
-          hipLaunchKernelGGL(k_double_pass, dim3(nb), dim3(REBAL_THREADS), 0, h->stream, h->d_box,
-                             h->A, h->d_plan, g, c, p);
-        HIPCHECK(hipGetLastError());
+        for (int p = 0; p < d_max; ++p) {
+          int64_t m = 0;
+          for (int b = 0; b < nb; ++b) m = std::max(m, n_box_now[b]);
+          const dim3 gt(tiles_for(m), nb);
+          hipLaunchKernelGGL(k_double_count, gt, dim3(REBAL_THREADS), 0, h->stream, h->d_box,
+                             h->A, R, g, c, p);
+          hipLaunchKernelGGL(k_tile_scan, dim3(nb), dim3(REBAL_THREADS), 0, h->stream, h->d_box,
+                             R, 0);
+          hipLaunchKernelGGL(k_double_copy, gt, dim3(REBAL_THREADS), 0, h->stream, h->d_box,
+                             h->A, R, g, c, p);
+          hipLaunchKernelGGL(k_double_finish, gb, bb, 0, h->stream, h->d_box, R, nb, g, c, p);
+          HIPCHECK(hipGetLastError());
+          // the (g, c) count of a box doubles with each of its passes
+          for (int b = 0; b < nb; ++b)
+            if (plan[b] > p) n_box_now[b] += (int64_t)gc[key(b, g, c)] << p;
+        }
       }
-    HIPCHECK(count());
+    if (any_pass) {
+      count_tiles = tiles_for(*std::max_element(n_box_now.begin(), n_box_now.end()));
+      HIPCHECK(count());
+    }
   }
   if (allow_halving) {
+    bool slots = false;
     for (int g = 1; g <= ng_max; ++g)
       for (int c = 1; c <= nc_max; ++c)
         for (int k = 0;; ++k) {
@@ -1005,11 +1079,39 @@ int pmc_gpu_rebalance(pmc_gpu *h, int allow_doubling, int allow_halving) {
             any |= plan[b];
           }
           if (!any) break;
-          if (k >= 64) return fail(ERR_STATE, "rebalance: class (%d, %d) still above twice its ideal after 64 halvings", g, c);
-          HIPCHECK(hipMemcpyAsync(h->d_plan, plan.data(), sizeof(int) * nb, hipMemcpyHostToDevice,
+          if (k >= 64)
+            return fail(ERR_STATE, "rebalance: class (%d, %d) still above twice its ideal after 64 halvings", g, c);
+          if (!slots) {
+            if ((e = rebal_ensure(h, true))) return e;
+            // doubling passes may have dropped the sort: lists as they are now
+            HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * nb, hipMemcpyDeviceToHost));
+            slots = true;
+          }
+          int any_lists = 0, any_tiles = 0;
+          for (int b = 0; b < nb; ++b) {
+            const bool lists = hb[b].valid_sort && hb[b].xlists_valid;
+            plan[nb + b] = plan[b] && lists;
+            plan[b] = plan[b] && !lists;
+            any_lists |= plan[nb + b];
+            any_tiles |= plan[b];
+          }
+          HIPCHECK(hipMemcpyAsync(R.plan, plan.data(), sizeof(int) * 2 * nb, hipMemcpyHostToDevice,
                                   h->stream));
-          hipLaunchKernelGGL(k_halve_pass, dim3((nb + 63) / 64), dim3(64), 0, h->stream, h->d_box,
-                             h->A, nb, h->d_plan, g, c, k, h->seed, h->step);
+          if (any_lists)
+            hipLaunchKernelGGL(k_halve_lists, gb, bb, 0, h->stream, h->d_box, h->A, nb, R.plan + nb,
+                               g, c, k, h->seed, h->step);
+          if (any_tiles) {
+            const dim3 gt(tiles_for(n_max()), nb);
+            hipLaunchKernelGGL(k_halve_mark, gt, dim3(REBAL_THREADS), 0, h->stream, h->d_box, h->A,
+                               R, g, c, k, h->seed, h->step);
+            hipLaunchKernelGGL(k_tile_scan, dim3(nb), dim3(REBAL_THREADS), 0, h->stream, h->d_box,
+                               R, 1);
+            hipLaunchKernelGGL(k_halve_prefix, gt, dim3(REBAL_THREADS), 0, h->stream, h->d_box,
+                               h->A, R);
+            hipLaunchKernelGGL(k_halve_move, gt, dim3(REBAL_THREADS), 0, h->stream, h->d_box, h->A,
+                               R);
+            hipLaunchKernelGGL(k_halve_finish, gb, bb, 0, h->stream, h->d_box, R, nb, g, c);
+          }
           HIPCHECK(hipGetLastError());
           HIPCHECK(count());
         }

@@ -76,7 +76,7 @@ def test_exact_rebalance_per_box_plans():
 
 
 def test_parallel_mode_rebalance_matches_oracle_on_the_same_state():
-    pop = small(P.config_c2(), 8000)
+    pop = small(P.config_c2(), 30000)  # several tiles of particles
     g = gpu_boxes([pop], 5, capacity_factor=12)
     g.set_mode(MODE_PARALLEL, PREC_FP32_RECHECK, 1e-5)
     for _ in range(4):
