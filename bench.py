@@ -70,7 +70,7 @@ TIMED = ("k_prepare", "k_exact_sweep", "k_par_hist", "k_par_scatter", "k_par_pai
          "k_par_tests", "k_par_acc_tests", "k_par_apply", "k_par_tail", "k_par_compact")
 
 
-def run_gpu(pops, kernel, dt, mode, precision, steps, warmup, seed=20261202):
+def run_gpu(pops, kernel, dt, mode, precision, steps, warmup, seed=20261202, rebalance=False):
     g = CoagGPU(len(pops), pops[0].density, max_bin=256)
     g.reserve([p.n_part + p.n_part // 4 + 1024 for p in pops])
     g.set_rng(seed, 0)
@@ -111,8 +111,38 @@ def run_gpu(pops, kernel, dt, mode, precision, steps, warmup, seed=20261202):
         if n:
             out["timings"][name] = (ms, n)
     g.set_timing(False)
+    if rebalance:
+        out["rebalance"] = time_rebalance(g, len(pops))
     g.close()
     return out
+
+
+def time_rebalance(g, n_box):
+    """aero_state_rebalance on the evolved boxes, after the timed steps:
+    one halving pass per box (ideal count a third of the current one), then
+    one doubling pass back (ideal 2.5 times), then a call with nothing to
+    do.  Wall clock of the whole call, counts and host reads included."""
+    n0 = [g.n_part(b) for b in range(n_box)]
+    for b in range(n_box):
+        g.set_n_part_ideal(b, n0[b] / 3.0)
+    t = time.perf_counter()
+    g.rebalance()
+    t_halve = time.perf_counter() - t
+    n1 = [g.n_part(b) for b in range(n_box)]
+    for b in range(n_box):
+        g.set_n_part_ideal(b, n1[b] * 2.5)
+    t = time.perf_counter()
+    g.rebalance()
+    t_double = time.perf_counter() - t
+    n2 = [g.n_part(b) for b in range(n_box)]
+    t = time.perf_counter()
+    g.rebalance()
+    t_noop = time.perf_counter() - t
+    return {"particles_before": int(sum(n0)), "after_halving": int(sum(n1)),
+            "after_doubling": int(sum(n2)), "halve_ms": t_halve * 1e3,
+            "double_ms": t_double * 1e3, "no_op_ms": t_noop * 1e3,
+            "halve_particles_per_s": sum(n0) / t_halve,
+            "double_particles_per_s": sum(n1) / t_double}
 
 
 def cpu_baseline(pops, kernel, dt, budget_s=15.0):
@@ -157,6 +187,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--no-rebalance", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of the workload runs on cuda:0; "
@@ -169,14 +201,16 @@ def main():
     pops = build(args.scale)
     t_gen = time.perf_counter() - t_gen
     n_spec = pops[0].n_spec
-    r = run_gpu(pops, kernel, dt, args.mode, args.precision, args.steps, args.warmup)
+    r = run_gpu(pops, kernel, dt, args.mode, args.precision, args.steps, args.warmup,
+                seed=args.seed, rebalance=not args.no_rebalance)
     elapsed = r["elapsed"]
     n_samp, n_coag, psteps = r["n_samp"], r["n_coag"], r["part_steps"]
     timings = r["timings"]
     # the plain-FP64 comparison pass on the same populations and streams
     r64 = None
     if args.mode == "parallel" and args.precision == "fp32" and not args.no_fp64_compare:
-        r64 = run_gpu(pops, kernel, dt, args.mode, "fp64", args.steps, args.warmup)
+        r64 = run_gpu(pops, kernel, dt, args.mode, "fp64", args.steps, args.warmup,
+                      seed=args.seed)
     roofline = None
     if timings:
         dom = max(timings, key=lambda k: timings[k][0])
@@ -239,6 +273,8 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": base,
+        "rebalance": r.get("rebalance"),
+        "seed": args.seed,
         "setup_s": {"generate": t_gen},
     }
     print(json.dumps(line))

@@ -176,6 +176,37 @@ int pmc_gpu_set_n_part_ideal(pmc_gpu *h, int box, double n_part);
 int pmc_gpu_rebalance(pmc_gpu *h, int allow_doubling, int allow_halving);
 int pmc_gpu_get_weights(pmc_gpu *h, int box, int max_n, int *n_group, int *n_class, int *type,
                         double *magnitude, double *exponent);
+
+/* Sources and sinks of scenario_update_aero_state (src/scenario.F90:259-341).
+ * pmc_gpu_dilute: the dilution loss to the background,
+ * aero_state_sample_particles(aero_state, delta, 1 - p, AERO_INFO_DILUTION)
+ * with equal weights (src/scenario.F90:316-319, src/aero_state.F90:856-929):
+ * each particle of box b leaves with probability sample_prob[b], logged as
+ * AERO_INFO_DILUTION (1).
+ * pmc_gpu_particle_loss: scenario_particle_loss (src/scenario.F90:619-745)
+ * with loss function codes of src/scenario.F90:26-36 (1 none, 2 constant,
+ * 3 volume, 4 dry deposition; 5 chamber returns PMC_ERR_UNSUPPORTED): each
+ * particle leaves with probability 1 - exp(-del_t rate), logged as
+ * AERO_INFO_DILUTION.  height[b] is the box's mixing height (m), read by
+ * dry deposition only (may be NULL otherwise).
+ * Removal draws are per particle (the same distributions as the
+ * reference's binomial count with uniform picks and its geometric-skip
+ * accept-reject), from the handle's seed and current step.
+ * pmc_gpu_scale_weights: aero_weight_array_scale
+ * (src/aero_weight_array.F90:160-169), e.g. by
+ * old_temp * pressure / (temp * old_pressure) (src/scenario.F90:336-338).
+ * pmc_gpu_append: aero_state_add_particle for each of n particles
+ * (src/aero_state.F90:365-383), e.g. the emission and background samples
+ * of aero_state_add_aero_dist_sample; arguments as pmc_gpu_upload; id NULL
+ * assigns new ids with aero_particle_new_id.  A particle outside the bin
+ * grid invalidates the sort (the remake runs at the next step). */
+int pmc_gpu_dilute(pmc_gpu *h, const double *sample_prob);
+int pmc_gpu_particle_loss(pmc_gpu *h, int loss_function, double del_t, const double *height);
+int pmc_gpu_scale_weights(pmc_gpu *h, int box, double factor);
+int pmc_gpu_append(pmc_gpu *h, int box, int n, const double *vol, const int64_t *id,
+                   const int *group, const int *klass, const int *n_primary, const int *n_comp,
+                   const int *comp_source, const double *comp_time, const double *least,
+                   const double *greatest);
 /* Particle-steps per box since the last reset: the sum over steps of the
  * box's particle count at the start of each step. */
 int pmc_gpu_get_part_steps(pmc_gpu *h, int64_t *part_steps);

@@ -105,6 +105,11 @@ def lib():
             "orc_box_halve": (_i, [_p, _i, _i, _u32, _u32]),
             "orc_box_rebalance": (_i, [_p, _i, _i, _p, _u32]),
             "orc_box_get_weights": (None, [_p, _p]),
+            "orc_box_scale_weights": (None, [_p, _d]),
+            "orc_loss_rate_dry_dep": (_d, [_d, _d, _d, _d, _d]),
+            "orc_loss_rate": (_i, [_i, _d, _d, _d, _d, _d, _p]),
+            "orc_box_dilute": (_i, [_p, _d, _u32]),
+            "orc_box_particle_loss": (_i, [_p, _i, _d, _d, _u32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -174,6 +179,18 @@ def histogram_1d(edges, widths, x, w):
     out = np.zeros(len(wd))
     lib().orc_histogram_1d(len(wd), _ptr(e), _ptr(wd), len(xs), _ptr(xs), _ptr(ws), _ptr(out))
     return out
+
+
+def loss_rate(loss_type, vol, density, temp=288.0, pressure=1e5, height=1000.0):
+    """scenario_loss_rate (src/scenario.F90:398-437)."""
+    r = np.zeros(1)
+    err = lib().orc_loss_rate(loss_type, vol, density, temp, pressure, height, _ptr(r))
+    if err:
+        raise RuntimeError(f"oracle loss_rate error {err}")
+    return float(r[0])
+
+
+LOSS_NONE, LOSS_CONSTANT, LOSS_VOLUME, LOSS_DRYDEP, LOSS_CHAMBER = 1, 2, 3, 4, 5
 
 
 def kernel(kind, density, vol1, vol2, temp=288.0, pressure=1e5, beta=1000.0):
@@ -368,6 +385,19 @@ class Box:
 
     def halve(self, step, site, g=1, c=1):
         lib().orc_box_halve(self._h, g, c, step, site)
+
+    def scale_weights(self, factor):
+        lib().orc_box_scale_weights(self._h, factor)
+
+    def dilute(self, sample_prob, step):
+        err = lib().orc_box_dilute(self._h, sample_prob, step)
+        if err:
+            raise RuntimeError(f"oracle dilute error {err}")
+
+    def particle_loss(self, loss_type, del_t, height, step):
+        err = lib().orc_box_particle_loss(self._h, loss_type, del_t, height, step)
+        if err:
+            raise RuntimeError(f"oracle particle_loss error {err}")
 
     def weight_magnitudes(self):
         ng, nc = self.weights.n_group, self.weights.n_class

@@ -1255,8 +1255,16 @@ int orc_box_add_particles(orc_box *b, int n, const double *vol,
             p.comp_time[0] = 0.0;
         }
         if (b->valid_sort) {
+            /* aero_sorted_add_particle, src/aero_sorted.F90:397-449: a
+             * particle outside the grid makes the sort invalid; the remake
+             * is left to the next aero_state_sort, which sees every
+             * appended particle */
             int err = state_add(b, &p, 1);
-            if (err) return err;
+            if (err == 134572570) {
+                b->valid_sort = 0;
+            } else if (err) {
+                return err;
+            }
         } else {
             ensure_cap(b, b->n + 1);
             store_particle(b, b->n, &p);
@@ -1883,6 +1891,128 @@ int orc_box_rebalance(orc_box *b, int allow_doubling, int allow_halving,
                     k++;
                 }
             }
+    return 0;
+}
+
+/* aero_weight_array_scale, src/aero_weight_array.F90:160-169 */
+void orc_box_scale_weights(orc_box *b, double factor) {
+    for (int i = 0; i < b->n_group * b->n_class; ++i) b->wmag[i] *= factor;
+}
+
+/* ------------------------------------------------------------------ */
+/* sinks: dilution and particle loss                                   */
+/* ------------------------------------------------------------------ */
+/* scenario_loss_rate_dry_dep, src/scenario.F90:444-533 (spherical
+ * particles: aero_data_vol2diam = 2 vol2rad) */
+double orc_loss_rate_dry_dep(double vol, double density, double temp,
+                             double pressure, double height) {
+    const double u_mean = 5.0, z_ref = 20.0, z_rough = 0.1;
+    const double A = 2.0 / 1000.0, alpha = 1.2, beta = 2.0, gamma = 0.54;
+    double d_p = 2.0 * orc_sphere_vol2rad(vol);
+    double density_air = (C_AIR_MW * pressure) / (C_UNIV_GAS * temp);
+    double visc_d = 1.8325e-5 * (416.16 / (temp + 120.0)) * pow(temp / 296.16, 1.5);
+    double visc_k = visc_d / density_air;
+    double gas_speed = sqrt((8.0 * C_BOLTZMANN * temp * C_AVOGADRO) /
+                            (C_PI * C_AIR_MW));
+    double gas_mean_free_path = (2.0 * visc_d) / (density_air * gas_speed);
+    double knud = (2.0 * gas_mean_free_path) / d_p;
+    double cunning = 1.0 + knud * (1.257 + 0.4 * exp(-1.1 / knud));
+    double grav = 9.81;
+    double V_s = (density * pow(d_p, 2.0) * grav * cunning) / (18.0 * visc_d);
+    double u_star = 0.4 * u_mean / log(z_ref / z_rough);
+    double R_a = (1.0 / (0.4 * u_star)) * log(z_ref / z_rough);
+    double diff_p = (C_BOLTZMANN * temp * cunning) / (3.0 * C_PI * visc_d * d_p);
+    double Sc = visc_k / diff_p;
+    double E_B = pow(Sc, -gamma);
+    double E_IN = 0.5 * pow(d_p / A, 2.0);
+    double St = (V_s * u_star) / (grav * A);
+    double E_IM = pow(St / (alpha + St), beta);
+    double R1 = exp(-pow(St, 0.5));
+    double eps_0 = 3.0;
+    double R_s = 1.0 / (eps_0 * u_star * (E_B + E_IN + E_IM) * R1);
+    double V_d = V_s + (1.0 / (R_a + R_s + R_a * R_s * V_s));
+    return V_d / height;
+}
+
+/* scenario_loss_rate, src/scenario.F90:398-437, loss function codes
+ * :26-36.  The chamber function needs chamber geometry that is not on the
+ * path: ORC_ERR_UNSUPPORTED. */
+int orc_loss_rate(int type, double vol, double density, double temp,
+                  double pressure, double height, double *rate) {
+    switch (type) {
+    case ORC_LOSS_INVALID: case ORC_LOSS_NONE: *rate = 0.0; return 0;
+    case ORC_LOSS_CONSTANT: *rate = 1e-3; return 0;
+    case ORC_LOSS_VOLUME: *rate = 1e15 * vol; return 0;
+    case ORC_LOSS_DRYDEP:
+        *rate = orc_loss_rate_dry_dep(vol, density, temp, pressure, height);
+        return 0;
+    case ORC_LOSS_CHAMBER: return ORC_ERR_UNSUPPORTED;
+    default: return 201594391;
+    }
+}
+
+/* One removal pass: the particle at 1-based index i is removed with its
+ * probability prob[i-1] when the first uniform of stream (step, site 0,
+ * ordinal i, kind) is below it, from the last particle down, swap-last
+ * (through the lists when the sort is valid), logged with the action and
+ * other id 0.  Independent per-particle draws give the same distribution
+ * as the reference's binomial count plus uniform picks
+ * (aero_state_sample_particles) and its geometric-skip accept-reject
+ * (scenario_particle_loss); they are drawn per particle here so the pass
+ * runs in parallel on the device. */
+static void remove_pass(orc_box *b, const double *prob, int action,
+                        uint32_t step, uint32_t kind) {
+    for (int i = b->n - 1; i >= 0; --i) {
+        if (!(prob[i] > 0.0)) continue;
+        orc_stream s;
+        orc_stream_open(&s, b->seed, b->box_index, step, 0, (uint32_t)(i + 1), kind);
+        if (orc_stream_uniform(&s) < prob[i]) {
+            int64_t id = b->id[i];
+            if (b->valid_sort) {
+                state_remove(b, i, 1, id, action, 0);
+            } else {
+                apa_remove(b, i);
+                add_info(b, id, action, 0);
+            }
+        }
+    }
+}
+
+/* Dilution out of the box: aero_state_sample_particles(from, to, prob,
+ * AERO_INFO_DILUTION) with equal weights on both sides
+ * (src/aero_state.F90:856-929, called from src/scenario.F90:316-319): each
+ * particle leaves with probability sample_prob. */
+int orc_box_dilute(orc_box *b, double sample_prob, uint32_t step) {
+    if (!(sample_prob >= 0.0 && sample_prob <= 1.0)) return 721006962;
+    double *p = (double *)malloc(sizeof(double) * (size_t)(b->n > 0 ? b->n : 1));
+    for (int i = 0; i < b->n; ++i) p[i] = sample_prob;
+    remove_pass(b, p, ORC_INFO_DILUTION, step, ORC_RNG_DILUTE);
+    free(p);
+    return 0;
+}
+
+/* scenario_particle_loss, src/scenario.F90:619-704 with
+ * scenario_try_single_particle_loss :709-745: each particle is removed
+ * with probability 1 - exp(-del_t rate(vol, density)), logged as
+ * AERO_INFO_DILUTION. */
+int orc_box_particle_loss(orc_box *b, int loss_type, double del_t,
+                          double height, uint32_t step) {
+    if (loss_type == ORC_LOSS_NONE || loss_type == ORC_LOSS_INVALID) return 0;
+    double *p = (double *)malloc(sizeof(double) * (size_t)(b->n > 0 ? b->n : 1));
+    for (int i = 0; i < b->n; ++i) {
+        const double *v = b->vol + (size_t)i * b->n_spec;
+        double rate = 0.0;
+        int err = orc_loss_rate(loss_type, particle_volume(b->n_spec, v),
+                                particle_density(b->n_spec, b->density, v),
+                                b->temp, b->pressure, height, &rate);
+        if (err) {
+            free(p);
+            return err;
+        }
+        p[i] = 1.0 - exp(-del_t * rate);
+    }
+    remove_pass(b, p, ORC_INFO_DILUTION, step, ORC_RNG_LOSS);
+    free(p);
     return 0;
 }
 

@@ -44,6 +44,7 @@ extern "C" {
 #define ORC_WEIGHT_MFA 3
 
 /* removal actions, src/aero_info.F90:19-29 */
+#define ORC_INFO_DILUTION 1
 #define ORC_INFO_COAG 2
 #define ORC_INFO_HALVED 3
 #define ORC_INFO_WEIGHT 4
@@ -57,6 +58,17 @@ extern "C" {
 #define ORC_RNG_ACC_FINISH 6   /* binomial, prob_round, resampling         */
 #define ORC_RNG_ACC_TARGET 7   /* coag_target_with_source: merge, groups, dups */
 #define ORC_RNG_HALVE 8        /* aero_state_halve: one uniform per particle */
+#define ORC_RNG_DILUTE 9       /* dilution out: one uniform per particle    */
+#define ORC_RNG_LOSS 10        /* particle loss: one uniform per particle   */
+
+/* scenario loss function codes, src/scenario.F90:26-36 */
+#define ORC_LOSS_INVALID 0
+#define ORC_LOSS_NONE 1
+#define ORC_LOSS_CONSTANT 2
+#define ORC_LOSS_VOLUME 3
+#define ORC_LOSS_DRYDEP 4
+#define ORC_LOSS_CHAMBER 5
+#define ORC_ERR_UNSUPPORTED 4  /* a case the oracle does not restate */
 
 /* ---- Philox4x32-10 and the per-site uniform stream ---- */
 typedef struct {
@@ -191,6 +203,16 @@ int orc_box_halve(orc_box *b, int g, int c, uint32_t step, uint32_t site);
 int orc_box_rebalance(orc_box *b, int allow_doubling, int allow_halving,
                       const double *ideal, uint32_t step);
 void orc_box_get_weights(const orc_box *b, double *wmag);
+void orc_box_scale_weights(orc_box *b, double factor);
+
+/* sinks (src/scenario.F90:259-341) */
+double orc_loss_rate_dry_dep(double vol, double density, double temp,
+                             double pressure, double height);
+int orc_loss_rate(int type, double vol, double density, double temp,
+                  double pressure, double height, double *rate);
+int orc_box_dilute(orc_box *b, double sample_prob, uint32_t step);
+int orc_box_particle_loss(orc_box *b, int loss_type, double del_t,
+                          double height, uint32_t step);
 
 #ifdef __cplusplus
 }

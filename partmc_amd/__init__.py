@@ -44,6 +44,10 @@ ABI = {
     "pmc_gpu_size_dist": (_i, [_p, _i, _i, _i, _d, _d, _p]),
     "pmc_gpu_set_n_part_ideal": (_i, [_p, _i, _d]),
     "pmc_gpu_clear_removed": (_i, [_p, _i]),
+    "pmc_gpu_dilute": (_i, [_p, _p]),
+    "pmc_gpu_particle_loss": (_i, [_p, _i, _d, _p]),
+    "pmc_gpu_scale_weights": (_i, [_p, _i, _d]),
+    "pmc_gpu_append": (_i, [_p, _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "pmc_gpu_rebalance": (_i, [_p, _i, _i]),
     "pmc_gpu_get_weights": (_i, [_p, _i, _i, _p, _p, _p, _p, _p]),
     "pmc_gpu_get_error_stats": (_i, [_p, _p]),
@@ -238,6 +242,30 @@ class CoagGPU:
         out = np.zeros(self.n_box, dtype=np.int64)
         _check(load().pmc_gpu_get_part_steps(self._h, _ptr(out)))
         return out
+
+    def dilute(self, sample_prob):
+        """Dilution out of every box: particle leaves with sample_prob[box]."""
+        p = np.ascontiguousarray(sample_prob, dtype=np.float64).reshape(self.n_box)
+        _check(load().pmc_gpu_dilute(self._h, _ptr(p)))
+
+    def particle_loss(self, loss_function, del_t, height=None):
+        """scenario_particle_loss with the reference's loss function code."""
+        h = None if height is None else np.ascontiguousarray(height, dtype=np.float64).reshape(self.n_box)
+        _check(load().pmc_gpu_particle_loss(self._h, loss_function, del_t, _ptr(h)))
+
+    def scale_weights(self, box, factor):
+        _check(load().pmc_gpu_scale_weights(self._h, box, float(factor)))
+
+    def append(self, box, vol, group, klass, ids=None, n_primary=None):
+        """aero_state_add_particle for each row of vol; ids None: new ids."""
+        vol = np.ascontiguousarray(vol, dtype=np.float64).reshape(-1, self.n_spec)
+        n = vol.shape[0]
+        g = np.ascontiguousarray(group, dtype=np.int32)
+        c = np.ascontiguousarray(klass, dtype=np.int32)
+        i = None if ids is None else np.ascontiguousarray(ids, dtype=np.int64)
+        npr = None if n_primary is None else np.ascontiguousarray(n_primary, dtype=np.int32)
+        _check(load().pmc_gpu_append(self._h, box, n, _ptr(vol), _ptr(i), _ptr(g), _ptr(c),
+                                     _ptr(npr), None, None, None, None, None))
 
     def clear_removed(self, box):
         _check(load().pmc_gpu_clear_removed(self._h, box))

@@ -379,7 +379,7 @@ int pmc_gpu_destroy(pmc_gpu *h) {
   void *ptrs[] = {A.density, A.edges, A.centers, A.widths, A.kmin, A.kmax, A.kw, A.cc,
                   A.lstart,  A.llen,  A.lcap,    A.sort_scratch, h->d_box,
                   h->rw.gc,  h->rw.plan, h->rw.tile_a, h->rw.tile_off, h->rw.tile_kmax,
-                  h->rw.box_tot, h->rw.box_imax, h->rw.rm, h->rw.rp};
+                  h->rw.box_tot, h->rw.box_imax, h->rw.rm, h->rw.rp, h->rw.bval};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   parallel_free(h->work);
@@ -482,29 +482,15 @@ __global__ void k_default_components(Arrays A, int64_t base, int n, int set_comp
   }
 }
 
-int pmc_gpu_upload(pmc_gpu *h, int box, int n_part, int capacity, const double *vol,
-                   const int64_t *id, const int *group, const int *klass, const int *n_primary,
-                   const int *n_comp, const int *comp_source, const double *comp_time,
-                   const double *least, const double *greatest) {
-  int e = check_box(h, box);
-  if (e) return e;
-  if (n_part < 0 || (n_part > 0 && (!vol || !id || !group || !klass)))
-    return fail(ERR_ARG, "upload: bad arguments");
-  if (n_comp && (!comp_source || !comp_time))
-    return fail(ERR_ARG, "upload: n_comp given without comp_source/comp_time");
-  HIPCHECK(hipSetDevice(h->device));
-  HIPCHECK(hipStreamSynchronize(h->stream));
-  int64_t want = std::max<int64_t>(capacity, n_part);
-  if (want < 16) want = 16;
-  if (want > h->cap[box]) {
-    std::vector<int64_t> nc = h->cap;
-    nc[box] = want;
-    if ((e = layout(h, nc))) return e;
-  }
-  BoxDesc hb;
-  HIPCHECK(hipMemcpy(&hb, h->d_box + box, sizeof(BoxDesc), hipMemcpyDeviceToHost));
+// Particle records n_part .. into the box's slots [first, first + n_part):
+// volumes particle-major vol[i * n_spec + s] as aero_particle_t%vol, the
+// optional component and create-time arrays defaulted when absent.
+static int put_particles(pmc_gpu *h, const BoxDesc &hb, int64_t first, int n_part,
+                         const double *vol, const int64_t *id, const int *group, const int *klass,
+                         const int *n_primary, const int *n_comp, const int *comp_source,
+                         const double *comp_time, const double *least, const double *greatest) {
   Arrays &A = h->A;
-  const int64_t base = hb.off, tc = A.total_cap;
+  const int64_t base = hb.off + first, tc = A.total_cap;
   const int ns = h->n_spec;
   if (n_part > 0) {
     std::vector<double> row(n_part);
@@ -519,7 +505,7 @@ int pmc_gpu_upload(pmc_gpu *h, int box, int n_part, int capacity, const double *
       row[i] = v;
     }
     HIPCHECK(hipMemcpy(A.vtot + base, row.data(), n_part * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemcpy(A.id + base, id, n_part * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (id) HIPCHECK(hipMemcpy(A.id + base, id, n_part * sizeof(int64_t), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(A.group + base, group, n_part * sizeof(int), hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(A.klass + base, klass, n_part * sizeof(int), hipMemcpyHostToDevice));
     if (n_primary) {
@@ -555,6 +541,33 @@ int pmc_gpu_upload(pmc_gpu *h, int box, int n_part, int capacity, const double *
       HIPCHECK(hipStreamSynchronize(h->stream));
     }
   }
+  return 0;
+}
+
+int pmc_gpu_upload(pmc_gpu *h, int box, int n_part, int capacity, const double *vol,
+                   const int64_t *id, const int *group, const int *klass, const int *n_primary,
+                   const int *n_comp, const int *comp_source, const double *comp_time,
+                   const double *least, const double *greatest) {
+  int e = check_box(h, box);
+  if (e) return e;
+  if (n_part < 0 || (n_part > 0 && (!vol || !id || !group || !klass)))
+    return fail(ERR_ARG, "upload: bad arguments");
+  if (n_comp && (!comp_source || !comp_time))
+    return fail(ERR_ARG, "upload: n_comp given without comp_source/comp_time");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  int64_t want = std::max<int64_t>(capacity, n_part);
+  if (want < 16) want = 16;
+  if (want > h->cap[box]) {
+    std::vector<int64_t> nc = h->cap;
+    nc[box] = want;
+    if ((e = layout(h, nc))) return e;
+  }
+  BoxDesc hb;
+  HIPCHECK(hipMemcpy(&hb, h->d_box + box, sizeof(BoxDesc), hipMemcpyDeviceToHost));
+  if ((e = put_particles(h, hb, 0, n_part, vol, id, group, klass, n_primary, n_comp, comp_source,
+                         comp_time, least, greatest)))
+    return e;
   hb.n_part = n_part;
   hb.valid_sort = 0;
   hb.xlists_valid = 0;
@@ -925,6 +938,7 @@ static int rebal_ensure(pmc_gpu *h, bool slots) {
     HIPCHECK(dmalloc(&R.plan, (size_t)2 * nb));
     HIPCHECK(dmalloc(&R.box_tot, (size_t)nb));
     HIPCHECK(dmalloc(&R.box_imax, (size_t)nb));
+    HIPCHECK(dmalloc(&R.bval, (size_t)nb));
   }
   int64_t cap_max = 1;
   for (int64_t c : h->cap) cap_max = std::max(cap_max, c);
@@ -949,6 +963,45 @@ static int rebal_ensure(pmc_gpu *h, bool slots) {
     HIPCHECK(dmalloc(&R.rp, (size_t)h->A.total_cap));
     h->rw_slots = h->A.total_cap;
   }
+  return 0;
+}
+
+// One removal pass over the boxes with want[b] != 0.  Boxes with current
+// bin lists (exact mode) remove one at a time through the lists, in the
+// reference's order; the others settle every slot at once.  hb must hold
+// the boxes' current sort flags.
+static int run_removal(pmc_gpu *h, const std::vector<BoxDesc> &hb, const std::vector<int> &want,
+                       const Removal &X, int64_t n_max) {
+  int e = rebal_ensure(h, true);
+  if (e) return e;
+  RebalWork &R = h->rw;
+  const int nb = h->n_box;
+  std::vector<int> plan(2 * nb, 0);
+  int any_lists = 0, any_tiles = 0;
+  for (int b = 0; b < nb; ++b) {
+    const bool lists = hb[b].valid_sort && hb[b].xlists_valid;
+    plan[nb + b] = want[b] && lists;
+    plan[b] = want[b] && !lists;
+    any_lists |= plan[nb + b];
+    any_tiles |= plan[b];
+  }
+  HIPCHECK(hipMemcpyAsync(R.plan, plan.data(), sizeof(int) * 2 * nb, hipMemcpyHostToDevice,
+                          h->stream));
+  const dim3 bb(64), gb((nb + 63) / 64);
+  if (any_lists)
+    hipLaunchKernelGGL(k_remove_lists, gb, bb, 0, h->stream, h->d_box, h->A, nb, R.plan + nb, X,
+                       h->seed, h->step);
+  if (any_tiles) {
+    const dim3 gt((unsigned)std::max<int64_t>(1, (n_max + REBAL_TILE - 1) / REBAL_TILE), nb);
+    hipLaunchKernelGGL(k_remove_mark, gt, dim3(REBAL_THREADS), 0, h->stream, h->d_box, h->A, R, X,
+                       h->seed, h->step);
+    hipLaunchKernelGGL(k_tile_scan, dim3(nb), dim3(REBAL_THREADS), 0, h->stream, h->d_box, R, 1);
+    hipLaunchKernelGGL(k_remove_prefix, gt, dim3(REBAL_THREADS), 0, h->stream, h->d_box, h->A, R,
+                       X.kind == RM_HALVE ? INFO_HALVED : INFO_DILUTION);
+    hipLaunchKernelGGL(k_remove_move, gt, dim3(REBAL_THREADS), 0, h->stream, h->d_box, h->A, R);
+    hipLaunchKernelGGL(k_remove_finish, gb, bb, 0, h->stream, h->d_box, R, nb, X);
+  }
+  HIPCHECK(hipGetLastError());
   return 0;
 }
 
@@ -1069,56 +1122,164 @@ int pmc_gpu_rebalance(pmc_gpu *h, int allow_doubling, int allow_halving) {
     }
   }
   if (allow_halving) {
-    bool slots = false;
+    bool fresh = false;
     for (int g = 1; g <= ng_max; ++g)
       for (int c = 1; c <= nc_max; ++c)
         for (int k = 0;; ++k) {
+          std::vector<int> want(nb, 0);
           int any = 0;
           for (int b = 0; b < nb; ++b) {
-            plan[b] = (has(b, g, c) && (double)gc[key(b, g, c)] > ideal(b) * 2.0) ? 1 : 0;
-            any |= plan[b];
+            want[b] = (has(b, g, c) && (double)gc[key(b, g, c)] > ideal(b) * 2.0) ? 1 : 0;
+            any |= want[b];
           }
           if (!any) break;
           if (k >= 64)
             return fail(ERR_STATE, "rebalance: class (%d, %d) still above twice its ideal after 64 halvings", g, c);
-          if (!slots) {
-            if ((e = rebal_ensure(h, true))) return e;
-            // doubling passes may have dropped the sort: lists as they are now
+          if (!fresh) {
+            // doubling passes may have dropped the sort: flags as they are now
             HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * nb, hipMemcpyDeviceToHost));
-            slots = true;
+            fresh = true;
           }
-          int any_lists = 0, any_tiles = 0;
-          for (int b = 0; b < nb; ++b) {
-            const bool lists = hb[b].valid_sort && hb[b].xlists_valid;
-            plan[nb + b] = plan[b] && lists;
-            plan[b] = plan[b] && !lists;
-            any_lists |= plan[nb + b];
-            any_tiles |= plan[b];
-          }
-          HIPCHECK(hipMemcpyAsync(R.plan, plan.data(), sizeof(int) * 2 * nb, hipMemcpyHostToDevice,
-                                  h->stream));
-          if (any_lists)
-            hipLaunchKernelGGL(k_halve_lists, gb, bb, 0, h->stream, h->d_box, h->A, nb, R.plan + nb,
-                               g, c, k, h->seed, h->step);
-          if (any_tiles) {
-            const dim3 gt(tiles_for(n_max()), nb);
-            hipLaunchKernelGGL(k_halve_mark, gt, dim3(REBAL_THREADS), 0, h->stream, h->d_box, h->A,
-                               R, g, c, k, h->seed, h->step);
-            hipLaunchKernelGGL(k_tile_scan, dim3(nb), dim3(REBAL_THREADS), 0, h->stream, h->d_box,
-                               R, 1);
-            hipLaunchKernelGGL(k_halve_prefix, gt, dim3(REBAL_THREADS), 0, h->stream, h->d_box,
-                               h->A, R);
-            hipLaunchKernelGGL(k_halve_move, gt, dim3(REBAL_THREADS), 0, h->stream, h->d_box, h->A,
-                               R);
-            hipLaunchKernelGGL(k_halve_finish, gb, bb, 0, h->stream, h->d_box, R, nb, g, c);
-          }
-          HIPCHECK(hipGetLastError());
+          Removal X{};
+          X.kind = RM_HALVE;
+          X.g = g;
+          X.c = c;
+          X.k = k;
+          if ((e = run_removal(h, hb, want, X, n_max()))) return e;
           HIPCHECK(count());
         }
   }
   HIPCHECK(hipStreamSynchronize(h->stream));
   HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * nb, hipMemcpyDeviceToHost));
   return check_box_errors(h, hb);
+}
+
+// A removal pass of every box with a per-box parameter (dilution
+// probability or mixing height) after the usual checks.
+static int removal_all(pmc_gpu *h, Removal X, const double *per_box) {
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  int e = rebal_ensure(h, true);
+  if (e) return e;
+  const int nb = h->n_box;
+  std::vector<BoxDesc> hb(nb);
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * nb, hipMemcpyDeviceToHost));
+  if ((e = check_box_errors(h, hb))) return e;
+  if (per_box) {
+    HIPCHECK(hipMemcpyAsync(h->rw.bval, per_box, sizeof(double) * nb, hipMemcpyHostToDevice,
+                            h->stream));
+    if (X.kind == RM_DILUTE) X.prob = h->rw.bval;
+    else X.height = h->rw.bval;
+  }
+  std::vector<int> want(nb, 0);
+  int64_t n_max = 0;
+  for (int b = 0; b < nb; ++b) {
+    want[b] = hb[b].n_part > 0 && (X.kind != RM_DILUTE || per_box[b] > 0.0);
+    n_max = std::max<int64_t>(n_max, hb[b].n_part);
+  }
+  if ((e = run_removal(h, hb, want, X, n_max))) return e;
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * nb, hipMemcpyDeviceToHost));
+  return check_box_errors(h, hb);
+}
+
+int pmc_gpu_dilute(pmc_gpu *h, const double *sample_prob) {
+  if (!h || !sample_prob) return fail(ERR_ARG, "dilute: null argument");
+  for (int b = 0; b < h->n_box; ++b)
+    if (!(sample_prob[b] >= 0.0 && sample_prob[b] <= 1.0))
+      return fail(721006962, "dilute: box %d probability %g outside [0, 1]", b, sample_prob[b]);
+  Removal X{};
+  X.kind = RM_DILUTE;
+  return removal_all(h, X, sample_prob);
+}
+
+int pmc_gpu_particle_loss(pmc_gpu *h, int loss_function, double del_t, const double *height) {
+  if (!h) return fail(ERR_ARG, "null handle");
+  if (loss_function == LOSS_NONE || loss_function == LOSS_INVALID) return 0;
+  if (loss_function == LOSS_CHAMBER)
+    return fail(ERR_UNSUPPORTED, "particle_loss: the chamber loss function needs chamber geometry");
+  if (loss_function < LOSS_INVALID || loss_function > LOSS_CHAMBER)
+    return fail(201594391, "Unknown loss function id: %d", loss_function);
+  if (!(del_t >= 0.0)) return fail(ERR_ARG, "particle_loss: del_t must be >= 0");
+  if (loss_function == LOSS_DRYDEP) {
+    if (!height) return fail(ERR_ARG, "particle_loss: dry deposition needs the mixing height");
+    for (int b = 0; b < h->n_box; ++b)
+      if (!(height[b] > 0.0)) return fail(ERR_ARG, "particle_loss: box %d height must be > 0", b);
+  }
+  Removal X{};
+  X.kind = RM_LOSS;
+  X.loss = loss_function;
+  X.del_t = del_t;
+  return removal_all(h, X, loss_function == LOSS_DRYDEP ? height : nullptr);
+}
+
+int pmc_gpu_append(pmc_gpu *h, int box, int n, const double *vol, const int64_t *id,
+                   const int *group, const int *klass, const int *n_primary, const int *n_comp,
+                   const int *comp_source, const double *comp_time, const double *least,
+                   const double *greatest) {
+  int e = check_box(h, box);
+  if (e) return e;
+  if (n < 0 || (n > 0 && (!vol || !group || !klass)))
+    return fail(ERR_ARG, "append: bad arguments");
+  if (n_comp && (!comp_source || !comp_time))
+    return fail(ERR_ARG, "append: n_comp given without comp_source/comp_time");
+  if (n == 0) return 0;
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  BoxDesc hb;
+  HIPCHECK(hipMemcpy(&hb, h->d_box + box, sizeof(BoxDesc), hipMemcpyDeviceToHost));
+  if (hb.error) {
+    std::vector<BoxDesc> all(h->n_box);
+    HIPCHECK(hipMemcpy(all.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
+    return check_box_errors(h, all);
+  }
+  for (int i = 0; i < n; ++i)
+    if (group[i] < 1 || group[i] > hb.w.n_group || klass[i] < 1 || klass[i] > hb.w.n_class)
+      return fail(ERR_ARG, "append: particle %d has group %d / class %d outside the weights", i,
+                  group[i], klass[i]);
+  const int first = hb.n_part;
+  if ((int64_t)first + n > h->cap[box]) {
+    std::vector<int64_t> nc = h->cap;
+    nc[box] = std::max<int64_t>(2 * h->cap[box], (int64_t)first + n);
+    if ((e = layout(h, nc))) return e;
+    HIPCHECK(hipMemcpy(&hb, h->d_box + box, sizeof(BoxDesc), hipMemcpyDeviceToHost));
+  }
+  if ((e = put_particles(h, hb, first, n, vol, id, group, klass, n_primary, n_comp, comp_source,
+                         comp_time, least, greatest)))
+    return e;
+  if ((e = rebal_ensure(h, false))) return e;
+  if (!id)
+    hipLaunchKernelGGL(k_assign_ids, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_box,
+                       h->A, box, first, n);
+  HIPCHECK(hipMemsetAsync(h->rw.box_tot, 0, sizeof(int), h->stream));
+  hipLaunchKernelGGL(k_append_check, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_box,
+                     h->A, box, first, n, h->rw.box_tot);
+  hipLaunchKernelGGL(k_append_finish, dim3(1), dim3(64), 0, h->stream, h->d_box, h->A, box, first,
+                     n, h->rw.box_tot, id ? 0 : n);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  HIPCHECK(hipMemcpy(&hb, h->d_box + box, sizeof(BoxDesc), hipMemcpyDeviceToHost));
+  if (hb.error) {
+    std::vector<BoxDesc> all(h->n_box);
+    HIPCHECK(hipMemcpy(all.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
+    return check_box_errors(h, all);
+  }
+  return 0;
+}
+
+int pmc_gpu_scale_weights(pmc_gpu *h, int box, double factor) {
+  int e = check_box(h, box);
+  if (e) return e;
+  if (!(factor > 0.0)) return fail(ERR_ARG, "scale_weights: factor must be > 0");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  Weights w;
+  HIPCHECK(hipMemcpy(&w, (char *)(h->d_box + box) + offsetof(BoxDesc, w), sizeof(Weights),
+                     hipMemcpyDeviceToHost));
+  for (int i = 0; i < w.n_group * w.n_class; ++i) w.mag[i] *= factor;
+  if ((e = SET_FIELD(h, box, w, w))) return e;
+  int zero = 0;
+  return SET_FIELD(h, box, tables_valid, zero);
 }
 
 int pmc_gpu_get_part_steps(pmc_gpu *h, int64_t *part_steps) {

@@ -40,6 +40,8 @@ constexpr uint32_t RNG_ACC_SAMPLE = 5;
 constexpr uint32_t RNG_ACC_FINISH = 6;
 constexpr uint32_t RNG_ACC_TARGET = 7;
 constexpr uint32_t RNG_HALVE = 8;
+constexpr uint32_t RNG_DILUTE = 9;
+constexpr uint32_t RNG_LOSS = 10;
 
 constexpr double C_PI = 3.14159265358979323846;
 constexpr double C_BOLTZMANN = 1.3806505e-23;

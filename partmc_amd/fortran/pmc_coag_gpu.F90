@@ -167,6 +167,42 @@ module pmc_coag_gpu
        real(c_double), intent(out) :: hist(*)
      end function pmc_gpu_size_dist
 
+     integer(c_int) function pmc_gpu_dilute(h, sample_prob) &
+          bind(c, name="pmc_gpu_dilute")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: sample_prob(*)
+     end function pmc_gpu_dilute
+
+     integer(c_int) function pmc_gpu_particle_loss(h, loss_function, del_t, &
+          height) bind(c, name="pmc_gpu_particle_loss")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int), value :: loss_function
+       real(c_double), value :: del_t
+       real(c_double), intent(in) :: height(*)
+     end function pmc_gpu_particle_loss
+
+     integer(c_int) function pmc_gpu_scale_weights(h, box, factor) &
+          bind(c, name="pmc_gpu_scale_weights")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int), value :: box
+       real(c_double), value :: factor
+     end function pmc_gpu_scale_weights
+
+     integer(c_int) function pmc_gpu_append(h, box, n, vol, id, group, klass, &
+          n_primary, n_comp, comp_source, comp_time, least, greatest) &
+          bind(c, name="pmc_gpu_append")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int), value :: box, n
+       real(c_double), intent(in) :: vol(*)
+       type(c_ptr), value :: id
+       integer(c_int), intent(in) :: group(*), klass(*)
+       type(c_ptr), value :: n_primary, n_comp, comp_source, comp_time, least, greatest
+     end function pmc_gpu_append
+
      integer(c_int) function pmc_gpu_clear_removed(h, box) &
           bind(c, name="pmc_gpu_clear_removed")
        import :: c_int, c_ptr

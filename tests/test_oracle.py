@@ -494,3 +494,88 @@ def test_rebalance_doubles_and_halves_to_the_ideal():
         # number concentration n * mag within 6 sigma of its expectation
         exp_n = before[c - 1] * m[0, c - 1] / 2 ** passes
         assert abs(n - exp_n) < 6 * math.sqrt(exp_n) + 1
+
+
+# ------------------------------------------------------------- sinks
+def test_dry_deposition_rate_physics():
+    """scenario_loss_rate_dry_dep: the rate is V_d / height; settling
+    dominates for large drops, where V_d approaches the Stokes velocity
+    rho g d^2 / (18 mu) (mu(288 K) = 1.79e-5 Pa s); the deposition
+    velocity has its minimum in the accumulation mode."""
+    rho, T, p = 1000.0, 288.0, 1e5
+    vol = lambda d: math.pi / 6 * d ** 3
+    r1 = oracle.loss_rate(oracle.LOSS_DRYDEP, vol(1e-6), rho, T, p, 1000.0)
+    r2 = oracle.loss_rate(oracle.LOSS_DRYDEP, vol(1e-6), rho, T, p, 250.0)
+    assert r2 == pytest.approx(4 * r1, rel=1e-14)
+    d = 200e-6
+    v_stokes = rho * 9.81 * d ** 2 / (18 * 1.79e-5)
+    vd = oracle.loss_rate(oracle.LOSS_DRYDEP, vol(d), rho, T, p, 1.0)
+    assert vd == pytest.approx(v_stokes, rel=0.02)
+    vdep = {dd: oracle.loss_rate(oracle.LOSS_DRYDEP, vol(dd), rho, T, p, 1.0)
+            for dd in (10e-9, 0.5e-6, 20e-6)}
+    assert vdep[0.5e-6] < vdep[10e-9] and vdep[0.5e-6] < vdep[20e-6]
+    # the other loss functions: constant 1e-3 s^-1 and 1e15 vol
+    assert oracle.loss_rate(oracle.LOSS_CONSTANT, vol(d), rho) == 1e-3
+    assert oracle.loss_rate(oracle.LOSS_VOLUME, 2e-18, rho) == pytest.approx(2e-3, rel=1e-15)
+    assert oracle.loss_rate(oracle.LOSS_NONE, 2e-18, rho) == 0.0
+    with pytest.raises(RuntimeError):
+        oracle.loss_rate(oracle.LOSS_CHAMBER, 2e-18, rho)
+
+
+@pytest.mark.parametrize("sort", [True, False])
+def test_dilution_removes_each_particle_with_its_probability(sort):
+    """Dilution out of the box: each particle leaves with probability p,
+    logged as AERO_INFO_DILUTION from the last index down in the swap-last
+    order; the weights are unchanged."""
+    ob = two_class_box(20000, 7, sort)
+    d0 = ob.particles()
+    ob.dilute(0.3, step=2)
+    d = ob.particles()
+    ids, act, oth = ob.info()
+    idx = {int(v): i for i, v in enumerate(d0["id"])}
+    removed = [idx[int(v)] for v in ids]
+    assert np.all(act == 1) and np.all(oth == 0)
+    assert removed == sorted(removed, reverse=True)
+    n = len(d0["id"])
+    assert abs(len(removed) - 0.3 * n) < 5 * math.sqrt(n * 0.3 * 0.7)
+    np.testing.assert_array_equal(d["id"], d0["id"][swap_last_order(n, set(removed))])
+    np.testing.assert_array_equal(ob.weight_magnitudes(), [[1e9, 3e8]])
+    if sort:
+        lists_consistent(ob)
+
+
+def test_particle_loss_probability_follows_the_rate():
+    """scenario_particle_loss with the volume loss function: a particle of
+    volume v leaves with probability 1 - exp(-dt 1e15 v)."""
+    ob = two_class_box(40000, 8)
+    d0 = ob.particles()
+    v = d0["vol"].sum(axis=1)
+    dt = 200.0
+    ob.particle_loss(oracle.LOSS_VOLUME, dt, 1000.0, step=5)
+    gone = set(ob.info()[0].tolist())
+    lost = np.array([int(i) in gone for i in d0["id"]])
+    p = 1.0 - np.exp(-dt * 1e15 * v)
+    for lo, hi in ((0.0, 0.1), (0.1, 0.5), (0.5, 1.0)):
+        sel = (p >= lo) & (p < hi)
+        if sel.sum() < 50:
+            continue
+        exp_n, var = p[sel].sum(), (p[sel] * (1 - p[sel])).sum()
+        assert abs(lost[sel].sum() - exp_n) < 5 * math.sqrt(var) + 1
+    lists_consistent(ob)
+
+
+def test_append_outside_the_grid_defers_the_remake():
+    """An appended particle outside the bin grid invalidates the sort; the
+    next step remakes the grid around every particle."""
+    rng = np.random.default_rng(9)
+    ob = oracle.Box(np.array([1000.0]), oracle.Weights([[1]], [[1e9]], [[0.0]]), seed=9)
+    ob.add_particles(10 ** rng.uniform(-24, -21, size=(500, 1)), np.arange(1, 501),
+                     np.ones(500), np.ones(500))
+    ob.next_id = 501
+    ob.sort_with_grid(50, 1e-10, 1e-5)
+    ob.add_particles(np.array([[4e-14]]), [501], [1], [1])  # r = 2.1e-5 m
+    ob.mc_coag(P.KERNEL_CONSTANT, 1e-3, 0)
+    e, _, _ = ob.grid()
+    r = radius(ob.particles()["vol"])
+    assert e[0] < r.min() and r.max() < e[-1]
+    lists_consistent(ob)
