@@ -281,15 +281,22 @@ class Box:
     def next_id(self, v):
         lib().orc_box_set_next_id(self._h, v)
 
-    def add_particles(self, vol, ids, group, klass, n_primary=None):
+    def add_particles(self, vol, ids, group, klass, n_primary=None, n_comp=None,
+                      comp_source=None, comp_time=None):
         vol = np.ascontiguousarray(vol, dtype=np.float64).reshape(-1, self.n_spec)
         n = vol.shape[0]
         ids = np.ascontiguousarray(ids, dtype=np.int64)
         g = np.ascontiguousarray(group, dtype=np.int32)
         c = np.ascontiguousarray(klass, dtype=np.int32)
         npr = None if n_primary is None else np.ascontiguousarray(n_primary, dtype=np.int32)
+        comps = (None, None, None)
+        if n_comp is not None:
+            comps = (np.ascontiguousarray(n_comp, dtype=np.int32),
+                     np.ascontiguousarray(comp_source, dtype=np.int32).reshape(n, MAX_COMP),
+                     np.ascontiguousarray(comp_time, dtype=np.float64).reshape(n, MAX_COMP))
         err = lib().orc_box_add_particles(self._h, n, _ptr(vol), _ptr(ids), _ptr(g), _ptr(c),
-                                          None if npr is None else _ptr(npr), None, None, None)
+                                          None if npr is None else _ptr(npr),
+                                          *[None if a is None else _ptr(a) for a in comps])
         if err:
             raise RuntimeError(f"oracle add error {err}")
 

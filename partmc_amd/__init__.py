@@ -164,7 +164,7 @@ class CoagGPU:
 
     # -- particles -----------------------------------------------------
     def upload(self, box, vol, ids, group, klass, n_primary=None, capacity=0,
-               n_comp=None, comp_source=None, comp_time=None):
+               n_comp=None, comp_source=None, comp_time=None, least=None, greatest=None):
         vol = np.ascontiguousarray(vol, dtype=np.float64).reshape(-1, self.n_spec)
         n = vol.shape[0]
         ids = np.ascontiguousarray(ids, dtype=np.int64)
@@ -174,9 +174,42 @@ class CoagGPU:
         nco = None if n_comp is None else np.ascontiguousarray(n_comp, dtype=np.int32)
         cs = None if comp_source is None else np.ascontiguousarray(comp_source, dtype=np.int32)
         ct = None if comp_time is None else np.ascontiguousarray(comp_time, dtype=np.float64)
+        le = None if least is None else np.ascontiguousarray(least, dtype=np.float64)
+        gr = None if greatest is None else np.ascontiguousarray(greatest, dtype=np.float64)
         _check(load().pmc_gpu_upload(self._h, box, n, capacity, _ptr(vol), _ptr(ids), _ptr(g),
                                      _ptr(c), _ptr(npr), _ptr(nco), _ptr(cs), _ptr(ct),
-                                     None, None))
+                                     _ptr(le), _ptr(gr)))
+
+    def save_state(self, box, filename, species_names=None, record_removals=True):
+        """Write the box as aero_state_output_netcdf lays it out (NetCDF
+        classic; see partmc_amd.state_io)."""
+        from .state_io import write_aero_state
+        d = self.download(box)
+        wt, wm, we = self.weights(box)
+        r = (3.0 * d["vol"].sum(axis=1) / 4.0 / np.pi) ** (1.0 / 3.0)
+        inv = np.zeros(len(r))
+        for gi in range(wt.shape[0]):
+            ci = d["klass"] - 1
+            w = wm[gi, ci] * np.where(wt[gi, ci] == 1, 1.0, r ** we[gi, ci])
+            inv += 1.0 / w
+        d["num_conc"] = 1.0 / inv if len(r) else inv
+        rem = self.removed(box) if record_removals else None
+        write_aero_state(filename, d, self.density, (wt, wm, we), species_names, rem)
+
+    def load_state(self, box, filename, capacity=0):
+        """Restart the box from a file written by save_state, as
+        aero_state_input_netcdf reads one: volumes are mass / density."""
+        from .state_io import read_aero_state
+        s = read_aero_state(filename)
+        self.set_weights(box, *s["weights"])
+        if "mass" not in s:
+            self.upload(box, np.zeros((0, self.n_spec)), np.zeros(0, np.int64), np.zeros(0),
+                        np.zeros(0), capacity=capacity)
+            return s
+        vol = s["mass"] / self.density[None, :]
+        self.upload(box, vol, s["id"], s["group"], s["klass"], s["n_primary"], capacity,
+                    s["n_comp"], s["comp_source"], s["comp_time"], s["least"], s["greatest"])
+        return s
 
     def set_bin_grid(self, box, n_bin, r_min, r_max):
         _check(load().pmc_gpu_set_bin_grid(self._h, box, n_bin, r_min, r_max))
@@ -197,13 +230,16 @@ class CoagGPU:
         nco = np.zeros(m, dtype=np.int32)
         cs = np.zeros((m, MAX_COMP), dtype=np.int32)
         ct = np.zeros((m, MAX_COMP))
+        le = np.zeros(m)
+        gr = np.zeros(m)
         cnt = np.zeros(1, dtype=np.int32)
         _check(load().pmc_gpu_download(self._h, box, m, _ptr(cnt), _ptr(vol), _ptr(ids), _ptr(g),
                                        _ptr(c), _ptr(npr), _ptr(nco), _ptr(cs), _ptr(ct),
-                                       None, None))
+                                       _ptr(le), _ptr(gr)))
         n = int(cnt[0])
         return dict(vol=vol[:n], id=ids[:n], group=g[:n], klass=c[:n], n_primary=npr[:n],
-                    n_comp=nco[:n], comp_source=cs[:n], comp_time=ct[:n])
+                    n_comp=nco[:n], comp_source=cs[:n], comp_time=ct[:n], least=le[:n],
+                    greatest=gr[:n])
 
     # -- the step --------------------------------------------------------
     def mc_coag(self, coag_kernel_type: int, del_t: float):
