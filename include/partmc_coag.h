@@ -216,10 +216,11 @@ int pmc_gpu_get_part_steps(pmc_gpu *h, int64_t *part_steps);
  * re-draw limit.  For the handle (rounds[3]): rounds[0] = device-wide
  * claim rounds run by the parallel mode, rounds[1] = tests it finished in
  * the per-box tail that follows them, rounds[2] = the tail's claim rounds
- * summed over boxes.  No counterpart in the reference, which has no
- * parallel mode. */
+ * summed over boxes.  box_tail (may be NULL) receives per box the tests
+ * finished in the tail and its claim rounds, [2 * b] and [2 * b + 1].  No
+ * counterpart in the reference, which has no parallel mode. */
 int pmc_gpu_get_path_stats(pmc_gpu *h, int64_t *n_accel_pairs, int64_t *n_dropped,
-                           int64_t *rounds);
+                           int64_t *rounds, int64_t *box_tail);
 /* Per-kernel timing with HIP events on the handle's stream: when on, every
  * launch of the named kernels is bracketed by events; totals accumulate
  * until the next pmc_gpu_set_timing call. */

@@ -40,7 +40,7 @@ ABI = {
     "pmc_gpu_get_next_id": (_i, [_p, _i, _p]),
     "pmc_gpu_reserve": (_i, [_p, _p]),
     "pmc_gpu_get_part_steps": (_i, [_p, _p]),
-    "pmc_gpu_get_path_stats": (_i, [_p, _p, _p, _p]),
+    "pmc_gpu_get_path_stats": (_i, [_p, _p, _p, _p, _p]),
     "pmc_gpu_size_dist": (_i, [_p, _i, _i, _i, _d, _d, _p]),
     "pmc_gpu_set_n_part_ideal": (_i, [_p, _i, _d]),
     "pmc_gpu_clear_removed": (_i, [_p, _i]),
@@ -342,9 +342,12 @@ class CoagGPU:
         acc = np.zeros(self.n_box, dtype=np.int64)
         drop = np.zeros(self.n_box, dtype=np.int64)
         rounds = np.zeros(3, dtype=np.int64)
-        _check(load().pmc_gpu_get_path_stats(self._h, _ptr(acc), _ptr(drop), _ptr(rounds)))
+        tail = np.zeros((self.n_box, 2), dtype=np.int64)
+        _check(load().pmc_gpu_get_path_stats(self._h, _ptr(acc), _ptr(drop), _ptr(rounds),
+                                             _ptr(tail)))
         return {"n_accel_pairs": acc, "n_dropped": drop, "n_rounds": int(rounds[0]),
-                "n_tail": int(rounds[1]), "n_tail_rounds": int(rounds[2])}
+                "n_tail": int(rounds[1]), "n_tail_rounds": int(rounds[2]),
+                "box_tail_tests": tail[:, 0].copy(), "box_tail_rounds": tail[:, 1].copy()}
 
     def reserve(self, capacity):
         cap = np.ascontiguousarray(capacity, dtype=np.int64)

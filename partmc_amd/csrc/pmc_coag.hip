@@ -795,6 +795,7 @@ int pmc_gpu_reset_counters(pmc_gpu *h) {
     B.max_rel_err = 0.0;
     B.max_p_err = 0.0;
     B.n_rounds = B.n_dropped = 0;
+    B.n_tail_tests = 0;
     B.n_accel_pairs = 0;
     B.part_steps = 0;
   }
@@ -816,7 +817,7 @@ int pmc_gpu_get_error_stats(pmc_gpu *h, double *max_p_err) {
 }
 
 int pmc_gpu_get_path_stats(pmc_gpu *h, int64_t *n_accel_pairs, int64_t *n_dropped,
-                           int64_t *rounds) {
+                           int64_t *rounds, int64_t *box_tail) {
   if (!h || !n_accel_pairs || !n_dropped || !rounds)
     return fail(ERR_ARG, "get_path_stats: bad arguments");
   HIPCHECK(hipSetDevice(h->device));
@@ -826,6 +827,10 @@ int pmc_gpu_get_path_stats(pmc_gpu *h, int64_t *n_accel_pairs, int64_t *n_droppe
   for (int b = 0; b < h->n_box; ++b) {
     n_accel_pairs[b] = hb[b].n_accel_pairs;
     n_dropped[b] = hb[b].n_dropped;
+    if (box_tail) {
+      box_tail[2 * b] = hb[b].n_tail_tests;
+      box_tail[2 * b + 1] = hb[b].n_rounds;
+    }
   }
   rounds[0] = h->work.n_rounds;
   rounds[1] = h->work.n_tail;

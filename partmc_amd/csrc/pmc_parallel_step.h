@@ -52,6 +52,7 @@ struct PW {  // device view of the parallel work arrays
   int *key;         // [total_cap] list key, -1 for no key
   uint8_t *status;  // [total_cap]
   uint32_t *claim;  // [total_cap]
+  uint32_t *claim_wr;  // [total_cap] tail: earliest claim of a test that changes the particle
   uint32_t *dead_t; // [total_cap] test index of the winner that consumed it
   int *dead_r;      // [total_cap] claim round (step-wide count) of that winner
   float4 *terms;    // [total_cap]
@@ -1955,8 +1956,18 @@ constexpr int TF_EFFECT = 1, TF_ACC = 2, TF_RUN = 4, TF_WARN = 8;
 __device__ inline uint32_t claim_load(const PW &W, int p) {
   return __hip_atomic_load(&W.claim[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ inline uint32_t claim_wr_load(const PW &W, int p) {
+  return __hip_atomic_load(&W.claim_wr[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ inline void claim_release(const PW &W, int p) {
   __hip_atomic_store(&W.claim[p], UNCLAIMED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&W.claim_wr[p], UNCLAIMED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// a tail claim on a particle the test changes (removes, or a target it
+// merges into): it excludes every other claimant, earlier or later
+__device__ inline void claim_write(const PW &W, int p, uint32_t t) {
+  atomicMin(&W.claim[p], t);
+  atomicMin(&W.claim_wr[p], t);
 }
 
 // a phase boundary of the tail: earlier writes of the box's workgroup
@@ -1977,6 +1988,15 @@ __device__ inline void tail_barrier() {
 // that lost with its inputs unchanged keeps its outcome and claims again.
 // The window's earliest test always wins or is final, so every round makes
 // progress.
+// Claims distinguish reading from changing.  Every particle of an accepted
+// test is claimed in W.claim (earliest accessor); the particles the test
+// changes are also claimed in W.claim_wr (earliest writer).  A per-set
+// merge changes only the parents it removes: a parent that stays is read
+// and left as it was.  A test wins when it is the earliest accessor of
+// every particle it changes and no earlier test changes a particle it only
+// reads.  Tests that only read a particle therefore win together (the
+// large-weight particle a bin pair draws over and over, which the
+// weighting keeps), and a later test that changes it waits for them.
 __global__ void __launch_bounds__(TAIL_THREADS)
 k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pend, int type,
            uint64_t seed, uint32_t step, int precision, double band, double del_t,
@@ -2055,23 +2075,40 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
       }
     }
     const bool effect = fl & TF_EFFECT;
+    // the merge decisions of an accepted per-set test, from the particles as
+    // they are now: they say which parents it changes
+    int o_win = -1;
+    Stream ms;  // a per-set test's merge stream, after its decisions
+    if (effect && !is_acc) {
+      int c1, c2, b1, b2;
+      decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
+      const int nb = B->n_bin, nc = B->w.n_class;
+      const int cc = A.cc[table_index(A, box, nb, nc, b1, b2, c1, c2)];
+      ms.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), t - W.pair_exc[k], RNG_MERGE);
+      int g;
+      o_win = merge_decide(A, W, B, ms, p1, p2, c1, c2, cc, &g) | (g << 16);
+    }
     // 2. claims
     if (effect) {
-      atomicMin(&W.claim[p1], t);
-      if (is_acc) rm_walk(W, rec, [&](int q) { atomicMin(&W.claim[q], t); });
-      else atomicMin(&W.claim[p2], t);
+      if (is_acc) {
+        claim_write(W, p1, t);
+        rm_walk(W, rec, [&](int q) { claim_write(W, q, t); });
+      } else {
+        if (o_win & O_RM1) claim_write(W, p1, t);
+        else atomicMin(&W.claim[p1], t);
+        if (o_win & O_RM2) claim_write(W, p2, t);
+        else atomicMin(&W.claim[p2], t);
+      }
     }
     __syncthreads();
     // 3. resolve, and what each winner creates
     bool win = false;
-    int o_win = -1;
-    Stream ms;  // a per-set winner's merge stream, after its decisions
     int c_app = 0, c_id = 0, c_info = 0, c_hole = 0;
     if (effect) {
       if (is_acc) {
         win = claim_load(W, p1) == t;
         rm_walk(W, rec, [&](int q) { win = win && claim_load(W, q) == t; });
-        rd_walk(W, rec, [&](int q) { win = win && claim_load(W, q) >= t; });
+        rd_walk(W, rec, [&](int q) { win = win && claim_wr_load(W, q) >= t; });
         if (win) {
           const int died = (W.r_flags[rec] & RF_DIED) ? 1 : 0;
           c_app = W.r_ndup[rec] - W.r_ndead[rec];
@@ -2080,17 +2117,10 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
           c_hole = W.r_nrm[rec] + died;
         }
       } else {
-        win = claim_load(W, p1) == t && claim_load(W, p2) == t;
+        const int o = o_win & 0xFFFF;
+        win = ((o & O_RM1) ? claim_load(W, p1) == t : claim_wr_load(W, p1) > t) &&
+              ((o & O_RM2) ? claim_load(W, p2) == t : claim_wr_load(W, p2) > t);
         if (win) {
-          int c1, c2, b1, b2;
-          decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
-          const int nb = B->n_bin, nc = B->w.n_class;
-          const int cc = A.cc[table_index(A, box, nb, nc, b1, b2, c1, c2)];
-          ms.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), t - W.pair_exc[k],
-                  RNG_MERGE);
-          int g;
-          const int o = merge_decide(A, W, B, ms, p1, p2, c1, c2, cc, &g);
-          o_win = o | (g << 16);
           c_app = (o & O_APPEND) ? 1 : 0;
           c_id = (o & O_NEWID) ? 1 : 0;
           c_info = ((o & O_INFO1) ? 1 : 0) + ((o & O_INFO2) ? 1 : 0);
@@ -2198,7 +2228,11 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     atomicAdd((unsigned long long *)&B->step_coag, (unsigned long long)c_coag);
   }
   if (c_warn) atomicAdd((unsigned long long *)&B->n_warn, (unsigned long long)c_warn);
-  if (tid == 0 && n_rounds) atomicAdd(rounds_out, n_rounds);
+  if (tid == 0 && n_rounds) {
+    atomicAdd(rounds_out, n_rounds);
+    B->n_rounds += (int)n_rounds;
+    B->n_tail_tests += hi - lo;
+  }
 }
 
 // the pair of every pending test, found in parallel before the tail
@@ -2312,7 +2346,6 @@ __global__ void k_par_end(BoxDesc *boxes, int n_box) {
   BoxDesc *B = &boxes[box];
   B->valid_sort = 1;
   B->xlists_valid = 0;
-  B->n_rounds += 0;
 }
 
 // ------------------------------------------------------------ host side
@@ -2368,7 +2401,7 @@ inline int parallel_alloc_records(ParallelWork &W, int n_spec) {
 
 inline void parallel_free_arrays(ParallelWork &W) {
   PW &d = W.d;
-  void *ptrs[] = {d.key,  d.status, d.claim, d.terms, d.perm, d.tile_box, d.tile_start,
+  void *ptrs[] = {d.key,  d.status, d.claim, d.claim_wr, d.terms, d.perm, d.tile_box, d.tile_start,
                   d.tiles_in_box, d.hist_off, d.hist, d.hist_scan, d.lstart, d.lcnt, d.live,
                   d.pair_n, d.pair_exc, d.acc, d.acc_scan, d.tp1, d.tp2, d.tid, d.pend, d.ev_t,
                   d.ev_p1, d.ev_p2, d.ev_box, d.ev_out, d.f_app, d.f_id, d.f_info, d.f_hole,
@@ -2414,6 +2447,7 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.key, tc));
   PCHECK(pmalloc(&d.status, tc));
   PCHECK(pmalloc(&d.claim, tc));
+  PCHECK(pmalloc(&d.claim_wr, tc));
   PCHECK(pmalloc(&d.dead_t, tc));
   PCHECK(pmalloc(&d.dead_r, tc));
   PCHECK(pmalloc(&d.terms, tc));
@@ -2498,6 +2532,7 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
                                            (int)W.hole_cap));
   PCHECK(hipMalloc(&W.sort_tmp, W.sort_tmp_bytes));
   hipLaunchKernelGGL(k_fill_u32, dim3((tc + 255) / 256), dim3(256), 0, 0, d.claim, tc, UNCLAIMED);
+  hipLaunchKernelGGL(k_fill_u32, dim3((tc + 255) / 256), dim3(256), 0, 0, d.claim_wr, tc, UNCLAIMED);
   hipLaunchKernelGGL(k_fill_u8, dim3((tc + 255) / 256), dim3(256), 0, 0, d.status, tc, ST_LIVE);
   PCHECK(hipGetLastError());
   PCHECK(hipDeviceSynchronize());

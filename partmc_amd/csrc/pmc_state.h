@@ -39,7 +39,8 @@ struct BoxDesc {
   int grid_req_n_bin;        // explicit grid request (-1 = none)
   double grid_req_min, grid_req_max;
   int error;
-  int n_rounds, n_dropped;   // parallel-mode retry statistics
+  int n_rounds, n_dropped;   // parallel mode: tail claim rounds, dropped samples
+  int64_t n_tail_tests;      // parallel mode: tests finished in the tail
   double log_lo, log_hi;     // log(edges[0]), log(edges[n_bin])
   Weights w;
   Env env;
