@@ -708,6 +708,106 @@ __device__ inline void perset_test(BoxDesc *B, const Arrays &A, const PW &W, int
   o.warn = !(pr <= 1.0);
 }
 
+// outcome bits of ev_out
+constexpr int O_WIN = 1, O_RM1 = 2, O_RM2 = 4, O_NEW = 8, O_SLOT1 = 16, O_SLOT2 = 32,
+              O_APPEND = 64, O_ID1 = 128, O_ID2 = 256, O_NEWID = 512, O_INFO1 = 1024,
+              O_INFO2 = 2048;
+
+struct MergeCtx {
+  int box, c1, c2, cc, b1, b2;
+  uint32_t site, j;
+};
+
+__device__ inline MergeCtx merge_ctx(BoxDesc *boxes, const PW &W, int n_box, uint32_t t) {
+  MergeCtx m;
+  const int64_t NP = (int64_t)n_box * W.P_stride;
+  const int64_t k = find_pair(W.pair_exc, NP, t);
+  m.box = (int)(k / W.P_stride);
+  decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &m.c1, &m.c2, &m.b1, &m.b2);
+  m.j = t - W.pair_exc[k];
+  return m;
+}
+
+// coagulate_weighting (src/coagulation.F90:810-922): decisions only
+__device__ inline int merge_decide(const Arrays &A, const PW &W, const BoxDesc *B, Stream &s,
+                                   int64_t p1, int64_t p2, int c1, int c2, int cc,
+                                   int *new_group) {
+  const Weights &w = B->w;
+  double r1 = vol2rad(particle_volume(A, p1));
+  double r2 = vol2rad(particle_volume(A, p2));
+  double rc = vol2rad(rad2vol(r1) + rad2vol(r2));
+  double nc1 = nc_at_radius(w, c1, r1);
+  double nc2 = nc_at_radius(w, c2, r2);
+  double ncc = nc_at_radius(w, cc, rc);
+  double pdf[MAX_GROUP];
+  for (int g = 0; g < w.n_group; ++g) {
+    int idx = g + w.n_group * (cc - 1);
+    pdf[g] = 1.0 / weight_nc(w.type[idx], w.mag[idx], w.expo[idx], rc);
+  }
+  *new_group = sample_cts_pdf(s, w.n_group, pdf);
+  double nc_min = fmin(nc1, fmin(nc2, ncc));
+  bool rm1 = s.uniform() < nc_min / nc1;
+  bool rm2 = s.uniform() < nc_min / nc2;
+  bool cr = s.uniform() < nc_min / ncc;
+  int o = 0;
+  if (rm1) o |= O_RM1;
+  if (rm2) o |= O_RM2;
+  if (cr) o |= O_NEW;
+  bool id1_lost, id2_lost;
+  if (cr) {
+    id1_lost = false;
+    id2_lost = false;
+    if (rm1 && rm2) {
+      if (particle_volume(A, p1) > particle_volume(A, p2)) id2_lost = true;
+      else id1_lost = true;
+    }
+    if (rm1 && !id1_lost) o |= O_ID1;
+    else if (rm2 && !id2_lost) o |= O_ID2;
+    else o |= O_NEWID;
+    // the new particle takes a freed slot when there is one: the slot of
+    // the parent whose list it stays in, if any, so that it can still be
+    // drawn from that list in this step (in the reference it joins its
+    // bin's list)
+    const int nb = B->n_bin;
+    const int b_new = bin_find_log(nb, B->log_lo, B->log_hi, rc);
+    const int k_new = (b_new >= 1 && b_new <= nb) ? par_key(b_new, cc, nb) : -1;
+    if (rm1 && rm2) o |= (W.key[p2] == k_new && W.key[p1] != k_new) ? O_SLOT2 : O_SLOT1;
+    else if (rm1) o |= O_SLOT1;
+    else if (rm2) o |= O_SLOT2;
+    else o |= O_APPEND;
+  } else {
+    id1_lost = rm1;
+    id2_lost = rm2;
+  }
+  if (rm1 && id1_lost) o |= O_INFO1;
+  if (rm2 && id2_lost) o |= O_INFO2;
+  return o;
+}
+
+// A claim on a particle the test changes (removes, or a target it merges
+// into): recorded as the earliest accessor and the earliest writer.  A
+// particle a test only reads is claimed as accessor only: tests that read
+// it win together, a test that changes it waits for earlier readers, and a
+// reader waits for earlier writers.
+__device__ inline void claim_write(const PW &W, int p, uint32_t t) {
+  atomicMin(&W.claim[p], t);
+  atomicMin(&W.claim_wr[p], t);
+}
+
+// the merge decisions of an accepted per-set test (which parents it
+// removes), on the particles as they are before the round's winners apply
+__device__ inline int perset_decide(BoxDesc *B, const Arrays &A, const PW &W, int box, int64_t k,
+                                    uint32_t t, int p1, int p2, uint64_t seed, uint32_t step,
+                                    Stream &ms) {
+  int c1, c2, b1, b2;
+  decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
+  const int nb = B->n_bin, nc = B->w.n_class;
+  const int cc = A.cc[table_index(A, box, nb, nc, b1, b2, c1, c2)];
+  ms.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), t - W.pair_exc[k], RNG_MERGE);
+  int g;
+  return merge_decide(A, W, B, ms, p1, p2, c1, c2, cc, &g) | (g << 16);
+}
+
 // one thread per test: draw, evaluate, accept, claim
 __global__ void __launch_bounds__(256)
 k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, uint32_t step,
@@ -735,8 +835,12 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
   W.tns[i] = o.warn ? ACC_WARN : 0;  // counted once the test is final
   if (o.accept) {
     W.acc[i] = 1;
-    atomicMin(&W.claim[o.p1], t);
-    atomicMin(&W.claim[o.p2], t);
+    Stream ms;
+    const int d = perset_decide(B, A, W, box, k, t, o.p1, o.p2, seed, step, ms);
+    if (d & O_RM1) claim_write(W, o.p1, t);
+    else atomicMin(&W.claim[o.p1], t);
+    if (d & O_RM2) claim_write(W, o.p2, t);
+    else atomicMin(&W.claim[o.p2], t);
   }
 }
 
@@ -1042,7 +1146,7 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
         if (smp.uniform() < prob_remove_i / prm) {
           if (!rm_push(W, chain, ps)) return ERR_CAPACITY;
           n_rm_here++;
-          if (claim) atomicMin(&W.claim[ps], t);
+          if (claim) claim_write(W, ps, t);
         }
       }
     }
@@ -1282,7 +1386,7 @@ __device__ __noinline__ int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &
   W.r_dhead[rec] = dup_head;
   W.r_flags[rec] = flags;
   W.r_err[rec] = err_code;
-  if (claim) atomicMin(&W.claim[p], t);
+  if (claim) claim_write(W, p, t);
   return 1;
 }
 
@@ -1355,82 +1459,6 @@ __global__ void k_par_gather(PW W, int n) {
   W.ev_rec[e] = W.trec[i];
 }
 
-// outcome bits of ev_out
-constexpr int O_WIN = 1, O_RM1 = 2, O_RM2 = 4, O_NEW = 8, O_SLOT1 = 16, O_SLOT2 = 32,
-              O_APPEND = 64, O_ID1 = 128, O_ID2 = 256, O_NEWID = 512, O_INFO1 = 1024,
-              O_INFO2 = 2048;
-
-struct MergeCtx {
-  int box, c1, c2, cc, b1, b2;
-  uint32_t site, j;
-};
-
-__device__ inline MergeCtx merge_ctx(BoxDesc *boxes, const PW &W, int n_box, uint32_t t) {
-  MergeCtx m;
-  const int64_t NP = (int64_t)n_box * W.P_stride;
-  const int64_t k = find_pair(W.pair_exc, NP, t);
-  m.box = (int)(k / W.P_stride);
-  decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &m.c1, &m.c2, &m.b1, &m.b2);
-  m.j = t - W.pair_exc[k];
-  return m;
-}
-
-// coagulate_weighting (src/coagulation.F90:810-922): decisions only
-__device__ inline int merge_decide(const Arrays &A, const PW &W, const BoxDesc *B, Stream &s,
-                                   int64_t p1, int64_t p2, int c1, int c2, int cc,
-                                   int *new_group) {
-  const Weights &w = B->w;
-  double r1 = vol2rad(particle_volume(A, p1));
-  double r2 = vol2rad(particle_volume(A, p2));
-  double rc = vol2rad(rad2vol(r1) + rad2vol(r2));
-  double nc1 = nc_at_radius(w, c1, r1);
-  double nc2 = nc_at_radius(w, c2, r2);
-  double ncc = nc_at_radius(w, cc, rc);
-  double pdf[MAX_GROUP];
-  for (int g = 0; g < w.n_group; ++g) {
-    int idx = g + w.n_group * (cc - 1);
-    pdf[g] = 1.0 / weight_nc(w.type[idx], w.mag[idx], w.expo[idx], rc);
-  }
-  *new_group = sample_cts_pdf(s, w.n_group, pdf);
-  double nc_min = fmin(nc1, fmin(nc2, ncc));
-  bool rm1 = s.uniform() < nc_min / nc1;
-  bool rm2 = s.uniform() < nc_min / nc2;
-  bool cr = s.uniform() < nc_min / ncc;
-  int o = 0;
-  if (rm1) o |= O_RM1;
-  if (rm2) o |= O_RM2;
-  if (cr) o |= O_NEW;
-  bool id1_lost, id2_lost;
-  if (cr) {
-    id1_lost = false;
-    id2_lost = false;
-    if (rm1 && rm2) {
-      if (particle_volume(A, p1) > particle_volume(A, p2)) id2_lost = true;
-      else id1_lost = true;
-    }
-    if (rm1 && !id1_lost) o |= O_ID1;
-    else if (rm2 && !id2_lost) o |= O_ID2;
-    else o |= O_NEWID;
-    // the new particle takes a freed slot when there is one: the slot of
-    // the parent whose list it stays in, if any, so that it can still be
-    // drawn from that list in this step (in the reference it joins its
-    // bin's list)
-    const int nb = B->n_bin;
-    const int b_new = bin_find_log(nb, B->log_lo, B->log_hi, rc);
-    const int k_new = (b_new >= 1 && b_new <= nb) ? par_key(b_new, cc, nb) : -1;
-    if (rm1 && rm2) o |= (W.key[p2] == k_new && W.key[p1] != k_new) ? O_SLOT2 : O_SLOT1;
-    else if (rm1) o |= O_SLOT1;
-    else if (rm2) o |= O_SLOT2;
-    else o |= O_APPEND;
-  } else {
-    id1_lost = rm1;
-    id2_lost = rm2;
-  }
-  if (rm1 && id1_lost) o |= O_INFO1;
-  if (rm2 && id2_lost) o |= O_INFO2;
-  return o;
-}
-
 __global__ void k_par_resolve(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, uint64_t seed,
                               uint32_t step) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1445,7 +1473,7 @@ __global__ void k_par_resolve(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, 
     // ... and no earlier event claimed a source it read
     bool win = W.claim[p1] == t;
     rm_walk(W, r, [&](int q) { win = win && W.claim[q] == t; });
-    rd_walk(W, r, [&](int q) { win = win && W.claim[q] >= t; });
+    rd_walk(W, r, [&](int q) { win = win && W.claim_wr[q] >= t; });
     const int died = (W.r_flags[r] & RF_DIED) ? 1 : 0;
     W.ev_out[e] = win ? O_WIN : 0;
     W.f_app[e] = win ? W.r_ndup[r] - W.r_ndead[r] : 0;
@@ -1455,17 +1483,19 @@ __global__ void k_par_resolve(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, 
     W.f_lose[e] = win ? 0 : 1;
     return;
   }
-  bool win = W.claim[p1] == t && W.claim[p2] == t;
+  // the decisions say which parents the merge changes (removes); a parent
+  // it keeps is only read
+  BoxDesc *B = &boxes[m.box];
+  const int nb = B->n_bin, nc = B->w.n_class;
+  int cc = A.cc[table_index(A, m.box, nb, nc, m.b1, m.b2, m.c1, m.c2)];
+  Stream s;
+  s.open(seed, m.box, step, pair_site(nb, nc, m.b1, m.b2, m.c1, m.c2), m.j, RNG_MERGE);
+  int g;
+  const int d = merge_decide(A, W, B, s, p1, p2, m.c1, m.c2, cc, &g);
+  const bool win = ((d & O_RM1) ? W.claim[p1] == t : W.claim_wr[p1] > t) &&
+                   ((d & O_RM2) ? W.claim[p2] == t : W.claim_wr[p2] > t);
   int o = 0;
-  if (win) {
-    BoxDesc *B = &boxes[m.box];
-    const int nb = B->n_bin, nc = B->w.n_class;
-    int cc = A.cc[table_index(A, m.box, nb, nc, m.b1, m.b2, m.c1, m.c2)];
-    Stream s;
-    s.open(seed, m.box, step, pair_site(nb, nc, m.b1, m.b2, m.c1, m.c2), m.j, RNG_MERGE);
-    int g;
-    o = merge_decide(A, W, B, s, p1, p2, m.c1, m.c2, cc, &g) | O_WIN | (g << 16);
-  }
+  if (win) o = d | O_WIN | (g << 16);
   W.ev_out[e] = o;
   W.f_app[e] = (o & O_APPEND) ? 1 : 0;
   W.f_id[e] = (o & O_NEWID) ? 1 : 0;
@@ -1883,9 +1913,17 @@ __global__ void k_par_release(PW W, int E) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   W.claim[W.ev_p1[e]] = UNCLAIMED;
+  W.claim_wr[W.ev_p1[e]] = UNCLAIMED;
   const int r = W.ev_rec[e];
-  if (r >= 0) rm_walk(W, r, [&](int q) { W.claim[q] = UNCLAIMED; });
-  else W.claim[W.ev_p2[e]] = UNCLAIMED;
+  if (r >= 0) {
+    rm_walk(W, r, [&](int q) {
+      W.claim[q] = UNCLAIMED;
+      W.claim_wr[q] = UNCLAIMED;
+    });
+  } else {
+    W.claim[W.ev_p2[e]] = UNCLAIMED;
+    W.claim_wr[W.ev_p2[e]] = UNCLAIMED;
+  }
 }
 
 // Tests of the round that must be drawn again: accepted tests that lost a
@@ -1963,12 +2001,7 @@ __device__ inline void claim_release(const PW &W, int p) {
   __hip_atomic_store(&W.claim[p], UNCLAIMED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_store(&W.claim_wr[p], UNCLAIMED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-// a tail claim on a particle the test changes (removes, or a target it
-// merges into): it excludes every other claimant, earlier or later
-__device__ inline void claim_write(const PW &W, int p, uint32_t t) {
-  atomicMin(&W.claim[p], t);
-  atomicMin(&W.claim_wr[p], t);
-}
+
 
 // a phase boundary of the tail: earlier writes of the box's workgroup
 // become visible to its next phase.  Only this workgroup touches the box,
