@@ -122,3 +122,43 @@ def test_doubling_beyond_capacity_fails_before_any_change():
     np.testing.assert_array_equal(g.weights(0)[1], [[pop.weight_magnitude]])
     with pytest.raises(CoagError):
         gpu_boxes([pop], 3).rebalance()  # no n_part_ideal
+
+
+def test_exact_two_classes_halve_one_double_the_other():
+    """Weight classes are rebalanced separately: with 90% of the particles
+    in class 1 and an ideal of 0.3 n per class, class 1 halves and class 2
+    doubles in the same call; the state and the steps after it match the
+    oracle."""
+    from partmc_amd import CoagGPU
+    rng = np.random.default_rng(12)
+    n = 6000
+    vol = 10 ** rng.uniform(-24, -21, size=(n, 1)) * rng.dirichlet([1, 1], size=n)
+    klass = np.where(rng.uniform(size=n) < 0.9, 1, 2).astype(np.int32)
+    dens = np.array([1800.0, 1000.0])
+    wt, wm, we = [[1, 1]], [[3e8, 1e8]], [[0.0, 0.0]]
+    ob = oracle.Box(dens, oracle.Weights(wt, wm, we), seed=12)
+    ob.add_particles(vol, np.arange(1, n + 1), np.ones(n), klass)
+    ob.next_id = n + 1
+    ob.sort_with_grid(60, 1e-10, 1e-5)
+    g = CoagGPU(1, dens, max_bin=256)
+    g.set_rng(12, 0)
+    g.set_weights(0, wt, wm, we)
+    g.upload(0, vol, np.arange(1, n + 1), np.ones(n), klass, capacity=4 * n)
+    g.set_next_id(0, n + 1)
+    g.set_bin_grid(0, 60, 1e-10, 1e-5)
+    for s in range(2):
+        os_, oc = ob.mc_coag(P.KERNEL_BROWN, 10.0, s)
+        gs, gc = g.mc_coag(P.KERNEL_BROWN, 10.0)
+        assert (os_, oc) == (int(gs[0]), int(gc[0]))
+    ob.rebalance(0.6 * ob.n_part, step=2)
+    g.set_n_part_ideal(0, 0.6 * g.n_part(0))
+    g.rebalance()
+    compare_state(ob, g, n_class=2, lists=False)
+    same_weights(ob, g)
+    m = ob.weight_magnitudes()
+    assert m[0, 0] > 3e8 and m[0, 1] < 1e8  # class 1 halved, class 2 doubled
+    for s in range(2, 4):
+        os_, oc = ob.mc_coag(P.KERNEL_BROWN, 10.0, s)
+        gs, gc = g.mc_coag(P.KERNEL_BROWN, 10.0)
+        assert (os_, oc) == (int(gs[0]), int(gc[0]))
+    compare_state(ob, g, n_class=2)
