@@ -105,6 +105,7 @@ struct PW {  // device view of the parallel work arrays
 constexpr int ACC_CAP = 1 << 18;
 constexpr int DUP_CAP = 1 << 18;
 constexpr int RM_CHUNK = 16;
+constexpr int CHAIN_WAVE_MIN = 32;  // removal chains longer than this are applied by a wavefront
 constexpr int RM_CAP = 1 << 24;
 constexpr int ACC_WARN = 1 << 30;
 constexpr int READ_TRACK = 128;  // source lists up to this size have their reads tracked
@@ -1574,6 +1575,78 @@ struct RankAlloc {
   }
 };
 
+// The round's allocator: counters kept in the lane and summed over the
+// wavefront per box before they reach the box (thousands of winners of one
+// box would otherwise serialise on the same counter words)
+struct ApplyAlloc : RankAlloc {
+  int64_t n_samp, n_coag, n_warn;
+  int lk[3], ld[3], nl;  // the first live-count changes, summed per wavefront later
+  __device__ void count(int64_t samp, int64_t coag, int warn) {
+    n_samp += samp;
+    n_coag += coag;
+    n_warn += warn;
+  }
+  __device__ void live_add(int key, int d) {
+    if (nl < 3) {
+      lk[nl] = key;
+      ld[nl] = d;
+      ++nl;
+    } else {
+      atomicAdd(&live[key], d);
+    }
+  }
+};
+
+// Sum live-count changes over the wavefront per (list, key) and add each
+// sum once; key < 0: nothing from this lane.  Every lane takes part.
+__device__ inline void wave_flush_live(int *live, int key, int d) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t addr = key >= 0 ? (uint64_t)(live + key) : 0ull;
+  uint64_t pending = __ballot(key >= 0 && d != 0);
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint64_t la = __shfl(addr, leader);
+    const bool mine = ((pending >> lane) & 1ull) && addr == la;
+    int td = mine ? d : 0;
+    for (int off = 32; off > 0; off >>= 1) td += __shfl_xor(td, off);
+    if (lane == leader && td) atomicAdd((int *)la, td);
+    pending &= ~__ballot(mine);
+  }
+}
+
+// Sum each box's counter changes over the wavefront and add them with one
+// set of atomics per box (box < 0: nothing from this lane).  Every lane of
+// the wavefront takes part.
+__device__ inline void wave_flush_counts(BoxDesc *boxes, int box, int64_t s, int64_t c,
+                                         int64_t w) {
+  const int lane = threadIdx.x & 63;
+  uint64_t pending = __ballot(box >= 0 && (s | c | w));
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const int lb = __shfl(box, leader);
+    const bool mine = ((pending >> lane) & 1ull) && box == lb;
+    long long ts = mine ? s : 0, tc = mine ? c : 0, tw = mine ? w : 0;
+    for (int off = 32; off > 0; off >>= 1) {
+      ts += __shfl_xor(ts, off);
+      tc += __shfl_xor(tc, off);
+      tw += __shfl_xor(tw, off);
+    }
+    if (lane == leader) {
+      BoxDesc *B = &boxes[lb];
+      if (ts) {
+        atomicAdd((unsigned long long *)&B->n_samp, (unsigned long long)ts);
+        atomicAdd((unsigned long long *)&B->step_samp, (unsigned long long)ts);
+      }
+      if (tc) {
+        atomicAdd((unsigned long long *)&B->n_coag, (unsigned long long)tc);
+        atomicAdd((unsigned long long *)&B->step_coag, (unsigned long long)tc);
+      }
+      if (tw) atomicAdd((unsigned long long *)&B->n_warn, (unsigned long long)tw);
+    }
+    pending &= ~__ballot(mine);
+  }
+}
+
 // The tail's allocator: ranks as in a round, live-count changes gathered
 // in the workgroup's shared memory and counters in the lane, both flushed
 // once (many winners of one box would otherwise contend on the same words)
@@ -1801,7 +1874,8 @@ __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, in
 // copy that kept no copy of itself is removed with a weighting record.
 template <class Alloc>
 __device__ __noinline__ void acc_apply(BoxDesc *B, const Arrays &A, const PW &W, int box, uint32_t t,
-                                 int r, int64_t p, int type, int want_terms, int rid, Alloc &al) {
+                                 int r, int64_t p, int type, int want_terms, int rid, Alloc &al,
+                                 bool chain_elsewhere = false) {
   const int ns = A.n_spec;
   const int nb = B->n_bin;
   const int64_t target_id = A.id[p];
@@ -1814,19 +1888,26 @@ __device__ __noinline__ void acc_apply(BoxDesc *B, const Arrays &A, const PW &W,
     if (kk == 0) id_first = v;
   }
   auto copy_id = [&](int kk) { return id_first + (int64_t)(kk - 1) * B->id_stride; };
-  int64_t owner = target_id;
-  chain_walk(W, r, [&](int q) {
-    if (q < 0) {
-      owner = copy_id(-1 - q);
-      return;
-    }
-    W.dead_t[q] = t;
-    W.dead_r[q] = rid;
-    al.live_add(W.key[q], -1);
-    W.status[q] = ST_HOLE;
-    W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(q - B->off);
-    put_info(A, B, al, A.id[q], 2, owner);
-  });
+  if (chain_elsewhere) {
+    // the removed sources are applied by a wavefront (k_par_apply_chains)
+    // at this event's first hole and record ranks
+    al.hole += W.r_nrm[r];
+    al.info += W.r_nrm[r];
+  } else {
+    int64_t owner = target_id;
+    chain_walk(W, r, [&](int q) {
+      if (q < 0) {
+        owner = copy_id(-1 - q);
+        return;
+      }
+      W.dead_t[q] = t;
+      W.dead_r[q] = rid;
+      al.live_add(W.key[q], -1);
+      W.status[q] = ST_HOLE;
+      W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(q - B->off);
+      put_info(A, B, al, A.id[q], 2, owner);
+    });
+  }
   const unsigned long long n_samp = (unsigned long long)W.r_nsamp[r];
   const unsigned long long n_coag = (unsigned long long)W.r_ncoag[r];
   al.count((int64_t)n_samp, (int64_t)n_coag, (flags & RF_WARN) ? 1 : 0);
@@ -1864,25 +1945,109 @@ __device__ __noinline__ void acc_apply(BoxDesc *B, const Arrays &A, const PW &W,
 }
 
 // apply the round's winners, each at the positions its ranks give
-__global__ void k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, uint64_t seed,
-                            uint32_t step, int type, int want_terms, int64_t hole_base, int rid) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256)
+k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, uint64_t seed, uint32_t step,
+            int type, int want_terms, int64_t hole_base, int rid) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int box = -1;
+  int64_t cs = 0, cc = 0, cw = 0;
+  int nl = 0, lk[3] = {0, 0, 0}, ld[3] = {0, 0, 0};
+  int *live = W.live;
+  if (e < E && (W.ev_out[e] & O_WIN)) {
+    const uint32_t t = W.ev_t[e];
+    box = W.ev_box[e];
+    BoxDesc *B = &boxes[box];
+    const int first = W.box_first[box];
+    ApplyAlloc al;
+    al.B = B;
+    al.app = W.s_app[e] - W.s_app[first];
+    al.id = W.s_id[e] - W.s_id[first];
+    al.info = W.s_info[e] - W.s_info[first];
+    al.hole = hole_base + W.s_hole[e];
+    al.live = W.live + (int64_t)box * W.K_stride;
+    al.n_samp = al.n_coag = al.n_warn = 0;
+    al.nl = 0;
+    if (W.ev_rec[e] >= 0) {
+      acc_apply(B, A, W, box, t, W.ev_rec[e], W.ev_p1[e], type, want_terms, rid, al,
+                W.r_nchain[W.ev_rec[e]] > CHAIN_WAVE_MIN);
+    } else {
+      const int64_t k = find_pair(W.pair_exc, (int64_t)n_box * W.P_stride, t);
+      perset_merge(B, A, W, box, k, t, W.ev_p1[e], W.ev_p2[e], seed, step, type, want_terms, rid,
+                   al, W.ev_out[e] & ~O_WIN);
+    }
+    cs = al.n_samp;
+    cc = al.n_coag;
+    cw = al.n_warn;
+    nl = al.nl;
+    for (int i = 0; i < 3; ++i) {
+      lk[i] = al.lk[i];
+      ld[i] = al.ld[i];
+    }
+    live = al.live;
+  }
+  wave_flush_counts(boxes, box, cs, cc, cw);
+  for (int i = 0; i < 3; ++i) wave_flush_live(live, i < nl ? lk[i] : -1, i < nl ? ld[i] : 0);
+}
+
+// The removed sources of the round's winning target events with long
+// removal chains, one wavefront per event: each chunk's entries go to
+// lanes, the ranks of holes and removal records follow the chain order (a
+// ballot count of the entries before a lane), and each record names the
+// particle that took the source: the target, or the copy named by the last
+// copy marker before it in the chain (a ballot of the markers).
+__global__ void __launch_bounds__(256)
+k_par_apply_chains(BoxDesc *boxes, Arrays A, PW W, int E, int64_t hole_base, int rid) {
+  const int lane = threadIdx.x & 63;
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (e >= E) return;
   if (!(W.ev_out[e] & O_WIN)) return;
+  const int r = W.ev_rec[e];
+  if (r < 0 || W.r_nchain[r] <= CHAIN_WAVE_MIN) return;
   const uint32_t t = W.ev_t[e];
   const int box = W.ev_box[e];
   BoxDesc *B = &boxes[box];
   const int first = W.box_first[box];
-  RankAlloc al{B, W.s_app[e] - W.s_app[first], W.s_id[e] - W.s_id[first],
-               W.s_info[e] - W.s_info[first], hole_base + W.s_hole[e],
-               W.live + (int64_t)box * W.K_stride};
-  if (W.ev_rec[e] >= 0) {
-    acc_apply(B, A, W, box, t, W.ev_rec[e], W.ev_p1[e], type, want_terms, rid, al);
-    return;
+  const int64_t hole0 = hole_base + W.s_hole[e];
+  const int64_t info0 = B->n_info + (W.s_info[e] - W.s_info[first]);
+  const int64_t id_first =
+      (B->next_id + (W.s_id[e] - W.s_id[first]) - 1) * B->id_stride + B->id_offset + 1;
+  int *live = W.live + (int64_t)box * W.K_stride;
+  int64_t owner = A.id[W.ev_p1[e]];
+  int done = 0;  // removed sources before this chunk
+  int left = W.r_nchain[r];
+  for (int ch = W.r_head[r]; ch >= 0 && left > 0; ch = W.rm_buf[ch]) {
+    const int m = left < RM_CHUNK - 1 ? left : RM_CHUNK - 1;
+    const bool here = lane < m;
+    const int q = here ? W.rm_buf[ch + 1 + lane] : 0;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint64_t marks = __ballot(here && q < 0);
+    const uint64_t rems = __ballot(here && q >= 0);
+    // the owner of this lane's entry: the last marker at or before it
+    const uint64_t mk = marks & (below | (1ull << lane));
+    // every lane shuffles (no divergence around the cross-lane read)
+    const int src = mk ? 63 - __clzll((long long)mk) : lane;
+    const int qm = __shfl(q, src);
+    const int64_t own = mk ? id_first + (int64_t)(-1 - qm - 1) * B->id_stride : owner;
+    if (here && q >= 0) {
+      const int ord = done + __popcll(rems & below);
+      W.dead_t[q] = t;
+      W.dead_r[q] = rid;
+      atomicAdd(&live[W.key[q]], -1);
+      W.status[q] = ST_HOLE;
+      W.holes[hole0 + ord] = ((uint64_t)box << 32) | (uint64_t)(q - B->off);
+      const int64_t ki = info0 + ord;
+      if (ki < B->info_cap) {
+        A.info_id[B->info_off + ki] = A.id[q];
+        A.info_action[B->info_off + ki] = 2;
+        A.info_other[B->info_off + ki] = own;
+      }
+    }
+    const int last = marks ? 63 - __clzll((long long)marks) : 0;
+    const int ql = __shfl(q, last);
+    if (marks) owner = id_first + (int64_t)(-1 - ql - 1) * B->id_stride;
+    done += __popcll(rems);
+    left -= m;
   }
-  const int64_t k = find_pair(W.pair_exc, (int64_t)n_box * W.P_stride, t);
-  perset_merge(B, A, W, box, k, t, W.ev_p1[e], W.ev_p2[e], seed, step, type, want_terms, rid, al,
-               W.ev_out[e] & ~O_WIN);
 }
 
 // per-box totals of a round: appended slots, new ids, records, winners
@@ -2723,6 +2888,8 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, fs[q] + E, 1, 0);
         PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, fs[q], ss[q], E + 1, st));
       }
+      hipLaunchKernelGGL(k_par_apply_chains, dim3((E + 3) / 4), dim3(256), 0, st, d_box, A, d, E,
+                         n_holes, (int)W.rid);
       PLAUNCH(W, "k_par_apply", k_par_apply, dim3((E + 255) / 256), dim3(256), 0, st, d_box, A, d,
               n_box, E, seed, step, type, want_terms, n_holes, (int)W.rid);
       hipLaunchKernelGGL(k_par_box_totals, gb, bb, 0, st, d_box, d, n_box, E);
