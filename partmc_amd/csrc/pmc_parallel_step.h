@@ -2151,7 +2151,7 @@ __device__ inline int lower_bound_u32(const uint32_t *v, int n, uint32_t x) {
   return lo;
 }
 
-constexpr int TAIL_THREADS = 256;
+constexpr int TAIL_THREADS = 512;
 constexpr int TF_EFFECT = 1, TF_ACC = 2, TF_RUN = 4, TF_WARN = 8;
 
 // claims inside one kernel are read and released with device-scope atomics,
