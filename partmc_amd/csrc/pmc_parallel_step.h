@@ -2439,6 +2439,26 @@ __global__ void k_pend_pairs(PW W, int64_t NP, const uint32_t *pend, int n) {
   if (i < n) W.pend_k[i] = (int)find_pair(W.pair_exc, NP, pend[i]);
 }
 
+// the largest bin count among the boxes (their pair space for this step);
+// also clears the two entries after that pair space, which the pair scans
+// read as their last input
+__global__ void __launch_bounds__(256) k_max_nbin(const BoxDesc *boxes, int n_box, uint32_t *out) {
+  using Red = hipcub::BlockReduce<int, 256>;
+  __shared__ typename Red::TempStorage tr;
+  int m = 1;
+  for (int b = threadIdx.x; b < n_box; b += 256)
+    if (!boxes[b].error) m = max(m, boxes[b].n_bin);
+  const int r = Red(tr).Reduce(m, hipcub::Max());
+  if (threadIdx.x == 0) out[0] = (uint32_t)r;
+}
+
+__global__ void k_clear_pair_ends(PW W, int64_t NP) {
+  if (threadIdx.x == 0) {
+    W.pair_n[NP] = 0;
+    W.pair_accn[NP] = 0;
+  }
+}
+
 __global__ void k_set_u64(unsigned long long *p, unsigned long long v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
@@ -2792,7 +2812,18 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
   int key_bits = 1;
   while ((1 << key_bits) <= d.K_stride) key_bits++;
   PLAUNCH(W, "k_par_scatter", k_par_scatter, gt, bt, 0, st, d_box, d, key_bits);
-  // 4. Poisson test counts per bin pair, laid out in reference order
+  // 4. Poisson test counts per bin pair, laid out in reference order.  The
+  // pair space spans the largest grid of this step, not the allocation's.
+  hipLaunchKernelGGL(k_max_nbin, dim3(1), dim3(256), 0, st, d_box, n_box, d.misc);
+  PCHECK(hipMemcpyAsync(W.h_u32 + 8, d.misc, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  PCHECK(hipStreamSynchronize(st));
+  {
+    const int mb = std::max(1, std::min((int)W.h_u32[8], max_bin));
+    d.max_bin = mb;
+    d.P_stride = (int64_t)d.tab_classes * d.tab_classes * mb * mb;
+    W.NP = (int64_t)n_box * d.P_stride;
+    hipLaunchKernelGGL(k_clear_pair_ends, dim3(1), dim3(64), 0, st, d, W.NP);
+  }
   PLAUNCH(W, "k_par_pairs", k_par_pairs, dim3((unsigned)((d.P_stride + 255) / 256), n_box),
           dim3(256), 0, st, d_box, A, d, del_t, seed, step);
   hipLaunchKernelGGL(k_par_acc_first, dim3((d.K_stride + 63) / 64, n_box), dim3(64), 0, st, d_box, d);
