@@ -390,45 +390,57 @@ __global__ void k_par_lists(BoxDesc *boxes, PW W, int n_box) {
   W.lcnt[(int64_t)box * W.K_stride + k] = (int)(e - s);
 }
 
-// stable scatter: the tile's keys are radix-sorted (stable) so each item's
-// rank among equal keys of the tile is its sorted position minus the
-// position of the first item with that key.
+// Stable scatter.  The tile's keys are loaded coalesced and transposed to a
+// blocked arrangement (item order = index order), radix-sorted (stable), and
+// each item's rank among equal keys of the tile is its sorted position minus
+// that of the key's first item.  The first position and the list offset of
+// every key present are kept in shared memory (dynamic, 2 * K_stride ints),
+// written by the first item of each run.
 __global__ void __launch_bounds__(TILE_THREADS)
 k_par_scatter(BoxDesc *boxes, PW W, int key_bits) {
+  using BLD = hipcub::BlockLoad<int, TILE_THREADS, TILE_ITEMS, hipcub::BLOCK_LOAD_TRANSPOSE>;
   using BRS = hipcub::BlockRadixSort<int, TILE_THREADS, TILE_ITEMS, int>;
-  __shared__ typename BRS::TempStorage tmp;
+  __shared__ union {
+    typename BLD::TempStorage load;
+    typename BRS::TempStorage sort;
+  } tmp;
   __shared__ int s_keys[TILE];
+  extern __shared__ int s_run[];  // [0, K): first sorted position; [K, 2K): list offset
   const int tile = blockIdx.x;
   const int box = W.tile_box[tile];
   BoxDesc *B = &boxes[box];
   const int n = B->error ? 0 : B->n_part;
   const int64_t base = B->off;
   const int sentinel = (1 << key_bits) - 1;
+  const int start = W.tile_start[tile];
+  const int valid = max(0, min(TILE, n - start));
   int keys[TILE_ITEMS], vals[TILE_ITEMS];
-  for (int it = 0; it < TILE_ITEMS; ++it) {
-    int i = W.tile_start[tile] + threadIdx.x * TILE_ITEMS + it;
-    vals[it] = i;
-    keys[it] = (i < n) ? W.key[base + i] : sentinel;
-  }
-  BRS(tmp).Sort(keys, vals, 0, key_bits);
+  BLD(tmp.load).Load(W.key + base + start, keys, valid, sentinel);
+  for (int it = 0; it < TILE_ITEMS; ++it) vals[it] = start + threadIdx.x * TILE_ITEMS + it;
+  __syncthreads();
+  BRS(tmp.sort).Sort(keys, vals, 0, key_bits);
   for (int it = 0; it < TILE_ITEMS; ++it) s_keys[threadIdx.x * TILE_ITEMS + it] = keys[it];
   __syncthreads();
-  const int t_in_box = W.tile_start[tile] / TILE;
+  const int K = W.K_stride;
+  const int t_in_box = start / TILE;
   const int nt = W.tiles_in_box[box];
   const int64_t o = W.hist_off[box];
   const uint32_t bbase = W.hist_scan[o];
   for (int it = 0; it < TILE_ITEMS; ++it) {
-    int pos = threadIdx.x * TILE_ITEMS + it;
-    int k = keys[it];
+    const int pos = threadIdx.x * TILE_ITEMS + it;
+    const int k = keys[it];
     if (k == sentinel) continue;
-    // first position of key k in the sorted tile (binary search)
-    int lo = 0, hi = pos;
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      if (s_keys[mid] < k) lo = mid + 1; else hi = mid;
+    if (pos == 0 || s_keys[pos - 1] != k) {
+      s_run[k] = pos;
+      s_run[K + k] = (int)(W.hist_scan[o + (int64_t)k * nt + t_in_box] - bbase);
     }
-    int rank = pos - lo;
-    uint32_t dst = W.hist_scan[o + (int64_t)k * nt + t_in_box] - bbase + (uint32_t)rank;
+  }
+  __syncthreads();
+  for (int it = 0; it < TILE_ITEMS; ++it) {
+    const int pos = threadIdx.x * TILE_ITEMS + it;
+    const int k = keys[it];
+    if (k == sentinel) continue;
+    const int dst = s_run[K + k] + (pos - s_run[k]);
     W.perm[base + dst] = (int)(base + vals[it]);
   }
 }
@@ -2793,6 +2805,16 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
   hipLaunchKernelGGL(k_par_zero_live, dim3(1, n_box), dim3(256), 0, st, d_box, d, n_box, 1);
   hipLaunchKernelGGL(k_par_rekey, gt, bt, 0, st, d_box, A, d, type, 0, 1);
   hipLaunchKernelGGL(k_par_finish_rekey, gb, bb, 0, st, d_box, n_box, type, 0, 1);
+  // The largest grid among the boxes this step: the pair space spans it
+  // (not the allocation's max_bin) and the sort keys need only its bits.
+  hipLaunchKernelGGL(k_max_nbin, dim3(1), dim3(256), 0, st, d_box, n_box, d.misc);
+  PCHECK(hipMemcpyAsync(W.h_u32 + 8, d.misc, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  PCHECK(hipStreamSynchronize(st));
+  const int mb_step = std::max(1, std::min((int)W.h_u32[8], max_bin));
+  d.max_bin = mb_step;
+  d.P_stride = (int64_t)d.tab_classes * d.tab_classes * mb_step * mb_step;
+  W.NP = (int64_t)n_box * d.P_stride;
+  hipLaunchKernelGGL(k_clear_pair_ends, dim3(1), dim3(64), 0, st, d, W.NP);
   // 2. majorant and pair tables (shared with the exact mode)
   {
     const int mb = max_bin;
@@ -2809,21 +2831,12 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
                                           (int)W.H, st));
   hipLaunchKernelGGL(k_par_lists, dim3((d.K_stride + 255) / 256, n_box), dim3(256), 0, st, d_box, d,
                      n_box);
+  // keys are below mb_step * classes; the sentinel 2^key_bits - 1 above them
   int key_bits = 1;
-  while ((1 << key_bits) <= d.K_stride) key_bits++;
-  PLAUNCH(W, "k_par_scatter", k_par_scatter, gt, bt, 0, st, d_box, d, key_bits);
-  // 4. Poisson test counts per bin pair, laid out in reference order.  The
-  // pair space spans the largest grid of this step, not the allocation's.
-  hipLaunchKernelGGL(k_max_nbin, dim3(1), dim3(256), 0, st, d_box, n_box, d.misc);
-  PCHECK(hipMemcpyAsync(W.h_u32 + 8, d.misc, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  PCHECK(hipStreamSynchronize(st));
-  {
-    const int mb = std::max(1, std::min((int)W.h_u32[8], max_bin));
-    d.max_bin = mb;
-    d.P_stride = (int64_t)d.tab_classes * d.tab_classes * mb * mb;
-    W.NP = (int64_t)n_box * d.P_stride;
-    hipLaunchKernelGGL(k_clear_pair_ends, dim3(1), dim3(64), 0, st, d, W.NP);
-  }
+  while ((1 << key_bits) <= mb_step * d.tab_classes) key_bits++;
+  PLAUNCH(W, "k_par_scatter", k_par_scatter, gt, bt, 2 * d.K_stride * sizeof(int), st, d_box, d,
+          key_bits);
+  // 4. Poisson test counts per bin pair, laid out in reference order
   PLAUNCH(W, "k_par_pairs", k_par_pairs, dim3((unsigned)((d.P_stride + 255) / 256), n_box),
           dim3(256), 0, st, d_box, A, d, del_t, seed, step);
   hipLaunchKernelGGL(k_par_acc_first, dim3((d.K_stride + 63) / 64, n_box), dim3(64), 0, st, d_box, d);
