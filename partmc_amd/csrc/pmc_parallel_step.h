@@ -41,7 +41,7 @@ constexpr int TILE_ITEMS = 16;
 constexpr int TILE = TILE_THREADS * TILE_ITEMS;
 constexpr uint32_t UNCLAIMED = 0xFFFFFFFFu;
 constexpr uint8_t ST_LIVE = 0, ST_HOLE = 1, ST_NEW = 2;
-constexpr int PAR_ROUNDS = 1;  // device-wide claim rounds before the per-box tail
+constexpr int PAR_ROUNDS = 2;  // device-wide claim rounds before the per-box tail
 static_assert(PAR_ROUNDS >= 1, "the tail runs on the pending list of a round");
 constexpr int MAX_REDRAW = 64;
 
@@ -99,6 +99,8 @@ struct PW {  // device view of the parallel work arrays
   uint32_t *misc;              // [4] batch bounds from k_batch_end
   unsigned long long *hole_top;  // [1] hole counter of the tail
   int *pend_k;                 // [T_cap] pair of each pending test entering the tail
+  uint32_t *tp_in, *tpair;     // [T_cap] pair of every test of a single-batch step
+  int tpair_ok;                // tpair is current for this step
   unsigned long long *tail_rounds;  // [1] claim rounds run by the tail, summed over boxes
 };
 
@@ -604,6 +606,19 @@ __device__ inline int64_t find_pair(const uint32_t *exc, int64_t n, uint32_t t) 
   return lo;
 }
 
+// the pair of test t: from the step's test-to-pair table when it covers
+// the step, else by binary search
+__device__ inline int64_t test_pair(const PW &W, int64_t NP, uint32_t t) {
+  return W.tpair_ok ? (int64_t)W.tpair[t] : find_pair(W.pair_exc, NP, t);
+}
+
+// the first test of every non-empty pair marks that pair; a max-scan then
+// gives every test its pair
+__global__ void k_mark_pair_starts(PW W, int64_t NP) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < NP && W.pair_n[k] > 0) W.tp_in[W.pair_exc[k]] = (uint32_t)k;
+}
+
 __device__ inline void atomic_max_double(double *addr, double v) {
   unsigned long long *a = (unsigned long long *)addr;
   unsigned long long nv = (unsigned long long)__double_as_longlong(v);
@@ -734,7 +749,7 @@ struct MergeCtx {
 __device__ inline MergeCtx merge_ctx(BoxDesc *boxes, const PW &W, int n_box, uint32_t t) {
   MergeCtx m;
   const int64_t NP = (int64_t)n_box * W.P_stride;
-  const int64_t k = find_pair(W.pair_exc, NP, t);
+  const int64_t k = test_pair(W, NP, t);
   m.box = (int)(k / W.P_stride);
   decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &m.c1, &m.c2, &m.b1, &m.b2);
   m.j = t - W.pair_exc[k];
@@ -830,7 +845,7 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
   if (i >= n) return;
   const uint32_t t = pend ? pend[i] : t0 + (uint32_t)i;
   const int64_t NP = (int64_t)n_box * W.P_stride;
-  const int64_t k = find_pair(W.pair_exc, NP, t);
+  const int64_t k = test_pair(W, NP, t);
   if (W.pair_acc[k]) return;  // a target slot: k_par_acc_tests
   W.acc[i] = 0;
   W.tid[i] = t;
@@ -1418,7 +1433,7 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
     if (idx >= n) return;
     i = idx;
     t = pend[i];
-    k = find_pair(W.pair_exc, NP, t);
+    k = test_pair(W, NP, t);
     if (!W.pair_acc[k]) return;
   } else {
     if (idx >= n_acc) return;
@@ -1443,7 +1458,7 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
 // records; out[0] = t1, out[1] = acc_count(t0), out[2] = acc_count(t1)
 __device__ inline uint32_t acc_count(const PW &W, int64_t NP, uint32_t T, uint32_t t) {
   if (t >= T) return W.acc_exc[NP];
-  int64_t k = find_pair(W.pair_exc, NP, t);
+  int64_t k = test_pair(W, NP, t);
   return W.acc_exc[k] + (W.pair_acc[k] ? t - W.pair_exc[k] : 0u);
 }
 
@@ -1452,6 +1467,7 @@ __global__ void k_batch_end(PW W, int64_t NP, uint32_t T, uint32_t t0, uint32_t 
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const uint32_t a0 = acc_count(W, NP, T, t0);
   uint32_t lo = t0 + 1, hi = (T - t0 < t_cap) ? T : t0 + t_cap;
+  if (acc_count(W, NP, T, hi) - a0 <= a_cap) lo = hi;  // the usual case: all of it fits
   while (lo < hi) {  // largest t1 in [lo, hi] with acc_count(t1) - a0 <= a_cap
     uint32_t mid = lo + (hi - lo + 1) / 2;
     if (acc_count(W, NP, T, mid) - a0 <= a_cap) lo = mid; else hi = mid - 1;
@@ -1983,7 +1999,7 @@ k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, uint64_t seed, uin
       acc_apply(B, A, W, box, t, W.ev_rec[e], W.ev_p1[e], type, want_terms, rid, al,
                 W.r_nchain[W.ev_rec[e]] > CHAIN_WAVE_MIN);
     } else {
-      const int64_t k = find_pair(W.pair_exc, (int64_t)n_box * W.P_stride, t);
+      const int64_t k = test_pair(W, (int64_t)n_box * W.P_stride, t);
       perset_merge(B, A, W, box, k, t, W.ev_p1[e], W.ev_p2[e], seed, step, type, want_terms, rid,
                    al, W.ev_out[e] & ~O_WIN);
     }
@@ -2129,7 +2145,7 @@ __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
     f = changed ? 1 : 0;
     int v = W.tns[i];
     if (!f && v) {
-      const int64_t k = find_pair(W.pair_exc, (int64_t)n_box * W.P_stride, t);
+      const int64_t k = test_pair(W, (int64_t)n_box * W.P_stride, t);
       BoxDesc *B = &boxes[k / W.P_stride];
       unsigned long long ns = (unsigned long long)(v & (ACC_WARN - 1));
       atomicAdd((unsigned long long *)&B->n_samp, ns);
@@ -2141,7 +2157,7 @@ __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
   }
   if (!f && W.trec[i] < 0 && (W.tns[i] & ACC_WARN)) {
     // a per-set test's k_max warning (src/coagulation.F90:749-751), once final
-    const int64_t k = find_pair(W.pair_exc, (int64_t)n_box * W.P_stride, t);
+    const int64_t k = test_pair(W, (int64_t)n_box * W.P_stride, t);
     atomicAdd((unsigned long long *)&boxes[k / W.P_stride].n_warn, 1ull);
   }
   W.f_pend[i] = f;
@@ -2448,7 +2464,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
 // the pair of every pending test, found in parallel before the tail
 __global__ void k_pend_pairs(PW W, int64_t NP, const uint32_t *pend, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) W.pend_k[i] = (int)find_pair(W.pair_exc, NP, pend[i]);
+  if (i < n) W.pend_k[i] = (int)test_pair(W, NP, pend[i]);
 }
 
 // the largest bin count among the boxes (their pair space for this step);
@@ -2639,7 +2655,7 @@ inline void parallel_free_arrays(ParallelWork &W) {
                   d.s_lose, d.box_first,
                   d.holes, d.holes_sorted, d.tail_live, W.scan_tmp, W.sort_tmp,
                   d.pair_acc, d.pair_accn, d.acc_exc, d.trec, d.tns, d.ev_rec, d.misc,
-                  d.hole_top, d.tail_rounds, d.pend_k};
+                  d.hole_top, d.tail_rounds, d.pend_k, d.tp_in, d.tpair};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   parallel_free_records(W);
@@ -2739,6 +2755,8 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.ev_out, T));
   PCHECK(pmalloc(&d.trec, T));
   PCHECK(pmalloc(&d.pend_k, T));
+  PCHECK(pmalloc(&d.tp_in, T));
+  PCHECK(pmalloc(&d.tpair, T));
   PCHECK(pmalloc(&d.tns, T));
   PCHECK(pmalloc(&d.ev_rec, T));
   int **fl[] = {&d.f_app, &d.f_id, &d.f_info, &d.f_hole, &d.f_lose, &d.f_pend,
@@ -2756,7 +2774,9 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, d.hist, d.hist_scan, (int)std::max<int64_t>(H, 1)));
   PCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, d.pair_n, d.pair_exc, (int)(W.NP + 1)));
   PCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, d.acc, d.acc_scan, (int)(T + 1)));
-  W.scan_tmp_bytes = std::max(b1, std::max(b2, b3));
+  size_t b4 = 0;
+  PCHECK(hipcub::DeviceScan::InclusiveScan(nullptr, b4, d.tp_in, d.tpair, hipcub::Max(), (int)T));
+  W.scan_tmp_bytes = std::max(std::max(b1, b4), std::max(b2, b3));
   PCHECK(hipMalloc(&W.scan_tmp, W.scan_tmp_bytes));
   PCHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, W.sort_tmp_bytes, d.holes, d.holes_sorted,
                                            (int)W.hole_cap));
@@ -2793,6 +2813,7 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
   const int want_terms = (precision != 0) ? 1 : 0;
   const dim3 gb((n_box + 63) / 64), bb(64);
   const dim3 gt(W.n_tiles), bt(TILE_THREADS);
+  d.tpair_ok = 0;  // built once the step's tests are laid out
   hipLaunchKernelGGL(k_step_begin, gb, bb, 0, st, d_box, n_box);
   // 1. keys for boxes with a valid grid but stale keys, radius extremes for
   // unsorted boxes, the remake test, and keys again after a new grid
@@ -2849,6 +2870,17 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
   PCHECK(hipStreamSynchronize(st));
   const uint32_t T = W.h_u32[0];
   const uint32_t T_acc = W.h_u32[1];
+  // the test-to-pair table, when the step is one batch
+  d.tpair_ok = 0;
+  if (T > 0 && (int64_t)T <= W.T_cap) {
+    hipLaunchKernelGGL(k_fill_u32, dim3((T + 255) / 256), dim3(256), 0, st, d.tp_in, (int64_t)T,
+                       0u);
+    hipLaunchKernelGGL(k_mark_pair_starts, dim3((unsigned)((W.NP + 255) / 256)), dim3(256), 0, st,
+                       d, W.NP);
+    PCHECK(hipcub::DeviceScan::InclusiveScan(W.scan_tmp, W.scan_tmp_bytes, d.tp_in, d.tpair,
+                                             hipcub::Max(), (int)T, st));
+    d.tpair_ok = 1;
+  }
   if (T_acc > 0 && d.r_nsamp == nullptr) {
     int e = parallel_alloc_records(W, A.n_spec);
     if (e) return e;
