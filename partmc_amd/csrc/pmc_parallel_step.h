@@ -2907,8 +2907,12 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
     const uint32_t batch_t0 = t0;
     t0 = t1;
     const uint32_t *pend = nullptr;
+    const int n_batch = n;
     for (int round = 0; n > 0; ++round) {
-      if (round >= PAR_ROUNDS) {
+      // more device-wide rounds only while they settle most of the batch:
+      // when over half of it is still pending, chains dominate and the tail
+      // takes them at once
+      if (round >= PAR_ROUNDS || (round >= 1 && n > n_batch / 2)) {
         // the rest per box, one wavefront each
         if (n_acc > 0) {
           hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.rm_top, 1, 0);
