@@ -114,14 +114,14 @@ static void timing_begin(pmc_gpu *h, const char *name, hipEvent_t *start) {
   (void)name;
   if (!h->timing) return;
   (void)hipEventCreate(start);
-  (void)hipEventRecord(*start, h->stream);
+  (void)hipEventRecord(*start, h->work.tst ? h->work.tst : h->stream);
 }
 
 static void timing_end(pmc_gpu *h, const char *name, hipEvent_t start) {
   if (!h->timing) return;
   hipEvent_t stop;
   (void)hipEventCreate(&stop);
-  (void)hipEventRecord(stop, h->stream);
+  (void)hipEventRecord(stop, h->work.tst ? h->work.tst : h->stream);
   h->pending.push_back({name, start, stop});
 }
 

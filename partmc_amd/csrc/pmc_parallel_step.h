@@ -132,6 +132,11 @@ struct ParallelWork {
   void (*tbegin)(void *, const char *, hipEvent_t *) = nullptr;
   void (*tend)(void *, const char *, hipEvent_t) = nullptr;
   void *tctx = nullptr;
+  hipStream_t tst = nullptr;  // stream of the launch being timed, when not the handle's
+  // a second stream (default priority) for the target walks, which run
+  // beside the per-set tests of the same round
+  hipStream_t st2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int64_t n_tail = 0;     // tests finished in the tail
   int64_t n_tail_rounds = 0;  // the tail's claim rounds, summed over boxes
   int64_t rid = 1;        // round identifiers: a fresh one for every claim round
@@ -2602,6 +2607,9 @@ inline int parallel_init(ParallelWork &W, int n_box, int max_bin) {
   if (hipHostMalloc((void **)&W.h_u32, 64 * sizeof(uint32_t)) != hipSuccess) return ERR_HIP;
   if (hipHostMalloc((void **)&W.h_i32, 64 * sizeof(int)) != hipSuccess) return ERR_HIP;
   if (hipHostMalloc((void **)&W.h_u64, 8 * sizeof(unsigned long long)) != hipSuccess) return ERR_HIP;
+  if (hipStreamCreate(&W.st2) != hipSuccess) return ERR_HIP;
+  if (hipEventCreateWithFlags(&W.ev_fork, hipEventDisableTiming) != hipSuccess) return ERR_HIP;
+  if (hipEventCreateWithFlags(&W.ev_join, hipEventDisableTiming) != hipSuccess) return ERR_HIP;
   return 0;
 }
 
@@ -2679,6 +2687,11 @@ inline void parallel_free(ParallelWork &W) {
   W.h_u64 = nullptr;
   W.h_u32 = nullptr;
   W.h_i32 = nullptr;
+  if (W.ev_fork) (void)hipEventDestroy(W.ev_fork);
+  if (W.ev_join) (void)hipEventDestroy(W.ev_join);
+  if (W.st2) (void)hipStreamDestroy(W.st2);
+  W.ev_fork = W.ev_join = nullptr;
+  W.st2 = nullptr;
 }
 
 // (re)allocate everything sized by particle capacities and the class count
@@ -2943,13 +2956,21 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.rm_top, 1, 0);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.dup_top, 1, 0);
       }
+      // the target walks run on the second stream beside the per-set tests
+      // (disjoint slots; both only claim with atomics)
+      if (n_acc > 0) {
+        PCHECK(hipEventRecord(W.ev_fork, st));
+        PCHECK(hipStreamWaitEvent(W.st2, W.ev_fork, 0));
+        const int na = pend ? n : n_acc;
+        W.tst = W.st2;
+        PLAUNCH(W, "k_par_acc_tests", k_par_acc_tests, dim3((na + 63) / 64), dim3(64), 0, W.st2,
+                d_box, A, d, n_box, type, seed, step, batch_t0, n, pend, del_t, a0, n_acc);
+        W.tst = nullptr;
+        PCHECK(hipEventRecord(W.ev_join, W.st2));
+      }
       PLAUNCH(W, "k_par_tests", k_par_tests, dim3((n + 255) / 256), dim3(256), 0, st, d_box, A, d,
               n_box, type, seed, step, batch_t0, n, pend, precision, band, validation);
-      if (n_acc > 0) {
-        const int na = pend ? n : n_acc;
-        PLAUNCH(W, "k_par_acc_tests", k_par_acc_tests, dim3((na + 63) / 64), dim3(64), 0, st,
-                d_box, A, d, n_box, type, seed, step, batch_t0, n, pend, del_t, a0, n_acc);
-      }
+      if (n_acc > 0) PCHECK(hipStreamWaitEvent(st, W.ev_join, 0));
       hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.acc + n, 1, 0);
       PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.acc, d.acc_scan, n + 1, st));
       PCHECK(hipMemcpyAsync(W.h_i32, d.acc_scan + n, sizeof(int), hipMemcpyDeviceToHost, st));
