@@ -70,7 +70,7 @@ struct PW {  // device view of the parallel work arrays
   int *s_app, *s_id, *s_info, *s_hole, *s_lose;
   int *box_first;
   uint64_t *holes, *holes_sorted;  // (box << 32 | slot)
-  int *tail_live;
+  int *cmp_lo, *cmp_hi, *cmp_head;  // [n_box] compaction: hole range and holes below the final count
   // accelerated per-particle path (try_per_particle_coag): one test slot per
   // target particle, one event record per target slot of a batch
   uint8_t *pair_acc;           // [NP] 0 per-set; 1 targets in (b1, c1); 2 targets in (b2, c2)
@@ -2518,70 +2518,62 @@ __global__ void k_step_begin(BoxDesc *boxes, int n_box) {
   B->part_steps += B->n_part;
 }
 
-// tail-fill compaction, one block per box: holes (sorted by slot) below the
-// final count receive the live particles of the tail, in slot order
-__global__ void __launch_bounds__(256)
-k_par_compact(BoxDesc *boxes, Arrays A, PW W, int n_holes) {
-  const int box = blockIdx.x;
-  BoxDesc *B = &boxes[box];
-  __shared__ int s_lo, s_hi, s_cnt;
-  if (threadIdx.x == 0) {
-    // hole range of this box in the sorted list
-    uint64_t key = (uint64_t)box << 32;
-    int lo = 0, hi = n_holes;
+// Tail-fill compaction over all holes at once.  Per box: the range of its
+// holes in the sorted list and how many lie below the final count.  Then
+// the j-th such head hole receives the j-th live slot of the tail
+// [n_final, n) in slot order: v = n_final + j + m with m the number of the
+// tail's holes below v, found by binary search over those holes.  Sources
+// lie at or above n_final and destinations below it, so the moves run in
+// place.  Live particles of the tail keep their slot order.
+__global__ void k_compact_ranges(BoxDesc *boxes, PW W, int n_box, int n_holes) {
+  const int box = blockIdx.x * blockDim.x + threadIdx.x;
+  if (box >= n_box) return;
+  auto lower = [&](uint64_t key, int lo, int hi) {
     while (lo < hi) {
-      int mid = (lo + hi) >> 1;
+      const int mid = (lo + hi) >> 1;
       if (W.holes_sorted[mid] < key) lo = mid + 1; else hi = mid;
     }
-    int a = lo;
-    lo = a;
-    hi = n_holes;
-    uint64_t key2 = (uint64_t)(box + 1) << 32;
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      if (W.holes_sorted[mid] < key2) lo = mid + 1; else hi = mid;
-    }
-    s_lo = a;
-    s_hi = lo;
-    s_cnt = 0;
+    return lo;
+  };
+  const int lo = lower((uint64_t)box << 32, 0, n_holes);
+  const int hi = lower((uint64_t)(box + 1) << 32, lo, n_holes);
+  const int n_final = boxes[box].n_part - (hi - lo);
+  W.cmp_lo[box] = lo;
+  W.cmp_hi[box] = hi;
+  W.cmp_head[box] = lower(((uint64_t)box << 32) | (uint64_t)(uint32_t)n_final, lo, hi) - lo;
+}
+
+__global__ void k_compact_move(BoxDesc *boxes, Arrays A, PW W, int n_holes) {
+  const int jg = blockIdx.x * blockDim.x + threadIdx.x;
+  if (jg >= n_holes) return;
+  const uint64_t h = W.holes_sorted[jg];
+  const int box = (int)(h >> 32);
+  const int lo = W.cmp_lo[box], hi = W.cmp_hi[box], head = W.cmp_head[box];
+  const int j = jg - lo;
+  if (j >= head) return;
+  const BoxDesc *B = &boxes[box];
+  const int n_final = B->n_part - (hi - lo);
+  // tail holes: sorted slots at [lo + head, hi); the smallest m with
+  // n_final + j + m < tail_hole[m] (or m = their count)
+  int a = 0, b = hi - lo - head;
+  while (a < b) {
+    const int m = (a + b) >> 1;
+    const int slot = (int)(W.holes_sorted[lo + head + m] & 0xFFFFFFFFu);
+    if (n_final + j + m < slot) b = m; else a = m + 1;
   }
-  __syncthreads();
-  const int H = s_hi - s_lo;
-  if (H == 0) return;
-  const int n = B->n_part;
-  const int n_final = n - H;
-  const int64_t base = B->off;
-  // live slots of the tail [n_final, n), in slot order (block scan)
-  using BS = hipcub::BlockScan<int, 256>;
-  __shared__ typename BS::TempStorage tmp;
-  __shared__ int s_run;
-  if (threadIdx.x == 0) s_run = 0;
-  __syncthreads();
-  int *tail = W.tail_live + (int64_t)blockIdx.x * 0;  // per-box region below
-  for (int c0 = n_final; c0 < n; c0 += 256) {
-    int i = c0 + threadIdx.x;
-    int f = (i < n && W.status[base + i] != ST_HOLE) ? 1 : 0;
-    int r, tot;
-    BS(tmp).ExclusiveSum(f, r, tot);
-    if (f) W.tail_live[(int64_t)s_lo + s_run + r] = i;
-    __syncthreads();
-    if (threadIdx.x == 0) s_run += tot;
-    __syncthreads();
-  }
-  (void)tail;
-  // head holes are the first entries of the sorted range with slot < n_final
-  for (int j = threadIdx.x; j < H; j += 256) {
-    int hole = (int)(W.holes_sorted[s_lo + j] & 0xFFFFFFFFu);
-    if (hole >= n_final) continue;
-    int from = W.tail_live[s_lo + j];
-    int64_t pf = base + from, pt = base + hole;
-    copy_particle(A, pf, pt);
-    W.key[pt] = W.key[pf];
-    W.terms[pt] = W.terms[pf];
-    W.status[pt] = W.status[pf];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) B->n_part = n_final;
+  const int from = n_final + j + a;
+  const int hole = (int)(h & 0xFFFFFFFFu);
+  const int64_t pf = B->off + from, pt = B->off + hole;
+  copy_particle(A, pf, pt);
+  W.key[pt] = W.key[pf];
+  W.terms[pt] = W.terms[pf];
+  W.status[pt] = W.status[pf];
+}
+
+__global__ void k_compact_finish(BoxDesc *boxes, PW W, int n_box) {
+  const int box = blockIdx.x * blockDim.x + threadIdx.x;
+  if (box >= n_box) return;
+  boxes[box].n_part -= W.cmp_hi[box] - W.cmp_lo[box];
 }
 
 __global__ void k_par_invalidate(BoxDesc *boxes, int n_box) {
@@ -2661,7 +2653,8 @@ inline void parallel_free_arrays(ParallelWork &W) {
                   d.ev_p1, d.ev_p2, d.ev_box, d.ev_out, d.f_app, d.f_id, d.f_info, d.f_hole,
                   d.f_lose, d.f_pend, d.s_pend, d.dead_t, d.dead_r, d.s_app, d.s_id, d.s_info, d.s_hole,
                   d.s_lose, d.box_first,
-                  d.holes, d.holes_sorted, d.tail_live, W.scan_tmp, W.sort_tmp,
+                  d.holes, d.holes_sorted, d.cmp_lo, d.cmp_hi, d.cmp_head,
+                  W.scan_tmp, W.sort_tmp,
                   d.pair_acc, d.pair_accn, d.acc_exc, d.trec, d.tns, d.ev_rec, d.misc,
                   d.hole_top, d.tail_rounds, d.pend_k, d.tp_in, d.tpair};
   for (void *p : ptrs)
@@ -2781,7 +2774,9 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   W.hole_cap = total_cap + 1024;
   PCHECK(pmalloc(&d.holes, W.hole_cap));
   PCHECK(pmalloc(&d.holes_sorted, W.hole_cap));
-  PCHECK(pmalloc(&d.tail_live, W.hole_cap));
+  PCHECK(pmalloc(&d.cmp_lo, n_box));
+  PCHECK(pmalloc(&d.cmp_hi, n_box));
+  PCHECK(pmalloc(&d.cmp_head, n_box));
   // scan scratch: the largest of the histogram, pair and test scans
   size_t b1 = 0, b2 = 0, b3 = 0;
   PCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, d.hist, d.hist_scan, (int)std::max<int64_t>(H, 1)));
@@ -3018,8 +3013,10 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
   if (n_holes > 0) {
     PCHECK(hipcub::DeviceRadixSort::SortKeys(W.sort_tmp, W.sort_tmp_bytes, d.holes, d.holes_sorted,
                                              (int)n_holes, 0, 64, st));
-    PLAUNCH(W, "k_par_compact", k_par_compact, dim3(n_box), dim3(256), 0, st, d_box, A, d,
-            (int)n_holes);
+    hipLaunchKernelGGL(k_compact_ranges, gb, bb, 0, st, d_box, d, n_box, (int)n_holes);
+    PLAUNCH(W, "k_par_compact", k_compact_move, dim3((unsigned)((n_holes + 255) / 256)), dim3(256),
+            0, st, d_box, A, d, (int)n_holes);
+    hipLaunchKernelGGL(k_compact_finish, gb, bb, 0, st, d_box, d, n_box);
   }
   hipLaunchKernelGGL(k_par_end, gb, bb, 0, st, d_box, n_box);
   PCHECK(hipGetLastError());
