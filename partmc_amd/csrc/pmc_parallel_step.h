@@ -50,7 +50,8 @@ struct PW {  // device view of the parallel work arrays
   int64_t P_stride; // pair slots per box
   int tab_classes, max_bin;
   int *key;         // [total_cap] list key, -1 for no key
-  uint8_t *status;  // [total_cap]
+  uint8_t *status;  // [total_cap] (epoch << 2) | state; another step's epoch reads as live
+  int epoch;        // this step's status epoch, 1..63
   uint32_t *claim;  // [total_cap]
   uint32_t *claim_wr;  // [total_cap] tail: earliest claim of a test that changes the particle
   uint32_t *dead_t; // [total_cap] test index of the winner that consumed it
@@ -133,6 +134,7 @@ struct ParallelWork {
   void (*tend)(void *, const char *, hipEvent_t) = nullptr;
   void *tctx = nullptr;
   hipStream_t tst = nullptr;  // stream of the launch being timed, when not the handle's
+  int epoch = 0;              // status epoch of the last step (0: the array is clear)
   // a second stream (default priority) for the target walks, which run
   // beside the per-set tests of the same round
   hipStream_t st2 = nullptr;
@@ -630,11 +632,22 @@ __device__ inline void atomic_max_double(double *addr, double v) {
   atomicMax(a, nv);  // v >= 0
 }
 
+// A particle's status this step.  Statuses carry the epoch of the step that
+// set them, so a new step starts with every particle live without clearing
+// the array (it is cleared when the epoch wraps).
+__device__ inline uint8_t st_get(const PW &W, int64_t p) {
+  const uint8_t v = W.status[p];
+  return (int)(v >> 2) == W.epoch ? (uint8_t)(v & 3) : ST_LIVE;
+}
+__device__ inline void st_set(const PW &W, int64_t p, uint8_t s) {
+  W.status[p] = (uint8_t)((W.epoch << 2) | s);
+}
+
 // a slot of a step-start list can be drawn while it holds a particle of
 // that list's (bin, class): untouched, or changed in place by an earlier
 // merge without leaving the list
 __device__ inline bool drawable(const PW &W, int p, int key) {
-  return W.status[p] != ST_HOLE && W.key[p] == key;
+  return st_get(W, p) != ST_HOLE && W.key[p] == key;
 }
 
 // find_rand_pair + maybe_coag_pair (src/coagulation.F90:712-803) for test t
@@ -1259,7 +1272,7 @@ __device__ __noinline__ int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &
   const int n_t = W.lcnt[lb + par_key(b_cur, ct, nb)];
   const int e = n_t - (int)(t - W.pair_exc[k]);
   const int p = W.perm[B->off + W.lstart[lb + par_key(b_cur, ct, nb)] + e - 1];
-  if (W.status[p] == ST_HOLE) return 0;  // removed earlier in this step
+  if (st_get(W, p) == ST_HOLE) return 0;  // removed earlier in this step
   if (W.key[p] != par_key(b_cur, ct, nb)) {
     // changed in place earlier in the step: walk on from its current list
     const int kp = W.key[p];
@@ -1566,7 +1579,7 @@ __device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc 
   }
   A.least[slot] = r.least;
   A.greatest[slot] = r.greatest;
-  W.status[slot] = ST_NEW;
+  st_set(W, slot, ST_NEW);
   int b = bin_find_log(nb, B->log_lo, B->log_hi, vol2rad(v));
   if (b < 1 || b > nb) {
     atomicExch(&B->error, 134572570);
@@ -1814,7 +1827,7 @@ __device__ inline int direct_merge(const Arrays &A, const PW &W, BoxDesc *B, int
   A.ncomp[dst] = n_out;
   A.least[dst] = least;
   A.greatest[dst] = greatest;
-  W.status[dst] = ST_NEW;
+  st_set(W, dst, ST_NEW);
   const int nb = B->n_bin;
   const int b = bin_find_log(nb, B->log_lo, B->log_hi, vol2rad(v));
   if (b < 1 || b > nb) {
@@ -1880,7 +1893,7 @@ __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, in
     W.dead_r[p1] = rid;
     al.live_add(key1, -1);
     if (!(o & O_SLOT1)) {
-      W.status[p1] = ST_HOLE;
+      st_set(W, p1, ST_HOLE);
       W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(p1 - B->off);
     }
   }
@@ -1889,7 +1902,7 @@ __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, in
     W.dead_r[p2] = rid;
     al.live_add(key2, -1);
     if (!(o & O_SLOT2)) {
-      W.status[p2] = ST_HOLE;
+      st_set(W, p2, ST_HOLE);
       W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(p2 - B->off);
     }
   }
@@ -1936,7 +1949,7 @@ __device__ __noinline__ void acc_apply(BoxDesc *B, const Arrays &A, const PW &W,
       W.dead_t[q] = t;
       W.dead_r[q] = rid;
       al.live_add(W.key[q], -1);
-      W.status[q] = ST_HOLE;
+      st_set(W, q, ST_HOLE);
       W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(q - B->off);
       put_info(A, B, al, A.id[q], 2, owner);
     });
@@ -1953,7 +1966,7 @@ __device__ __noinline__ void acc_apply(BoxDesc *B, const Arrays &A, const PW &W,
   al.live_add(W.key[p], -1);
   write_new_particle(A, W, B, p, np, nb, type, want_terms);
   if (flags & RF_DIED) {
-    W.status[p] = ST_HOLE;
+    st_set(W, p, ST_HOLE);
     W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(p - B->off);
     put_info(A, B, al, target_id, 4, 0);
   } else {
@@ -2066,7 +2079,7 @@ k_par_apply_chains(BoxDesc *boxes, Arrays A, PW W, int E, int64_t hole_base, int
       W.dead_t[q] = t;
       W.dead_r[q] = rid;
       atomicAdd(&live[W.key[q]], -1);
-      W.status[q] = ST_HOLE;
+      st_set(W, q, ST_HOLE);
       W.holes[hole0 + ord] = ((uint64_t)box << 32) | (uint64_t)(q - B->off);
       const int64_t ki = info0 + ord;
       if (ki < B->info_cap) {
@@ -2131,7 +2144,7 @@ __global__ void k_par_release(PW W, int E) {
 // seen the particles as that winner left them.  A particle changed in an
 // earlier round was drawn in its changed state.
 __device__ inline bool consumed_before(const PW &W, int p, uint32_t t, int rid) {
-  return p >= 0 && W.status[p] != ST_LIVE && W.dead_t[p] < t && W.dead_r[p] == rid;
+  return p >= 0 && st_get(W, p) != ST_LIVE && W.dead_t[p] < t && W.dead_r[p] == rid;
 }
 
 __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
@@ -2792,6 +2805,7 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   hipLaunchKernelGGL(k_fill_u32, dim3((tc + 255) / 256), dim3(256), 0, 0, d.claim, tc, UNCLAIMED);
   hipLaunchKernelGGL(k_fill_u32, dim3((tc + 255) / 256), dim3(256), 0, 0, d.claim_wr, tc, UNCLAIMED);
   hipLaunchKernelGGL(k_fill_u8, dim3((tc + 255) / 256), dim3(256), 0, 0, d.status, tc, ST_LIVE);
+  W.epoch = 0;
   PCHECK(hipGetLastError());
   PCHECK(hipDeviceSynchronize());
   return 0;
@@ -2893,8 +2907,14 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
     int e = parallel_alloc_records(W, A.n_spec);
     if (e) return e;
   }
-  hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((A.total_cap + 255) / 256)), dim3(256), 0, st,
-                     d.status, A.total_cap, ST_LIVE);
+  // every particle starts the step live: a fresh status epoch, and a clear
+  // array when the 6-bit epoch wraps
+  if (++W.epoch > 63) {
+    W.epoch = 1;
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((A.total_cap + 255) / 256)), dim3(256), 0, st,
+                       d.status, A.total_cap, ST_LIVE);
+  }
+  d.epoch = W.epoch;
   // 5. tests, claims and merges, in batches of T_cap tests
   int64_t n_holes = 0;
   for (uint32_t t0 = 0; t0 < T;) {
