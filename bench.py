@@ -97,6 +97,7 @@ def run_gpu(pops, kernel, dt, mode, precision, steps, warmup, seed=20261202, reb
         "tail_tests": st["n_tail"],
         "tail_rounds": st["n_tail_rounds"],
         "accel_pairs": int(st["n_accel_pairs"].sum()),
+        "acc_samples": st["acc_samples"],
         "elapsed": elapsed,
         "n_samp": int(c["n_samp"].sum()),
         "n_coag": int(c["n_coag"].sum()),
@@ -221,6 +222,9 @@ def main():
         if dom in ("k_par_tests", "k_par_apply"):
             traffic = kernel_bytes(dom, n_spec, args.precision, psteps / args.steps,
                                    n_samp / n, pairs, n_coag / n)
+        if dom == "k_par_acc_tests":
+            # per source sample: its list entry and its species volumes
+            traffic = (4.0 + 8.0 * n_spec) * r["acc_samples"] / n
         if dom == "k_par_tail":
             # the tests the tail finished, per launch (one launch per step)
             traffic = kernel_bytes(dom, n_spec, args.precision, psteps / args.steps,
@@ -258,6 +262,7 @@ def main():
         "tail_tests_per_step": r["tail_tests"] / args.steps,
         "tail_rounds_per_step": r["tail_rounds"] / args.steps,
         "accel_pairs_per_step": r["accel_pairs"] / args.steps,
+        "acc_samples_per_step": r["acc_samples"] / args.steps,
         "fp64_rechecks": r["n_recheck"],
         "kernel_ms": {k: v[0] / v[1] for k, v in timings.items()},
         "kernel_total_ms": {k: v[0] for k, v in timings.items()},

@@ -213,10 +213,11 @@ int pmc_gpu_get_part_steps(pmc_gpu *h, int64_t *part_steps);
 /* Path statistics since the last reset.  Per box: bin pairs that took the
  * per-particle path (try_per_particle_coag, src/coagulation.F90:150-241)
  * and parallel-mode draws that found no particle still present within the
- * re-draw limit.  For the handle (rounds[3]): rounds[0] = device-wide
+ * re-draw limit.  For the handle (rounds[4]): rounds[0] = device-wide
  * claim rounds run by the parallel mode, rounds[1] = tests it finished in
  * the per-box tail that follows them, rounds[2] = the tail's claim rounds
- * summed over boxes.  box_tail (may be NULL) receives per box the tests
+ * summed over boxes, rounds[3] = source samples drawn by the device-wide
+ * target walks (runs and re-runs).  box_tail (may be NULL) receives per box the tests
  * finished in the tail and its claim rounds, [2 * b] and [2 * b + 1].  No
  * counterpart in the reference, which has no parallel mode. */
 int pmc_gpu_get_path_stats(pmc_gpu *h, int64_t *n_accel_pairs, int64_t *n_dropped,

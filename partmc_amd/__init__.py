@@ -341,12 +341,13 @@ class CoagGPU:
         tail."""
         acc = np.zeros(self.n_box, dtype=np.int64)
         drop = np.zeros(self.n_box, dtype=np.int64)
-        rounds = np.zeros(3, dtype=np.int64)
+        rounds = np.zeros(4, dtype=np.int64)
         tail = np.zeros((self.n_box, 2), dtype=np.int64)
         _check(load().pmc_gpu_get_path_stats(self._h, _ptr(acc), _ptr(drop), _ptr(rounds),
                                              _ptr(tail)))
         return {"n_accel_pairs": acc, "n_dropped": drop, "n_rounds": int(rounds[0]),
                 "n_tail": int(rounds[1]), "n_tail_rounds": int(rounds[2]),
+                "acc_samples": int(rounds[3]),
                 "box_tail_tests": tail[:, 0].copy(), "box_tail_rounds": tail[:, 1].copy()}
 
     def reserve(self, capacity):

@@ -801,6 +801,7 @@ int pmc_gpu_reset_counters(pmc_gpu *h) {
   }
   h->work.n_rounds = 0;
   h->work.n_tail = 0;
+  if (h->work.d.acc_samp) HIPCHECK(hipMemset(h->work.d.acc_samp, 0, sizeof(unsigned long long)));
   h->work.n_tail_rounds = 0;
   HIPCHECK(hipMemcpy(h->d_box, hb.data(), sizeof(BoxDesc) * h->n_box, hipMemcpyHostToDevice));
   return 0;
@@ -835,6 +836,10 @@ int pmc_gpu_get_path_stats(pmc_gpu *h, int64_t *n_accel_pairs, int64_t *n_droppe
   rounds[0] = h->work.n_rounds;
   rounds[1] = h->work.n_tail;
   rounds[2] = h->work.n_tail_rounds;
+  unsigned long long acc_samp = 0;
+  if (h->work.d.acc_samp)
+    HIPCHECK(hipMemcpy(&acc_samp, h->work.d.acc_samp, sizeof(acc_samp), hipMemcpyDeviceToHost));
+  rounds[3] = (int64_t)acc_samp;
   return 0;
 }
 

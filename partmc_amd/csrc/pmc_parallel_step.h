@@ -71,7 +71,8 @@ struct PW {  // device view of the parallel work arrays
   int *s_app, *s_id, *s_info, *s_hole, *s_lose;
   int *box_first;
   uint64_t *holes, *holes_sorted;  // (box << 32 | slot)
-  int *cmp_lo, *cmp_hi, *cmp_head;  // [n_box] compaction: hole range and holes below the final count
+  int *cmp_lo, *cmp_hi, *cmp_head;
+  unsigned long long *acc_samp;  // samples drawn by the device-wide target walks  // [n_box] compaction: hole range and holes below the final count
   // accelerated per-particle path (try_per_particle_coag): one test slot per
   // target particle, one event record per target slot of a batch
   uint8_t *pair_acc;           // [NP] 0 per-set; 1 targets in (b1, c1); 2 targets in (b2, c2)
@@ -1470,6 +1471,9 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
   W.tp1[i] = p;
   W.tns[i] = tns;
   if (r == 1) W.acc[i] = 1;
+  // the walk's samples, for the algorithmic traffic of this kernel
+  const int ns = (r == 1) ? W.r_nsamp[rec] : (tns & (ACC_WARN - 1));
+  if (ns > 0) atomicAdd(W.acc_samp, (unsigned long long)ns);
 }
 
 // the largest batch end t1 <= min(T, t0 + T_cap) whose target slots fit the
@@ -2666,7 +2670,7 @@ inline void parallel_free_arrays(ParallelWork &W) {
                   d.ev_p1, d.ev_p2, d.ev_box, d.ev_out, d.f_app, d.f_id, d.f_info, d.f_hole,
                   d.f_lose, d.f_pend, d.s_pend, d.dead_t, d.dead_r, d.s_app, d.s_id, d.s_info, d.s_hole,
                   d.s_lose, d.box_first,
-                  d.holes, d.holes_sorted, d.cmp_lo, d.cmp_hi, d.cmp_head,
+                  d.holes, d.holes_sorted, d.cmp_lo, d.cmp_hi, d.cmp_head, d.acc_samp,
                   W.scan_tmp, W.sort_tmp,
                   d.pair_acc, d.pair_accn, d.acc_exc, d.trec, d.tns, d.ev_rec, d.misc,
                   d.hole_top, d.tail_rounds, d.pend_k, d.tp_in, d.tpair};
@@ -2788,6 +2792,8 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.holes, W.hole_cap));
   PCHECK(pmalloc(&d.holes_sorted, W.hole_cap));
   PCHECK(pmalloc(&d.cmp_lo, n_box));
+  PCHECK(pmalloc(&d.acc_samp, 1));
+  PCHECK(hipMemset(d.acc_samp, 0, sizeof(unsigned long long)));
   PCHECK(pmalloc(&d.cmp_hi, n_box));
   PCHECK(pmalloc(&d.cmp_head, n_box));
   // scan scratch: the largest of the histogram, pair and test scans
