@@ -87,26 +87,56 @@ __device__ inline void make_log_grid(int n_bin, double rmin, double rmax,
   for (int i = 0; i < n_bin; ++i) widths[i] = (lmax - lmin) / (double)n_bin;
 }
 
-// whole-record copy of particle slot `from` into slot `to`
+// whole-record copy of particle slot `from` into slot `to`.  Fields are
+// read in chunks before they are written: a store between two loads of
+// the same arrays makes the second wait for the first, one memory round
+// trip per field.
 __device__ inline void copy_particle(const Arrays &A, int64_t from, int64_t to) {
-  for (int s = 0; s < A.n_spec; ++s)
-    A.vol[(int64_t)s * A.total_cap + to] = A.vol[(int64_t)s * A.total_cap + from];
-  A.vtot[to] = A.vtot[from];
-  A.id[to] = A.id[from];
-  A.group[to] = A.group[from];
-  A.klass[to] = A.klass[from];
-  A.nprim[to] = A.nprim[from];
-  const int nco = A.ncomp[from];
-  A.ncomp[to] = nco;
-  for (int c = 0; c < nco; ++c) {  // entries past n_comp are never read
-    A.csrc[(int64_t)c * A.total_cap + to] = A.csrc[(int64_t)c * A.total_cap + from];
-    A.ctime[(int64_t)c * A.total_cap + to] = A.ctime[(int64_t)c * A.total_cap + from];
+  const int64_t tc = A.total_cap;
+  for (int s0 = 0; s0 < A.n_spec; s0 += 8) {
+    double x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (s0 + j < A.n_spec) x[j] = A.vol[(int64_t)(s0 + j) * tc + from];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (s0 + j < A.n_spec) A.vol[(int64_t)(s0 + j) * tc + to] = x[j];
   }
-  A.least[to] = A.least[from];
-  A.greatest[to] = A.greatest[from];
-  A.fwd_bin[to] = A.fwd_bin[from];
-  A.fwd_class[to] = A.fwd_class[from];
-  A.fwd_idx[to] = A.fwd_idx[from];
+  const double vtot = A.vtot[from];
+  const int64_t id = A.id[from];
+  const int group = A.group[from], klass = A.klass[from], nprim = A.nprim[from];
+  const int nco = A.ncomp[from];
+  const double least = A.least[from], greatest = A.greatest[from];
+  const int fb = A.fwd_bin[from], fc = A.fwd_class[from], fi = A.fwd_idx[from];
+  A.vtot[to] = vtot;
+  A.id[to] = id;
+  A.group[to] = group;
+  A.klass[to] = klass;
+  A.nprim[to] = nprim;
+  A.ncomp[to] = nco;
+  A.least[to] = least;
+  A.greatest[to] = greatest;
+  A.fwd_bin[to] = fb;
+  A.fwd_class[to] = fc;
+  A.fwd_idx[to] = fi;
+  for (int c0 = 0; c0 < nco; c0 += 10) {  // entries past n_comp are never read
+    int cs[10];
+    double ct[10];
+#pragma unroll
+    for (int j = 0; j < 10; ++j) {
+      if (c0 + j < nco) {
+        cs[j] = A.csrc[(int64_t)(c0 + j) * tc + from];
+        ct[j] = A.ctime[(int64_t)(c0 + j) * tc + from];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 10; ++j) {
+      if (c0 + j < nco) {
+        A.csrc[(int64_t)(c0 + j) * tc + to] = cs[j];
+        A.ctime[(int64_t)(c0 + j) * tc + to] = ct[j];
+      }
+    }
+  }
 }
 
 // -------------------------------------------------------------- prepare

@@ -465,19 +465,29 @@ __device__ inline void decode_pair(int64_t q, int tc, int mb, int *c1, int *c2, 
   *b2 = rem % mb + 1;
 }
 
-__device__ inline void load_rec(const Arrays &A, int64_t p, Rec &r) {
+// the record without its components (they are read in a second pass
+// that depends on ncomp, so a reader that may not need them defers it)
+__device__ inline void load_rec_head(const Arrays &A, int64_t p, Rec &r) {
   for (int s = 0; s < A.n_spec; ++s) r.vol[s] = A.vol[(int64_t)s * A.total_cap + p];
   r.id = A.id[p];
   r.group = A.group[p];
   r.klass = A.klass[p];
   r.nprim = A.nprim[p];
   r.ncomp = A.ncomp[p];
+  r.least = A.least[p];
+  r.greatest = A.greatest[p];
+}
+
+__device__ inline void load_rec_comps(const Arrays &A, int64_t p, Rec &r) {
   for (int c = 0; c < r.ncomp; ++c) {
     r.csrc[c] = A.csrc[(int64_t)c * A.total_cap + p];
     r.ctime[c] = A.ctime[(int64_t)c * A.total_cap + p];
   }
-  r.least = A.least[p];
-  r.greatest = A.greatest[p];
+}
+
+__device__ inline void load_rec(const Arrays &A, int64_t p, Rec &r) {
+  load_rec_head(A, p, r);
+  load_rec_comps(A, p, r);
 }
 
 // aero_particle_coagulate for the parallel path (same rules as rec_coagulate)
@@ -1176,7 +1186,7 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
     // target's draws: remember what was read
     if (n_s <= READ_TRACK && !rm_has(W, reads, ps) && !rm_push(W, reads, ps)) return ERR_CAPACITY;
     n_samp++;
-    load_rec(A, ps, pi);
+    load_rec_head(A, ps, pi);
     double kk;
     int err = weighted_kernel_rec(A, w, env, type, pi, target, cs, ct, ct, &kk);
     if (err) return err;
@@ -1184,6 +1194,7 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
     prob_coag_tot += prob_coag;
     if (smp.uniform() < prob_coag) {
       n_avg++;
+      load_rec_comps(A, ps, pi);
       err = par_rec_coagulate(ns, smp, source, pi, source);
       if (err) return err;
       for (int q = 0; q < ns; ++q) vol_sq[q] += pi.vol[q] * pi.vol[q];
@@ -1437,13 +1448,17 @@ __device__ __noinline__ int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &
   return 1;
 }
 
+// walks per wavefront: a walk is one lane's serial chain, and fewer walks a
+// wavefront spread the first round's walks over more compute units
+constexpr int ACC_LANES = 16;
 // Round 0 runs one thread per target slot of the batch (record a maps to
 // its slot through acc_exc); later rounds run over the pending slots.
 __global__ void __launch_bounds__(64)
 k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed,
                 uint32_t step, uint32_t t0, int n, const uint32_t *pend, double del_t,
                 uint32_t acc_base, int n_acc) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if ((int)threadIdx.x >= ACC_LANES) return;
+  const int idx = blockIdx.x * ACC_LANES + threadIdx.x;
   const int64_t NP = (int64_t)n_box * W.P_stride;
   uint32_t t;
   int i;
@@ -1722,29 +1737,39 @@ __device__ inline void put_info(const Arrays &A, BoxDesc *B, Alloc &al, int64_t 
   }
 }
 
+// Both parents' components, read before any is written (dst may be either
+// parent's slot) and all in flight at once: a load inside the selection
+// branches below would cost one memory round trip per component.
+__device__ inline void load_comps2(const Arrays &A, int64_t p1, int64_t p2, int n1, int n2,
+                                   int *s1, double *t1, int *s2, double *t2) {
+  const int64_t tc = A.total_cap;
+#pragma unroll
+  for (int c = 0; c < MAX_COMP; ++c) {
+    if (c < n1) {
+      s1[c] = A.csrc[(int64_t)c * tc + p1];
+      t1[c] = A.ctime[(int64_t)c * tc + p1];
+    }
+    if (c < n2) {
+      s2[c] = A.csrc[(int64_t)c * tc + p2];
+      t2[c] = A.ctime[(int64_t)c * tc + p2];
+    }
+  }
+}
+
 // the merged particle's components when the parents hold more than
 // MAX_COMP together (aero_particle_coagulate, src/aero_particle.F90:
 // 917-1033): prob_round of the first parent's share, then two selection
-// samples (sample_without_replacement keeps sorted order), streamed from
-// the parents into dst.  The parent that dst overwrites is read into
-// registers first; the loops are unrolled so the arrays stay in registers.
+// samples (sample_without_replacement keeps sorted order) from the parents'
+// components, which are read into registers first; the loops are unrolled
+// so the arrays stay in registers.
 __device__ inline void resample_components(const Arrays &A, Stream &s, int64_t p1, int64_t p2,
                                            int64_t dst, int n1, int n2, int np1, int np2) {
   const int64_t tc = A.total_cap;
+  int s1[MAX_COMP], s2[MAX_COMP];
+  double t1[MAX_COMP], t2[MAX_COMP];
+  load_comps2(A, p1, p2, n1, n2, s1, t1, s2, t2);
   const int n1_new = prob_round(s, (double)np1 / (double)(np1 + np2) * (double)MAX_COMP);
   const int n2_new = MAX_COMP - n1_new;
-  const int64_t keep = (dst == p1) ? p1 : p2;  // the parent read ahead
-  int ks[MAX_COMP];
-  double kt[MAX_COMP];
-  const int nk = (dst == p1) ? n1 : n2;
-#pragma unroll
-  for (int c = 0; c < MAX_COMP; ++c) {
-    if (c < nk) {
-      ks[c] = A.csrc[(int64_t)c * tc + keep];
-      kt[c] = A.ctime[(int64_t)c * tc + keep];
-    }
-  }
-  const bool ahead1 = (dst == p1), ahead2 = (dst == p2);
   // first parent: n1_new of its n1 components into positions 0 ..
   {
     int m = 0;
@@ -1753,10 +1778,8 @@ __device__ inline void resample_components(const Arrays &A, Stream &s, int64_t p
       if (c < n1 && m < n1_new) {
         const double r = s.uniform();
         if (!((double)(n1 - c) * r >= (double)(n1_new - m))) {
-          const int cs = ahead1 ? ks[c] : A.csrc[(int64_t)c * tc + p1];
-          const double ct = ahead1 ? kt[c] : A.ctime[(int64_t)c * tc + p1];
-          A.csrc[(int64_t)m * tc + dst] = cs;
-          A.ctime[(int64_t)m * tc + dst] = ct;
+          A.csrc[(int64_t)m * tc + dst] = s1[c];
+          A.ctime[(int64_t)m * tc + dst] = t1[c];
           m++;
         }
       }
@@ -1770,10 +1793,8 @@ __device__ inline void resample_components(const Arrays &A, Stream &s, int64_t p
       if (c < n2 && m < n2_new) {
         const double r = s.uniform();
         if (!((double)(n2 - c) * r >= (double)(n2_new - m))) {
-          const int cs = ahead2 ? ks[c] : A.csrc[(int64_t)c * tc + p2];
-          const double ct = ahead2 ? kt[c] : A.ctime[(int64_t)c * tc + p2];
-          A.csrc[(int64_t)(n1_new + m) * tc + dst] = cs;
-          A.ctime[(int64_t)(n1_new + m) * tc + dst] = ct;
+          A.csrc[(int64_t)(n1_new + m) * tc + dst] = s2[c];
+          A.ctime[(int64_t)(n1_new + m) * tc + dst] = t2[c];
           m++;
         }
       }
@@ -1796,31 +1817,40 @@ __device__ inline int direct_merge(const Arrays &A, const PW &W, BoxDesc *B, int
   if (np2 < n2) return 465791385;
   const double least = fmin(A.least[p1], A.least[p2]);
   const double greatest = fmax(A.greatest[p1], A.greatest[p2]);
+  // the summed volumes are read before any is written (dst may be a
+  // parent's slot), so the loads go out together
+  double xs[MAX_SPEC];
+#pragma unroll
+  for (int s = 0; s < MAX_SPEC; ++s)
+    if (s < A.n_spec) xs[s] = A.vol[(int64_t)s * tc + p1] + A.vol[(int64_t)s * tc + p2];
   double v = 0.0, m = 0.0;
-  for (int s = 0; s < A.n_spec; ++s) {
-    const double x = A.vol[(int64_t)s * tc + p1] + A.vol[(int64_t)s * tc + p2];
-    A.vol[(int64_t)s * tc + dst] = x;
-    v += x;
-    m += x * A.density[s];
+#pragma unroll
+  for (int s = 0; s < MAX_SPEC; ++s) {
+    if (s < A.n_spec) {
+      A.vol[(int64_t)s * tc + dst] = xs[s];
+      v += xs[s];
+      m += xs[s] * A.density[s];
+    }
   }
   // components: the first parent's, then the second's
   int n_out = n1 + n2;
   if (n1 + n2 > MAX_COMP) {
     resample_components(A, *rs, p1, p2, dst, n1, n2, np1, np2);
     n_out = MAX_COMP;
-  } else if (dst == p1) {
-    for (int c = 0; c < n2; ++c) {
-      A.csrc[(int64_t)(n1 + c) * tc + dst] = A.csrc[(int64_t)c * tc + p2];
-      A.ctime[(int64_t)(n1 + c) * tc + dst] = A.ctime[(int64_t)c * tc + p2];
-    }
   } else {
-    for (int c = n2 - 1; c >= 0; --c) {  // moves up: safe when dst == p2
-      A.csrc[(int64_t)(n1 + c) * tc + dst] = A.csrc[(int64_t)c * tc + p2];
-      A.ctime[(int64_t)(n1 + c) * tc + dst] = A.ctime[(int64_t)c * tc + p2];
-    }
-    for (int c = 0; c < n1; ++c) {
-      A.csrc[(int64_t)c * tc + dst] = A.csrc[(int64_t)c * tc + p1];
-      A.ctime[(int64_t)c * tc + dst] = A.ctime[(int64_t)c * tc + p1];
+    int s1[MAX_COMP], s2[MAX_COMP];
+    double t1[MAX_COMP], t2[MAX_COMP];
+    load_comps2(A, p1, p2, n1, n2, s1, t1, s2, t2);
+#pragma unroll
+    for (int c = 0; c < MAX_COMP; ++c) {
+      if (c < n1) {
+        A.csrc[(int64_t)c * tc + dst] = s1[c];
+        A.ctime[(int64_t)c * tc + dst] = t1[c];
+      }
+      if (c < n2) {
+        A.csrc[(int64_t)(n1 + c) * tc + dst] = s2[c];
+        A.ctime[(int64_t)(n1 + c) * tc + dst] = t2[c];
+      }
     }
   }
   A.vtot[dst] = v;
@@ -2984,7 +3014,7 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         PCHECK(hipStreamWaitEvent(W.st2, W.ev_fork, 0));
         const int na = pend ? n : n_acc;
         W.tst = W.st2;
-        PLAUNCH(W, "k_par_acc_tests", k_par_acc_tests, dim3((na + 63) / 64), dim3(64), 0, W.st2,
+        PLAUNCH(W, "k_par_acc_tests", k_par_acc_tests, dim3((na + ACC_LANES - 1) / ACC_LANES), dim3(64), 0, W.st2,
                 d_box, A, d, n_box, type, seed, step, batch_t0, n, pend, del_t, a0, n_acc);
         W.tst = nullptr;
         PCHECK(hipEventRecord(W.ev_join, W.st2));
