@@ -360,6 +360,16 @@ __device__ inline double coag_num_conc_factor(const Weights &w, double i_r,
   return i_nc * j_nc / nc_min;
 }
 
+// coag_num_conc_factor with the second particle's volume and weight given
+__device__ inline double coag_num_conc_factor_j(const Weights &w, double i_r, double j_v,
+                                                double j_nc, int ic, int ijc) {
+  double ij_r = vol2rad(rad2vol(i_r) + j_v);
+  double i_nc = nc_at_radius(w, ic, i_r);
+  double ij_nc = nc_at_radius(w, ijc, ij_r);
+  double nc_min = fmin(i_nc, fmin(j_nc, ij_nc));
+  return i_nc * j_nc / nc_min;
+}
+
 // coag_dest_class, src/coag_kernel.F90:382-416 (both radii at centers(i_bin))
 __device__ inline int coag_dest_class(const Weights &w, const double *centers,
                                       int i_bin, int ic, int jc) {
@@ -410,45 +420,55 @@ __host__ __device__ inline Env make_env(double temp, double pressure, double bet
   return e;
 }
 
-// kernel_brown_helper, src/coag_kernel_brown.F90:111-197 (spherical)
-__device__ inline double kernel_brown_helper(double vol_i, double den_i,
-                                             double vol_j, double den_j,
-                                             double temp, double pressure) {
+// kernel_brown_helper, src/coag_kernel_brown.F90:111-197 (spherical), in
+// three parts: the air's properties, the terms of one particle, and their
+// combination -- the same operations in the same order, so a caller that
+// keeps one particle's terms across many partners gets the same values
+struct BrownAir {
+  double viscosd, gasfreepath;
+};
+struct BrownTerms {
+  double rad, diffus, speedsq, deltasq;
+};
+
+__device__ inline BrownAir brown_air(double temp, double pressure) {
   double rhoair = (pressure * C_AIR_MW) / (C_UNIV_GAS * temp);
   double viscosd = 1.8325e-05 * (416.16 / (temp + 120.0)) * pow(temp / 296.16, 1.5);
   double viscosk = viscosd / rhoair;
   double gasspeed = sqrt((8.0 * C_BOLTZMANN * temp * C_AVOGADRO) / (C_PI * C_AIR_MW));
-  double gasfreepath = 2.0 * viscosk / gasspeed;
+  return BrownAir{viscosd, 2.0 * viscosk / gasspeed};
+}
 
-  double rad_i = vol2rad(vol_i);
-  double Rme_i = vol2rad(vol_i);
-  double knud = gasfreepath / Rme_i;
+__device__ inline BrownTerms brown_terms(const BrownAir &air, double vol, double den,
+                                         double temp) {
+  double rad = vol2rad(vol);
+  double Rme = vol2rad(vol);
+  double knud = air.gasfreepath / Rme;
   double cunning = 1.0 + knud * (1.249 + 0.42 * exp(-0.87 / knud));
-  double diffus_i = (C_BOLTZMANN * temp * cunning) / (6.0 * C_PI * Rme_i * viscosd);
-  double speedsq_i = 8.0 * C_BOLTZMANN * temp / (C_PI * den_i * vol_i);
-  double freepath = 8.0 * diffus_i / (C_PI * sqrt(speedsq_i));
-  double tmp1 = powi(2.0 * Rme_i + freepath, 3);
-  double tmp2 = pow(4.0 * Rme_i * Rme_i + freepath * freepath, 1.5);
-  double d = (tmp1 - tmp2) / (6.0 * Rme_i * freepath) - 2.0 * Rme_i;
-  double deltasq_i = d * d;
+  double diffus = (C_BOLTZMANN * temp * cunning) / (6.0 * C_PI * Rme * air.viscosd);
+  double speedsq = 8.0 * C_BOLTZMANN * temp / (C_PI * den * vol);
+  double freepath = 8.0 * diffus / (C_PI * sqrt(speedsq));
+  double tmp1 = powi(2.0 * Rme + freepath, 3);
+  double tmp2 = pow(4.0 * Rme * Rme + freepath * freepath, 1.5);
+  double d = (tmp1 - tmp2) / (6.0 * Rme * freepath) - 2.0 * Rme;
+  return BrownTerms{rad, diffus, speedsq, d * d};
+}
 
-  double rad_j = vol2rad(vol_j);
-  double Rme_j = vol2rad(vol_j);
-  knud = gasfreepath / Rme_j;
-  cunning = 1.0 + knud * (1.249 + 0.42 * exp(-0.87 / knud));
-  double diffus_j = (C_BOLTZMANN * temp * cunning) / (6.0 * C_PI * Rme_j * viscosd);
-  double speedsq_j = 8.0 * C_BOLTZMANN * temp / (C_PI * den_j * vol_j);
-  freepath = 8.0 * diffus_j / (C_PI * sqrt(speedsq_j));
-  tmp1 = powi(2.0 * Rme_j + freepath, 3);
-  tmp2 = pow(4.0 * Rme_j * Rme_j + freepath * freepath, 1.5);
-  d = (tmp1 - tmp2) / (6.0 * Rme_j * freepath) - 2.0 * Rme_j;
-  double deltasq_j = d * d;
-
-  double rad_sum = rad_i + rad_j;
-  double diffus_sum = diffus_i + diffus_j;
-  tmp1 = rad_sum / (rad_sum + sqrt(deltasq_i + deltasq_j));
-  tmp2 = 4.0 * diffus_sum / (rad_sum * sqrt(speedsq_i + speedsq_j));
+__device__ inline double brown_combine(const BrownTerms &i, const BrownTerms &j) {
+  double rad_sum = i.rad + j.rad;
+  double diffus_sum = i.diffus + j.diffus;
+  double tmp1 = rad_sum / (rad_sum + sqrt(i.deltasq + j.deltasq));
+  double tmp2 = 4.0 * diffus_sum / (rad_sum * sqrt(i.speedsq + j.speedsq));
   return 4.0 * C_PI * rad_sum * diffus_sum / (tmp1 + tmp2);
+}
+
+__device__ inline double kernel_brown_helper(double vol_i, double den_i,
+                                             double vol_j, double den_j,
+                                             double temp, double pressure) {
+  const BrownAir air = brown_air(temp, pressure);
+  const BrownTerms ti = brown_terms(air, vol_i, den_i, temp);
+  const BrownTerms tj = brown_terms(air, vol_j, den_j, temp);
+  return brown_combine(ti, tj);
 }
 
 // fall_g, src/coag_kernel_sedi.F90:110-172

@@ -1122,6 +1122,48 @@ __device__ inline void pool_load(const PW &W, int n_spec, int i, int klass, int6
 // the particles still present that this target has not removed; returns
 // an error code, 0 on success.  Sources it removes are pushed to `chain`
 // and claimed with index t.
+// The target's side of weighted_kernel_rec, the same for every source drawn
+// for one pair: its Brownian terms, volume, radius and weight.
+struct TargetSide {
+  BrownAir air;
+  BrownTerms terms;
+  double vol, rad, nc;
+};
+
+__device__ inline TargetSide target_side(const Arrays &A, const Weights &w, const Env &env,
+                                         int type, const Rec &b, int c2) {
+  TargetSide ts;
+  ts.vol = rec_vol(A.n_spec, b);
+  if (type == KERNEL_BROWN) {
+    ts.air = brown_air(env.temp, env.pressure);
+    ts.terms = brown_terms(ts.air, ts.vol, rec_density(A, b), env.temp);
+  }
+  ts.rad = vol2rad(ts.vol);
+  ts.nc = nc_at_radius(w, c2, ts.rad);
+  return ts;
+}
+
+// weighted_kernel_rec(a, b, c1, c2, cc) with b's side from target_side(b, c2)
+__device__ inline int weighted_kernel_tgt(const Arrays &A, const Weights &w, const Env &env,
+                                          int type, const Rec &a, const TargetSide &ts,
+                                          int c1, int cc, double *k_out) {
+  double v1 = rec_vol(A.n_spec, a), v2 = ts.vol;
+  double k = 0.0;
+  int err = 0;
+  switch (type) {
+    case KERNEL_ADDITIVE: k = env.beta * (v1 + v2); break;
+    case KERNEL_CONSTANT: k = C_BETA_0; break;
+    case KERNEL_ZERO: k = 0.0; break;
+    case KERNEL_BROWN:
+      k = brown_combine(brown_terms(ts.air, v1, rec_density(A, a), env.temp), ts.terms);
+      break;
+    case KERNEL_SEDI: err = kernel_sedi_helper(v1, v2, &k); break;
+    default: err = 200724934;
+  }
+  *k_out = k * coag_num_conc_factor_j(w, vol2rad(v1), rad2vol(ts.rad), ts.nc, c1, cc);
+  return err;
+}
+
 __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B, int type,
                                         const Rec &target, int ct, int cs, double nc_src_min,
                                         int64_t sbase, int skey, int n_s, double n_spt,
@@ -1156,6 +1198,7 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
   }
   source.least = source.greatest = 0.0;
   Rec pi;
+  const TargetSide ts = target_side(A, w, env, type, target, ct);
   for (int i_samp = 1; i_samp <= n_total; ++i_samp) {
     if (n_rm_here >= n_s) break;  // every source removed
     if (n_samp > n_remove && n_avg >= 2) {
@@ -1188,7 +1231,7 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
     n_samp++;
     load_rec_head(A, ps, pi);
     double kk;
-    int err = weighted_kernel_rec(A, w, env, type, pi, target, cs, ct, ct, &kk);
+    int err = weighted_kernel_tgt(A, w, env, type, pi, ts, cs, ct, &kk);
     if (err) return err;
     double prob_coag = kk * accept_factor;
     prob_coag_tot += prob_coag;
