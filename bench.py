@@ -176,6 +176,30 @@ def cpu_baseline(pops, kernel, dt, budget_s=15.0):
             "particle_steps_per_s": psteps / t_run}
 
 
+def copy_probe(nbytes, launches=5):
+    """A plain device-to-device copy of nbytes, for the scale of FETCH_SIZE and
+    WRITE_SIZE: each launch reads nbytes and writes nbytes."""
+    import torch
+    src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(launches):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "launches": launches}))
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, copy-scaled), or None"""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f)["kernels"].get(kernel)
+    return None if k is None else k["bytes"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,7 +214,13 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-rebalance", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (the scale of "
+                         "the HBM counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of the workload runs on cuda:0; "
               "multi-GPU runs are not part of this build (see DESIGN.md for sharding)",
@@ -232,7 +262,9 @@ def main():
         avg_s = ms / n / 1e3
         achieved = traffic / avg_s / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                    "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": measured_traffic(args.workload, dom) if args.mode == "parallel"
+                    and args.scale == 1.0 else None, "kernel": dom,
                     "avg_launch_ms": ms / n, "launches": n,
                     "algorithmic_bytes_per_launch": traffic}
     base = None if args.no_cpu_baseline else cpu_baseline(pops, kernel, dt, args.cpu_budget)
