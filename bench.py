@@ -207,7 +207,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c4", choices=sorted(WORKLOADS))
     ap.add_argument("--mode", default="parallel", choices=["exact", "parallel"])
-    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32"])
+    ap.add_argument("--precision", default=None, choices=["fp64", "fp32"],
+                    help="fp32 (with the FP64 recheck) by default; the exact mode is FP64 only")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true")
@@ -218,6 +219,8 @@ def main():
                     help="only run a plain device copy of this many bytes (the scale of "
                          "the HBM counters)")
     args = ap.parse_args()
+    if args.precision is None:
+        args.precision = "fp64" if args.mode == "exact" else "fp32"
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
