@@ -1198,7 +1198,8 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
   }
   source.least = source.greatest = 0.0;
   Rec pi;
-  const TargetSide ts = target_side(A, w, env, type, target, ct);
+  TargetSide ts;  // formed at the first sample: many pairs draw none
+  bool have_ts = false;
   for (int i_samp = 1; i_samp <= n_total; ++i_samp) {
     if (n_rm_here >= n_s) break;  // every source removed
     if (n_samp > n_remove && n_avg >= 2) {
@@ -1231,6 +1232,10 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
     n_samp++;
     load_rec_head(A, ps, pi);
     double kk;
+    if (!have_ts) {
+      ts = target_side(A, w, env, type, target, ct);
+      have_ts = true;
+    }
     int err = weighted_kernel_tgt(A, w, env, type, pi, ts, cs, ct, &kk);
     if (err) return err;
     double prob_coag = kk * accept_factor;
