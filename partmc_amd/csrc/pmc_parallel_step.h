@@ -46,7 +46,8 @@ static_assert(PAR_ROUNDS >= 1, "the tail runs on the pending list of a round");
 constexpr int MAX_REDRAW = 64;
 
 struct PW {  // device view of the parallel work arrays
-  int K_stride;     // keys per box in the histogram layout
+  int K_stride;     // keys per box of the list tables (the allocation's grid)
+  int K_hist;       // keys per box of this step's histogram (the step's grid)
   int64_t P_stride; // pair slots per box
   int tab_classes, max_bin;
   int *key;         // [total_cap] list key, -1 for no key
@@ -59,7 +60,7 @@ struct PW {  // device view of the parallel work arrays
   float4 *terms;    // [total_cap]
   int *perm;        // [total_cap] global slots sorted by (box, key)
   int *tile_box, *tile_start, *tiles_in_box;
-  int64_t *hist_off;  // [n_box]
+  int64_t *tile_first;  // [n_box] first tile of the box; its histogram starts at K_hist * this
   uint32_t *hist, *hist_scan;
   int *lstart, *lcnt, *live;  // [n_box * K_stride]
   uint32_t *pair_n, *pair_exc;  // [n_box * P_stride (+1)]
@@ -370,7 +371,7 @@ k_par_hist(BoxDesc *boxes, PW W) {
   const int box = W.tile_box[tile];
   BoxDesc *B = &boxes[box];
   extern __shared__ int s_h[];
-  const int K = W.K_stride;
+  const int K = W.K_hist;
   for (int k = threadIdx.x; k < K; k += TILE_THREADS) s_h[k] = 0;
   __syncthreads();
   const int n = B->error ? 0 : B->n_part;
@@ -382,7 +383,7 @@ k_par_hist(BoxDesc *boxes, PW W) {
   __syncthreads();
   const int t_in_box = W.tile_start[tile] / TILE;
   const int nt = W.tiles_in_box[box];
-  uint32_t *h = W.hist + W.hist_off[box];
+  uint32_t *h = W.hist + (int64_t)K * W.tile_first[box];
   for (int k = threadIdx.x; k < K; k += TILE_THREADS) h[(int64_t)k * nt + t_in_box] = (uint32_t)s_h[k];
 }
 
@@ -392,10 +393,14 @@ __global__ void k_par_lists(BoxDesc *boxes, PW W, int n_box) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= W.K_stride) return;
   const int nt = W.tiles_in_box[box];
-  const int64_t o = W.hist_off[box];
+  const int K = W.K_hist;
+  const int64_t o = (int64_t)K * W.tile_first[box];
   const uint32_t base = W.hist_scan[o];
-  const uint32_t s = W.hist_scan[o + (int64_t)k * nt];
-  const uint32_t e = W.hist_scan[o + (int64_t)k * nt + nt - 1] + W.hist[o + (int64_t)k * nt + nt - 1];
+  // keys beyond the step's grid hold no particle: their lists start at the
+  // end of the box's, as the last key's list ends
+  const int kk = k < K ? k : K - 1;
+  const uint32_t e = W.hist_scan[o + (int64_t)kk * nt + nt - 1] + W.hist[o + (int64_t)kk * nt + nt - 1];
+  const uint32_t s = k < K ? W.hist_scan[o + (int64_t)k * nt] : e;
   W.lstart[(int64_t)box * W.K_stride + k] = (int)(s - base);
   W.lcnt[(int64_t)box * W.K_stride + k] = (int)(e - s);
 }
@@ -434,7 +439,7 @@ k_par_scatter(BoxDesc *boxes, PW W, int key_bits) {
   const int K = W.K_stride;
   const int t_in_box = start / TILE;
   const int nt = W.tiles_in_box[box];
-  const int64_t o = W.hist_off[box];
+  const int64_t o = (int64_t)W.K_hist * W.tile_first[box];
   const uint32_t bbase = W.hist_scan[o];
   for (int it = 0; it < TILE_ITEMS; ++it) {
     const int pos = threadIdx.x * TILE_ITEMS + it;
@@ -2743,7 +2748,7 @@ inline int parallel_alloc_records(ParallelWork &W, int n_spec) {
 inline void parallel_free_arrays(ParallelWork &W) {
   PW &d = W.d;
   void *ptrs[] = {d.key,  d.status, d.claim, d.claim_wr, d.terms, d.perm, d.tile_box, d.tile_start,
-                  d.tiles_in_box, d.hist_off, d.hist, d.hist_scan, d.lstart, d.lcnt, d.live,
+                  d.tiles_in_box, d.tile_first, d.hist, d.hist_scan, d.lstart, d.lcnt, d.live,
                   d.pair_n, d.pair_exc, d.acc, d.acc_scan, d.tp1, d.tp2, d.tid, d.pend, d.ev_t,
                   d.ev_p1, d.ev_p2, d.ev_box, d.ev_out, d.f_app, d.f_id, d.f_info, d.f_hole,
                   d.f_lose, d.f_pend, d.s_pend, d.dead_t, d.dead_r, d.s_app, d.s_id, d.s_info, d.s_hole,
@@ -2812,7 +2817,7 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
       tb.push_back(b);
       ts.push_back(t * TILE);
     }
-    hoff[b] = H;
+    hoff[b] = (int64_t)tb.size() - nt;
     H += (int64_t)d.K_stride * nt;
   }
   W.n_tiles = (int)tb.size();
@@ -2820,11 +2825,12 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.tile_box, tb.size()));
   PCHECK(pmalloc(&d.tile_start, ts.size()));
   PCHECK(pmalloc(&d.tiles_in_box, n_box));
-  PCHECK(pmalloc(&d.hist_off, n_box));
+  PCHECK(pmalloc(&d.tile_first, n_box));
   PCHECK(hipMemcpy(d.tile_box, tb.data(), tb.size() * sizeof(int), hipMemcpyHostToDevice));
   PCHECK(hipMemcpy(d.tile_start, ts.data(), ts.size() * sizeof(int), hipMemcpyHostToDevice));
   PCHECK(hipMemcpy(d.tiles_in_box, W.tiles_in_box.data(), n_box * sizeof(int), hipMemcpyHostToDevice));
-  PCHECK(hipMemcpy(d.hist_off, hoff.data(), n_box * sizeof(int64_t), hipMemcpyHostToDevice));
+  PCHECK(hipMemcpy(d.tile_first, hoff.data(), n_box * sizeof(int64_t), hipMemcpyHostToDevice));
+  d.K_hist = d.K_stride;
   PCHECK(pmalloc(&d.hist, H));
   PCHECK(pmalloc(&d.hist_scan, H));
   PCHECK(pmalloc(&d.lstart, (size_t)n_box * d.K_stride));
@@ -2953,9 +2959,11 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
     hipLaunchKernelGGL(k_mark_tables, gb, bb, 0, st, d_box, n_box);
   }
   // 3. stable counting sort of the keys
-  PLAUNCH(W, "k_par_hist", k_par_hist, gt, bt, d.K_stride * sizeof(int), st, d_box, d);
+  // the histogram spans the step's grid: keys are below mb_step * classes
+  d.K_hist = mb_step * d.tab_classes;
+  PLAUNCH(W, "k_par_hist", k_par_hist, gt, bt, d.K_hist * sizeof(int), st, d_box, d);
   PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.hist, d.hist_scan,
-                                          (int)W.H, st));
+                                          (int)((int64_t)d.K_hist * W.n_tiles), st));
   hipLaunchKernelGGL(k_par_lists, dim3((d.K_stride + 255) / 256, n_box), dim3(256), 0, st, d_box, d,
                      n_box);
   // keys are below mb_step * classes; the sentinel 2^key_bits - 1 above them
