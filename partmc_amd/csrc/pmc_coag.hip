@@ -60,15 +60,15 @@ __constant__ double HALL_ECOLL[315] = {
 using namespace pmc;
 
 // ------------------------------------------------------------------ errors
-static thread_local std::string g_last_error;
+// text of the last error on this thread, written in place (an error path
+// allocates nothing)
+static thread_local char g_last_error[512];
 
 static int fail(int code, const char *fmt, ...) {
-  char buf[512];
   va_list ap;
   va_start(ap, fmt);
-  vsnprintf(buf, sizeof(buf), fmt, ap);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
   va_end(ap);
-  g_last_error = buf;
   return code;
 }
 
@@ -307,9 +307,41 @@ static int check_box(pmc_gpu *h, int box) {
   return 0;
 }
 
+// Size distributions (pmc_stats.h): argument checks shared by the entry
+// points, and the passes of every bin window enqueued on the handle's
+// stream.  partials holds DIST_BLOCKS * min(n_bin, DIST_WIN) doubles and is
+// fully rewritten by each window's partial pass before its reduce reads it.
+static int check_size_dist(pmc_gpu *h, int box, int quantity, int n_bin, double d_min,
+                           double d_max, const double *hist) {
+  int e = check_box(h, box);
+  if (e) return e;
+  if (!hist || n_bin < 1 || !(d_min > 0.0 && d_min < d_max))
+    return fail(ERR_ARG, "size_dist: need an output array, n_bin >= 1 and 0 < d_min < d_max");
+  if (quantity != DIST_NUM && quantity != DIST_MASS && quantity != DIST_VOL)
+    return fail(ERR_ARG, "size_dist: unknown quantity %d", quantity);
+  return 0;
+}
+
+static hipError_t size_dist_enqueue(pmc_gpu *h, int box, int quantity, int n_bin, double d_min,
+                                    double d_max, double *partials, double *d_hist) {
+  // the grid of bin_grid_make (log) over diameters: its end points
+  const double log_lo = std::log(d_min), log_hi = std::log(d_max);
+  for (int lo = 0; lo < n_bin; lo += DIST_WIN) {
+    const int win = std::min(DIST_WIN, n_bin - lo);
+    int key_bits = 1;
+    while ((1 << key_bits) <= win) key_bits++;  // keys 0..win, win for "not in this window"
+    hipLaunchKernelGGL(k_dist_partial, dim3(DIST_BLOCKS), dim3(DIST_THREADS), 0, h->stream,
+                       h->d_box, h->A, box, quantity, n_bin, log_lo, log_hi, lo, win, key_bits,
+                       partials);
+    hipLaunchKernelGGL(k_dist_reduce, dim3((win + 255) / 256), dim3(256), 0, h->stream, partials,
+                       DIST_BLOCKS, lo, win, n_bin, d_hist);
+  }
+  return hipGetLastError();
+}
+
 extern "C" {
 
-const char *pmc_gpu_last_error(void) { return g_last_error.c_str(); }
+const char *pmc_gpu_last_error(void) { return g_last_error; }
 
 int pmc_gpu_create(pmc_gpu **out, int device, int n_box, int n_spec, const double *density,
                    int max_bin) {
@@ -845,66 +877,19 @@ int pmc_gpu_get_path_stats(pmc_gpu *h, int64_t *n_accel_pairs, int64_t *n_droppe
 
 int pmc_gpu_size_dist(pmc_gpu *h, int box, int quantity, int n_bin, double d_min, double d_max,
                       double *hist) {
-  int e = check_box(h, box);
+  int e = check_size_dist(h, box, quantity, n_bin, d_min, d_max, hist);
   if (e) return e;
-  if (!hist || n_bin < 1 || !(d_min > 0.0 && d_min < d_max))
-    return fail(ERR_ARG, "size_dist: need n_bin >= 1 and 0 < d_min < d_max");
-  if (quantity != DIST_NUM && quantity != DIST_MASS && quantity != DIST_VOL)
-    return fail(ERR_ARG, "size_dist: unknown quantity %d", quantity);
   HIPCHECK(hipSetDevice(h->device));
-  HIPCHECK(hipStreamSynchronize(h->stream));
-  for (int i = 0; i < n_bin; ++i) hist[i] = 0.0;
-  int n = 0;
-  HIPCHECK(hipMemcpy(&n, (char *)(h->d_box + box) + offsetof(BoxDesc, n_part), sizeof(int),
-                     hipMemcpyDeviceToHost));
-  if (n == 0) return 0;
-  // the grid of bin_grid_make (log) over diameters: its end points and width
-  const double log_lo = std::log(d_min), log_hi = std::log(d_max);
-  int *key = nullptr, *key_s = nullptr, *ukey = nullptr, *n_runs = nullptr;
-  double *val = nullptr, *val_s = nullptr, *agg = nullptr, *d_hist = nullptr;
-  void *tmp = nullptr;
-  size_t b_sort = 0, b_red = 0;
-  hipError_t err = hipSuccess;
-  auto step = [&](hipError_t x) {
-    if (err == hipSuccess) err = x;
-  };
-  step(hipMalloc((void **)&key, sizeof(int) * n));
-  step(hipMalloc((void **)&key_s, sizeof(int) * n));
-  step(hipMalloc((void **)&ukey, sizeof(int) * n));
-  step(hipMalloc((void **)&n_runs, sizeof(int)));
-  step(hipMalloc((void **)&val, sizeof(double) * n));
-  step(hipMalloc((void **)&val_s, sizeof(double) * n));
-  step(hipMalloc((void **)&agg, sizeof(double) * n));
-  step(hipMalloc((void **)&d_hist, sizeof(double) * n_bin));
-  if (err == hipSuccess) {
-    int key_bits = 1;
-    while ((1 << key_bits) <= n_bin) key_bits++;
-    step(hipcub::DeviceRadixSort::SortPairs(nullptr, b_sort, key, key_s, val, val_s, n, 0,
-                                            key_bits, h->stream));
-    step(hipcub::DeviceReduce::ReduceByKey(nullptr, b_red, key_s, ukey, val_s, agg, n_runs,
-                                           hipcub::Sum(), n, h->stream));
-    step(hipMalloc(&tmp, std::max(b_sort, b_red)));
-    if (err == hipSuccess) {
-      hipLaunchKernelGGL(k_dist_values, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->d_box,
-                         h->A, box, quantity, n_bin, log_lo, log_hi, key, val);
-      size_t bs = std::max(b_sort, b_red);
-      step(hipcub::DeviceRadixSort::SortPairs(tmp, bs, key, key_s, val, val_s, n, 0, key_bits,
-                                              h->stream));
-      bs = std::max(b_sort, b_red);
-      step(hipcub::DeviceReduce::ReduceByKey(tmp, bs, key_s, ukey, val_s, agg, n_runs,
-                                             hipcub::Sum(), n, h->stream));
-      step(hipMemsetAsync(d_hist, 0, sizeof(double) * n_bin, h->stream));
-      hipLaunchKernelGGL(k_dist_scatter, dim3((n + 255) / 256), dim3(256), 0, h->stream, ukey, agg,
-                         n_runs, n_bin, d_hist);
-      step(hipGetLastError());
-      step(hipMemcpyAsync(hist, d_hist, sizeof(double) * n_bin, hipMemcpyDeviceToHost,
-                          h->stream));
-      step(hipStreamSynchronize(h->stream));
-    }
-  }
-  void *ptrs[] = {key, key_s, ukey, n_runs, val, val_s, agg, d_hist, tmp};
-  for (void *p : ptrs)
-    if (p) (void)hipFree(p);
+  double *partials = nullptr, *d_hist = nullptr;
+  hipError_t err = dmalloc(&partials, (size_t)DIST_BLOCKS * std::min(n_bin, DIST_WIN));
+  if (err == hipSuccess) err = dmalloc(&d_hist, n_bin);
+  if (err == hipSuccess)
+    err = size_dist_enqueue(h, box, quantity, n_bin, d_min, d_max, partials, d_hist);
+  if (err == hipSuccess)
+    err = hipMemcpyAsync(hist, d_hist, sizeof(double) * n_bin, hipMemcpyDeviceToHost, h->stream);
+  if (err == hipSuccess) err = hipStreamSynchronize(h->stream);
+  if (partials) (void)hipFree(partials);
+  if (d_hist) (void)hipFree(d_hist);
   if (err != hipSuccess) return fail(ERR_HIP, "size_dist: %s", hipGetErrorString(err));
   return 0;
 }
