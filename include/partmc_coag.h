@@ -156,6 +156,17 @@ int pmc_gpu_get_error_stats(pmc_gpu *h, double *max_p_err);
 #define PMC_DIST_VOL 2
 int pmc_gpu_size_dist(pmc_gpu *h, int box, int quantity, int n_bin, double d_min,
                       double d_max, double *hist);
+/* pmc_gpu_size_dist for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It has no counterpart in the reference.  It allocates
+ * nothing: its partial histograms live in a fixed 8 MiB device array per
+ * GPU, held until the process ends.  d_hist is caller-owned DEVICE memory
+ * of n_bin doubles; the call only enqueues work on the handle's stream
+ * (pmc_gpu_sync waits for it).  n_bin above 2048 returns PMC_ERR_CAPACITY.
+ * Calls that overlap on one GPU share that array and race: serialise
+ * them (one stream, or wait for the previous call). */
+int pmc_gpu_size_dist_noalloc(pmc_gpu *h, int box, int quantity, int n_bin, double d_min,
+                              double d_max, double *d_hist);
 /* Population control.
  * pmc_gpu_set_n_part_ideal: aero_state_set_n_part_ideal
  * (src/aero_state.F90:241-258); the box's ideal count is split evenly over

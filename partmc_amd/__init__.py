@@ -42,6 +42,7 @@ ABI = {
     "pmc_gpu_get_part_steps": (_i, [_p, _p]),
     "pmc_gpu_get_path_stats": (_i, [_p, _p, _p, _p, _p]),
     "pmc_gpu_size_dist": (_i, [_p, _i, _i, _i, _d, _d, _p]),
+    "pmc_gpu_size_dist_noalloc": (_i, [_p, _i, _i, _i, _d, _d, _p]),
     "pmc_gpu_set_n_part_ideal": (_i, [_p, _i, _d]),
     "pmc_gpu_clear_removed": (_i, [_p, _i]),
     "pmc_gpu_dilute": (_i, [_p, _p]),
@@ -334,6 +335,14 @@ class CoagGPU:
         out = np.zeros(n_bin)
         _check(load().pmc_gpu_size_dist(self._h, box, q, n_bin, d_min, d_max, _ptr(out)))
         return out
+
+    def size_dist_noalloc(self, box, n_bin, d_min, d_max, d_hist, quantity="num"):
+        """size_dist into caller-owned device memory (the integer address
+        d_hist of n_bin doubles), enqueued on the handle's stream without
+        any allocation; sync() waits for it."""
+        q = {"num": 0, "mass": 1, "vol": 2}[quantity]
+        _check(load().pmc_gpu_size_dist_noalloc(self._h, box, q, n_bin, d_min, d_max,
+                                                ctypes.c_void_p(int(d_hist))))
 
     def path_stats(self):
         """Per box: pairs on the per-particle path and dropped draws; for

@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -339,6 +340,32 @@ static hipError_t size_dist_enqueue(pmc_gpu *h, int box, int quantity, int n_bin
   return hipGetLastError();
 }
 
+// Scratch of pmc_gpu_size_dist_noalloc: the partial histograms of its
+// DIST_BLOCKS workgroups over one window of DIST_WIN bins, 8 MiB.  Every
+// device the library runs on holds its own copy until the process ends.
+// On the host the name only stands for the symbol: kernels get the device
+// address that scratch_setup records per device.
+static __device__ __align__(256) double scratch[DIST_BLOCKS * DIST_WIN];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static double *g_scratch[SCRATCH_MAX_DEVICES];  // device address of `scratch`, per device
+static std::mutex g_scratch_mu;
+
+// Called with `device` current, once per handle; the address is looked up
+// once per device.
+static int scratch_setup(int device) {
+  if (device < 0 || device >= SCRATCH_MAX_DEVICES)
+    return fail(ERR_ARG, "device %d out of range [0, %d)", device, SCRATCH_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return 0;
+  void *p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device,
+                hipGetErrorString(e));
+  g_scratch[device] = (double *)p;
+  return 0;
+}
+
 extern "C" {
 
 const char *pmc_gpu_last_error(void) { return g_last_error; }
@@ -356,6 +383,7 @@ int pmc_gpu_create(pmc_gpu **out, int device, int n_box, int n_spec, const doubl
   h->density.assign(density, density + n_spec);
   HIPCHECK(hipSetDevice(device));
   HIPCHECK(hipStreamCreate(&h->stream));
+  if (int e = scratch_setup(device)) return e;
   Arrays &A = h->A;
   A.n_spec = n_spec;
   A.max_bin = max_bin;
@@ -891,6 +919,21 @@ int pmc_gpu_size_dist(pmc_gpu *h, int box, int quantity, int n_bin, double d_min
   if (partials) (void)hipFree(partials);
   if (d_hist) (void)hipFree(d_hist);
   if (err != hipSuccess) return fail(ERR_HIP, "size_dist: %s", hipGetErrorString(err));
+  return 0;
+}
+
+int pmc_gpu_size_dist_noalloc(pmc_gpu *h, int box, int quantity, int n_bin, double d_min,
+                              double d_max, double *d_hist) {
+  int e = check_size_dist(h, box, quantity, n_bin, d_min, d_max, d_hist);
+  if (e) return e;
+  if (n_bin > DIST_WIN)
+    return fail(ERR_CAPACITY,
+                "size_dist_noalloc: n_bin %d exceeds the %d bins of its fixed scratch "
+                "(pmc_gpu_size_dist takes any n_bin)", n_bin, DIST_WIN);
+  HIPCHECK(hipSetDevice(h->device));
+  double *partials = g_scratch[h->device];
+  if (!partials) return fail(ERR_STATE, "size_dist_noalloc: no scratch on device %d", h->device);
+  HIPCHECK(size_dist_enqueue(h, box, quantity, n_bin, d_min, d_max, partials, d_hist));
   return 0;
 }
 
