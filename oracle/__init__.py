@@ -66,6 +66,7 @@ def lib():
             "orc_student_t_95_coeff": (_d, [_i]),
             "orc_fall_g": (None, [_d, _p]),
             "orc_effic": (_i, [_d, _d, _p]),
+            "orc_brown_terms": (None, [_d, _d, _d, _d, _p]),
             "orc_kernel_brown_helper": (_d, [_d, _d, _d, _d, _d, _d]),
             "orc_kernel_sedi_helper": (_i, [_d, _d, _p]),
             "orc_kernel": (_i, [_i, _i, _p, _p, _p, _d, _d, _d, _p]),
@@ -110,6 +111,9 @@ def lib():
             "orc_loss_rate": (_i, [_i, _d, _d, _d, _d, _d, _p]),
             "orc_box_dilute": (_i, [_p, _d, _u32]),
             "orc_box_particle_loss": (_i, [_p, _i, _d, _d, _u32]),
+            "orc_box_rand_group": (_i, [_p, _u64, _u32, _i, _d]),
+            "orc_box_coagulate_components": (_i, [_p, _u64, _u32, _i, _i, _p, _i, _i, _p, _p,
+                                                  _p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -203,6 +207,17 @@ def kernel(kind, density, vol1, vol2, temp=288.0, pressure=1e5, beta=1000.0):
     if err:
         raise RuntimeError(f"oracle kernel error {err}")
     return float(k[0])
+
+
+def brown_terms(vol, density, temp=288.0, pressure=1e5):
+    """kernel_brown_helper's air and particle terms: air mean free path,
+    radius, Knudsen number, Cunningham factor, diffusivity, mean speed
+    squared, particle mean free path and delta squared (SI units)."""
+    out = np.zeros(8)
+    lib().orc_brown_terms(vol, density, temp, pressure, _ptr(out))
+    keys = ("air_mfp", "radius", "knudsen", "cunningham", "diffusivity", "speed_sq",
+            "mfp", "delta_sq")
+    return dict(zip(keys, out))
 
 
 def est_k_minmax(kind, density, edges, temp=288.0, pressure=1e5, beta=1000.0):
@@ -411,6 +426,25 @@ class Box:
         m = np.zeros(ng * nc)
         lib().orc_box_get_weights(self._h, _ptr(m))
         return m.reshape(nc, ng).T.copy()
+
+    def rand_group(self, seed, ordinal, klass, radius):
+        """aero_weight_array_rand_group on stream (seed, ordinal)."""
+        return lib().orc_box_rand_group(self._h, seed, ordinal, klass, radius)
+
+    def coagulate_components(self, seed, ordinal, src1, np1, src2, np2):
+        """The merged particle's component sources from aero_particle_coagulate
+        of two particles with component sources src1, src2 and np1, np2
+        primary parts, on stream (seed, ordinal)."""
+        s1 = np.ascontiguousarray(src1, dtype=np.int32)
+        s2 = np.ascontiguousarray(src2, dtype=np.int32)
+        n = np.zeros(1, dtype=np.int32)
+        out = np.zeros(MAX_COMP, dtype=np.int32)
+        err = lib().orc_box_coagulate_components(self._h, seed, ordinal, len(s1), np1,
+                                                 _ptr(s1), len(s2), np2, _ptr(s2), _ptr(n),
+                                                 _ptr(out))
+        if err:
+            raise RuntimeError(f"oracle coagulate error {err}")
+        return out[: int(n[0])].copy()
 
     def mc_coag(self, kernel_type, del_t, step):
         ns = np.zeros(1, dtype=np.int64)

@@ -356,10 +356,12 @@ void orc_histogram_1d(int n_bin, const double *edges, const double *widths,
 /* ------------------------------------------------------------------ */
 /* kernels                                                             */
 /* ------------------------------------------------------------------ */
-/* kernel_brown_helper, src/coag_kernel_brown.F90:111-197 (spherical
- * particles: mobility radius = geometric radius, src/fractal.F90:334-337) */
-double orc_kernel_brown_helper(double vol_i, double den_i, double vol_j,
-                               double den_j, double temp, double pressure) {
+/* The air and one particle's terms of kernel_brown_helper
+ * (src/coag_kernel_brown.F90:143-189), in its order of operations:
+ * out = {air mean free path, radius, Knudsen number, Cunningham factor,
+ *        diffusivity, mean speed squared, particle mean free path, delta^2}. */
+void orc_brown_terms(double vol, double den, double temp, double pressure,
+                     double out[8]) {
     double rhoair = (pressure * C_AIR_MW) / (C_UNIV_GAS * temp);
     double viscosd = 1.8325e-05 * (416.16 / (temp + 120.0)) *
                      pow(temp / 296.16, 1.5);
@@ -368,36 +370,42 @@ double orc_kernel_brown_helper(double vol_i, double den_i, double vol_j,
                            (C_PI * C_AIR_MW));
     double gasfreepath = 2.0 * viscosk / gasspeed;
 
-    double rad_i = orc_sphere_vol2rad(vol_i);
-    double Rme_i = orc_sphere_vol2rad(vol_i);
-    double knud = gasfreepath / Rme_i;
+    double Rme = orc_sphere_vol2rad(vol);
+    double knud = gasfreepath / Rme;
     double cunning = 1.0 + knud * (1.249 + 0.42 * exp(-0.87 / knud));
-    double diffus_i = (C_BOLTZMANN * temp * cunning) /
-                      (6.0 * C_PI * Rme_i * viscosd);
-    double speedsq_i = 8.0 * C_BOLTZMANN * temp / (C_PI * den_i * vol_i);
-    double freepath = 8.0 * diffus_i / (C_PI * sqrt(speedsq_i));
-    double tmp1 = orc_powi(2.0 * Rme_i + freepath, 3);
-    double tmp2 = pow(4.0 * Rme_i * Rme_i + freepath * freepath, 1.5);
-    double d = (tmp1 - tmp2) / (6.0 * Rme_i * freepath) - 2.0 * Rme_i;
-    double deltasq_i = d * d;
+    double diffus = (C_BOLTZMANN * temp * cunning) /
+                    (6.0 * C_PI * Rme * viscosd);
+    double speedsq = 8.0 * C_BOLTZMANN * temp / (C_PI * den * vol);
+    double freepath = 8.0 * diffus / (C_PI * sqrt(speedsq));
+    double tmp1 = orc_powi(2.0 * Rme + freepath, 3);
+    double tmp2 = pow(4.0 * Rme * Rme + freepath * freepath, 1.5);
+    double d = (tmp1 - tmp2) / (6.0 * Rme * freepath) - 2.0 * Rme;
+    out[0] = gasfreepath;
+    out[1] = Rme;
+    out[2] = knud;
+    out[3] = cunning;
+    out[4] = diffus;
+    out[5] = speedsq;
+    out[6] = freepath;
+    out[7] = d * d;
+}
 
+/* kernel_brown_helper, src/coag_kernel_brown.F90:111-197 (spherical
+ * particles: mobility radius = geometric radius, src/fractal.F90:334-337) */
+double orc_kernel_brown_helper(double vol_i, double den_i, double vol_j,
+                               double den_j, double temp, double pressure) {
+    double ti[8], tj[8];
+    orc_brown_terms(vol_i, den_i, temp, pressure, ti);
+    orc_brown_terms(vol_j, den_j, temp, pressure, tj);
+    double rad_i = orc_sphere_vol2rad(vol_i);
     double rad_j = orc_sphere_vol2rad(vol_j);
-    double Rme_j = orc_sphere_vol2rad(vol_j);
-    knud = gasfreepath / Rme_j;
-    cunning = 1.0 + knud * (1.249 + 0.42 * exp(-0.87 / knud));
-    double diffus_j = (C_BOLTZMANN * temp * cunning) /
-                      (6.0 * C_PI * Rme_j * viscosd);
-    double speedsq_j = 8.0 * C_BOLTZMANN * temp / (C_PI * den_j * vol_j);
-    freepath = 8.0 * diffus_j / (C_PI * sqrt(speedsq_j));
-    tmp1 = orc_powi(2.0 * Rme_j + freepath, 3);
-    tmp2 = pow(4.0 * Rme_j * Rme_j + freepath * freepath, 1.5);
-    d = (tmp1 - tmp2) / (6.0 * Rme_j * freepath) - 2.0 * Rme_j;
-    double deltasq_j = d * d;
+    double diffus_i = ti[4], speedsq_i = ti[5], deltasq_i = ti[7];
+    double diffus_j = tj[4], speedsq_j = tj[5], deltasq_j = tj[7];
 
     double rad_sum = rad_i + rad_j;
     double diffus_sum = diffus_i + diffus_j;
-    tmp1 = rad_sum / (rad_sum + sqrt(deltasq_i + deltasq_j));
-    tmp2 = 4.0 * diffus_sum / (rad_sum * sqrt(speedsq_i + speedsq_j));
+    double tmp1 = rad_sum / (rad_sum + sqrt(deltasq_i + deltasq_j));
+    double tmp2 = 4.0 * diffus_sum / (rad_sum * sqrt(speedsq_i + speedsq_j));
     return 4.0 * C_PI * rad_sum * diffus_sum / (tmp1 + tmp2);
 }
 
@@ -2033,4 +2041,44 @@ void orc_sample_pmf(uint64_t seed, int which, double p1, double p2, int64_t n,
                              : orc_rand_binomial(&s, (int)p1, p2);
         if (k >= 0 && k <= kmax) counts[k]++;
     }
+}
+
+/* ------------------------------------------------------------------ */
+/* single draws of rand_group and of the component part of             */
+/* aero_particle_coagulate, for their statistics tests                 */
+/* ------------------------------------------------------------------ */
+/* aero_weight_array_rand_group (src/aero_weight_array.F90:374-393) of
+ * box b's weights on stream (seed, ordinal) */
+int orc_box_rand_group(orc_box *b, uint64_t seed, uint32_t ordinal, int klass,
+                       double radius) {
+    orc_stream s;
+    orc_stream_open(&s, seed, 0, 0, 0, ordinal, 0);
+    return rand_group(b, &s, klass, radius);
+}
+
+/* aero_particle_coagulate's components (src/aero_particle.F90:960-994) of
+ * two particles with n1 and n2 components of sources src1 and src2 and
+ * np1, np2 primary parts, on stream (seed, ordinal): n_out and src_out
+ * receive the merged particle's components */
+int orc_box_coagulate_components(orc_box *b, uint64_t seed, uint32_t ordinal,
+                                 int n1, int np1, const int *src1, int n2,
+                                 int np2, const int *src2, int *n_out,
+                                 int *src_out) {
+    if (n1 < 0 || n2 < 0 || n1 > ORC_MAX_COMP || n2 > ORC_MAX_COMP) return 1;
+    prec p1, p2, pn;
+    memset(&p1, 0, sizeof(p1));
+    memset(&p2, 0, sizeof(p2));
+    p1.n_comp = n1;
+    p1.n_primary = np1;
+    p2.n_comp = n2;
+    p2.n_primary = np2;
+    for (int i = 0; i < n1; ++i) p1.comp_source[i] = src1[i];
+    for (int i = 0; i < n2; ++i) p2.comp_source[i] = src2[i];
+    orc_stream s;
+    orc_stream_open(&s, seed, 0, 0, 0, ordinal, 0);
+    int err = particle_coagulate(b, &s, &p1, &p2, &pn);
+    if (err) return err;
+    *n_out = pn.n_comp;
+    for (int i = 0; i < pn.n_comp; ++i) src_out[i] = pn.comp_source[i];
+    return 0;
 }

@@ -104,6 +104,8 @@ int orc_bin_grid_find(int n_bin, const double *edges, double r);
 double orc_student_t_95_coeff(int n_sample);
 void orc_fall_g(double r, double *w_inf);
 int orc_effic(double r1, double r2, double *ec);
+void orc_brown_terms(double vol, double den, double temp, double pressure,
+                     double out[8]);
 double orc_kernel_brown_helper(double vol_i, double den_i, double vol_j,
                                double den_j, double temp, double pressure);
 int orc_kernel_sedi_helper(double v1, double v2, double *k);
@@ -213,6 +215,15 @@ int orc_loss_rate(int type, double vol, double density, double temp,
 int orc_box_dilute(orc_box *b, double sample_prob, uint32_t step);
 int orc_box_particle_loss(orc_box *b, int loss_type, double del_t,
                           double height, uint32_t step);
+
+/* single draws of rand_group and of aero_particle_coagulate's components
+ * on stream (seed, ordinal), for their statistics tests */
+int orc_box_rand_group(orc_box *b, uint64_t seed, uint32_t ordinal, int klass,
+                       double radius);
+int orc_box_coagulate_components(orc_box *b, uint64_t seed, uint32_t ordinal,
+                                 int n1, int np1, const int *src1, int n2,
+                                 int np2, const int *src2, int *n_out,
+                                 int *src_out);
 
 #ifdef __cplusplus
 }
