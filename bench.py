@@ -14,6 +14,7 @@ the CPU oracle timed on a bounded sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -66,11 +67,36 @@ def kernel_bytes(name, n_spec, precision, part_steps_per_launch, tests_per_launc
     return 0.0
 
 
+# What limits each kernel: the bandwidth roofline bounds the streaming
+# passes; the walks and the tail are chains of dependent loads and FP64
+# special functions, one walk or chain per lane.
+LIMITER = {
+    "k_par_acc_tests": "latency: each target walk is one lane's chain of dependent loads "
+                       "and FP64 special functions",
+    "k_par_tail": "latency: one workgroup per box runs the remaining claim rounds, one link "
+                  "of a chain per round",
+    "k_exact_sweep": "latency: one wavefront per box follows the sequential chain",
+    "k_par_tests": "latency of random gathers of two particles per test",
+    "k_par_apply": "hbm: random gathers of the merged particles' records",
+}
+
 TIMED = ("k_prepare", "k_exact_sweep", "k_par_hist", "k_par_scatter", "k_par_pairs",
          "k_par_tests", "k_par_acc_tests", "k_par_apply", "k_par_tail", "k_par_compact")
 
 
-def run_gpu(pops, kernel, dt, mode, precision, steps, warmup, seed=20261202, rebalance=False):
+def state_digest(g, n_box):
+    """SHA-1 over every box's particle ids and species volumes, in storage
+    order: equal digests mean equal final states."""
+    h = hashlib.sha1()
+    for b in range(n_box):
+        ids, vol = g.download_ids_vols(b)
+        h.update(ids.tobytes())
+        h.update(vol.tobytes())
+    return h.hexdigest()
+
+
+def run_gpu(pops, kernel, dt, mode, precision, steps, warmup, seed=20261202, rebalance=False,
+            validation=False, digest=False):
     g = CoagGPU(len(pops), pops[0].density, max_bin=256)
     g.reserve([p.n_part + p.n_part // 4 + 1024 for p in pops])
     g.set_rng(seed, 0)
@@ -79,11 +105,12 @@ def run_gpu(pops, kernel, dt, mode, precision, steps, warmup, seed=20261202, reb
     g.set_mode(partmc_amd.MODE_EXACT if mode == "exact" else partmc_amd.MODE_PARALLEL,
                partmc_amd.PREC_FP64 if precision == "fp64" else partmc_amd.PREC_FP32_RECHECK,
                1e-5)
+    g.set_validation(validation)
     for _ in range(warmup):
         g.mc_coag_async(kernel, dt)
     g.sync()
     g.reset_counters()
-    g.set_timing(True)
+    g.set_timing(not validation)
     g.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -98,12 +125,15 @@ def run_gpu(pops, kernel, dt, mode, precision, steps, warmup, seed=20261202, reb
         "tail_rounds": st["n_tail_rounds"],
         "accel_pairs": int(st["n_accel_pairs"].sum()),
         "acc_samples": st["acc_samples"],
+        "n_dropped": int(st["n_dropped"].sum()),
         "elapsed": elapsed,
         "n_samp": int(c["n_samp"].sum()),
         "n_coag": int(c["n_coag"].sum()),
         "n_warn": int(c["n_warn"].sum()),
         "n_recheck": int(c["n_recheck"].sum()),
         "n_flip": int(c["n_flip"].sum()),
+        "max_rel_err": float(c["max_rel_err"].max()),
+        "max_p_err": float(g.error_stats()["max_p_err"].max()),
         "part_steps": int(g.part_steps().sum()),
         "timings": {},
     }
@@ -112,6 +142,8 @@ def run_gpu(pops, kernel, dt, mode, precision, steps, warmup, seed=20261202, reb
         if n:
             out["timings"][name] = (ms, n)
     g.set_timing(False)
+    if digest:
+        out["digest"] = state_digest(g, len(pops))
     if rebalance:
         out["rebalance"] = time_rebalance(g, len(pops))
     g.close()
@@ -146,22 +178,25 @@ def time_rebalance(g, n_box):
             "double_particles_per_s": sum(n1) / t_double}
 
 
-def cpu_baseline(pops, kernel, dt, budget_s=15.0):
-    """The CPU oracle (sequential FP64 restatement of mc_coag), one core,
-    on the first box of the workload for as many steps as fit the budget."""
+def oracle_run(args):
+    """The CPU oracle (the sequential FP64 restatement of mc_coag, built
+    -O3) on one box for about budget_s seconds of mc_coag calls: test pairs,
+    particle-steps, steps and seconds.  Runs in a process that never loads
+    the HIP library."""
+    pop, kernel, dt, budget_s, seed = args
     import oracle  # the checker, used here only as the timed CPU baseline
-    pop = pops[0]
+    lib_fast = os.path.join(ROOT, "oracle", "libcoag_oracle_fast.so")
+    if os.path.exists(lib_fast):
+        oracle.LIB_PATH = lib_fast
     w = oracle.Weights([[pop.weight_type]], [[pop.weight_magnitude]], [[pop.weight_exponent]])
-    b = oracle.Box(pop.density, w, pop.temp, pop.pressure, pop.beta, seed=1)
+    b = oracle.Box(pop.density, w, pop.temp, pop.pressure, pop.beta, seed=seed)
     n = pop.n_part
     b.add_particles(pop.vol, pop.ids(), np.ones(n), np.ones(n))
     b.next_id = n + 1
     if pop.grid is not None:
         b.sort_with_grid(*pop.grid)
     b.mc_coag(kernel, dt, 0)  # first call sorts and builds the k bounds
-    samp = 0
-    psteps = 0
-    steps = 0
+    samp = psteps = steps = 0
     t_run = 0.0
     while t_run < budget_s and steps < 100000:
         psteps += b.n_part
@@ -170,10 +205,47 @@ def cpu_baseline(pops, kernel, dt, budget_s=15.0):
         t_run += time.perf_counter() - t0
         samp += s
         steps += 1
-    return {"value": samp / t_run, "unit": "test-pairs/s", "cores": 1, "kind": "port",
-            "sample": f"box 0 of the workload ({n} particles), {steps} steps, "
-                      f"{t_run:.1f} s of mc_coag",
-            "particle_steps_per_s": psteps / t_run}
+    return samp, psteps, steps, t_run, n, os.path.basename(oracle.LIB_PATH)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pops, kernel, dt, budget_s=15.0, procs=16):
+    """The CPU oracle timed on the host, before any GPU call: one process on
+    box 0, then `procs` processes side by side on boxes 0..procs-1 (boxes
+    cycle when the workload has fewer), each with its own seed, as the
+    reference runs independent repeats one per core.  Rates are sums over
+    the processes of their own test pairs (particle-steps) per second."""
+    import multiprocessing as mp
+    one = oracle_run((pops[0], kernel, dt, budget_s, 1))
+    n_cpu = os.cpu_count() or 1
+    procs = max(1, min(procs, n_cpu))
+    ctx = mp.get_context("spawn")
+    jobs = [(pops[i % len(pops)], kernel, dt, budget_s, 1 + i) for i in range(procs)]
+    with ctx.Pool(procs) as pool:
+        many = pool.map(oracle_run, jobs)
+    rate = sum(r[0] / r[3] for r in many)
+    prate = sum(r[1] / r[3] for r in many)
+    return {"value": one[0] / one[3], "unit": "test-pairs/s", "cores": 1, "kind": "port",
+            "sample": f"box 0 of the workload ({one[4]} particles), {one[2]} steps, "
+                      f"{one[3]:.1f} s of mc_coag in one process ({one[5]})",
+            "particle_steps_per_s": one[1] / one[3],
+            "cpu_model": cpu_model(), "host_cpus": n_cpu,
+            "per_core_processes": {
+                "processes": procs, "value": rate, "unit": "test-pairs/s",
+                "particle_steps_per_s": prate,
+                "sample": f"boxes 0..{procs - 1} of the workload, one process each "
+                          f"({sum(r[4] for r in many)} particles), about {budget_s:.0f} s "
+                          "of mc_coag each, side by side"}}
 
 
 def copy_probe(nbytes, launches=5):
@@ -187,6 +259,21 @@ def copy_probe(nbytes, launches=5):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes": nbytes, "launches": launches}))
+
+
+def probe_rates():
+    """Measured FP32 FMA, FP64 FMA and FP32 transcendental rates of this GPU
+    (partmc_amd/libpmc_probe.so): the ceilings of kernel evaluation."""
+    import ctypes
+    lib = ctypes.CDLL(os.path.join(ROOT, "partmc_amd", "libpmc_probe.so"))
+    rates = (ctypes.c_double * 3)()
+    if lib.pmc_probe_rates(0, rates) != 0:
+        raise RuntimeError("pmc_probe_rates failed")
+    out = {"fp32_fma_per_s": rates[0], "fp32_tflops": 2 * rates[0] / 1e12,
+           "fp64_fma_per_s": rates[1], "fp64_tflops": 2 * rates[1] / 1e12,
+           "fp32_transcendental_per_s": rates[2]}
+    print(json.dumps({"probe": out}))
+    return out
 
 
 def measured_traffic(workload, kernel):
@@ -215,6 +302,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-rebalance", action="store_true")
+    ap.add_argument("--no-validation", action="store_true")
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="processes of the per-core CPU baseline (at most the host's CPUs)")
+    ap.add_argument("--probe", action="store_true",
+                    help="only measure the FP32/FP64 FMA and FP32 transcendental rates")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a plain device copy of this many bytes (the scale of "
                          "the HBM counters)")
@@ -223,6 +315,9 @@ def main():
         args.precision = "fp64" if args.mode == "exact" else "fp32"
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.probe:
+        probe_rates()
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of the workload runs on cuda:0; "
@@ -235,16 +330,26 @@ def main():
     pops = build(args.scale)
     t_gen = time.perf_counter() - t_gen
     n_spec = pops[0].n_spec
+    # the CPU baseline first, in processes that never touch the GPU
+    base = None if args.no_cpu_baseline else cpu_baseline(pops, kernel, dt, args.cpu_budget,
+                                                         args.cpu_procs)
+    compare64 = (args.mode == "parallel" and args.precision == "fp32"
+                 and not args.no_fp64_compare)
     r = run_gpu(pops, kernel, dt, args.mode, args.precision, args.steps, args.warmup,
-                seed=args.seed, rebalance=not args.no_rebalance)
+                seed=args.seed, rebalance=not args.no_rebalance, digest=compare64)
     elapsed = r["elapsed"]
     n_samp, n_coag, psteps = r["n_samp"], r["n_coag"], r["part_steps"]
     timings = r["timings"]
     # the plain-FP64 comparison pass on the same populations and streams
     r64 = None
-    if args.mode == "parallel" and args.precision == "fp32" and not args.no_fp64_compare:
+    if compare64:
         r64 = run_gpu(pops, kernel, dt, args.mode, "fp64", args.steps, args.warmup,
-                      seed=args.seed)
+                      seed=args.seed, digest=True)
+    # the validation pass: FP32 with every test also evaluated in FP64
+    rv = None
+    if args.precision == "fp32" and not args.no_validation:
+        rv = run_gpu(pops, kernel, dt, args.mode, "fp32", args.steps, args.warmup,
+                     seed=args.seed, validation=True)
     roofline = None
     if timings:
         dom = max(timings, key=lambda k: timings[k][0])
@@ -268,9 +373,9 @@ def main():
                     "frac": achieved / HBM_PEAK_GBS,
                     "traffic": measured_traffic(args.workload, dom) if args.mode == "parallel"
                     and args.scale == 1.0 else None, "kernel": dom,
+                    "limiter": LIMITER.get(dom, "hbm"),
                     "avg_launch_ms": ms / n, "launches": n,
                     "algorithmic_bytes_per_launch": traffic}
-    base = None if args.no_cpu_baseline else cpu_baseline(pops, kernel, dt, args.cpu_budget)
     total_particles = sum(p.n_part for p in pops)
     line = {
         "metric": "coag test-pairs/s",
@@ -299,6 +404,16 @@ def main():
         "accel_pairs_per_step": r["accel_pairs"] / args.steps,
         "acc_samples_per_step": r["acc_samples"] / args.steps,
         "fp64_rechecks": r["n_recheck"],
+        "dropped_draws": r["n_dropped"],
+        "precision_check": None if rv is None else {
+            "decision_flips": rv["n_flip"],
+            "max_rel_kernel_err": rv["max_rel_err"],
+            "max_abs_p_err": rv["max_p_err"],
+            "recheck_band": 1e-5,
+            "tests_checked": rv["n_samp"],
+            "fp64_rechecks": rv["n_recheck"],
+            "note": "validation pass: the same steps with every FP32 test also evaluated in "
+                    "FP64; flips count tests whose FP32+recheck decision differs from FP64's"},
         "kernel_ms": {k: v[0] / v[1] for k, v in timings.items()},
         "kernel_total_ms": {k: v[0] for k, v in timings.items()},
         "fp64_compare": None if r64 is None else {
@@ -306,7 +421,8 @@ def main():
             "ms_per_step": r64["elapsed"] / args.steps * 1e3,
             "k_par_tests_ms": r64["timings"].get("k_par_tests", (0.0, 1))[0] / max(
                 r64["timings"].get("k_par_tests", (0.0, 1))[1], 1),
-            "same_decisions": r64["n_samp"] == n_samp and r64["n_coag"] == n_coag,
+            "same_final_state": r64["digest"] == r["digest"],
+            "state_digest": r["digest"],
             "ratio_step": (n_samp / elapsed) / max(r64["n_samp"] / r64["elapsed"], 1e-30),
             "ratio_test_kernel": (r64["timings"].get("k_par_tests", (0.0, 1))[0] /
                                   max(timings.get("k_par_tests", (1e-30, 1))[0], 1e-30)),

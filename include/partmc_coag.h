@@ -145,6 +145,39 @@ int pmc_gpu_reset_counters(pmc_gpu *h);
 /* Largest |p32 - p64| of the acceptance probability p = k/k_max seen in
  * validation mode, per box (the quantity the recheck band must exceed). */
 int pmc_gpu_get_error_stats(pmc_gpu *h, double *max_p_err);
+/* Record and replay of a box's random draws (exact mode).  The trace is
+ * the sequence of uniforms in [0, 1) that pmc_random returns
+ * (src/rand.F90:138-166), in call order.  A run without
+ * GSL draws everything from pmc_random, so a trace written at that one seam
+ * by a reference run drives this library through the same decisions, and a
+ * trace this library records drives a reference run.
+ * pmc_gpu_record: the next steps write every uniform the box draws (up to
+ *   capacity values; beyond it the step fails with PMC_ERR_CAPACITY).
+ * pmc_gpu_replay: the next steps draw the box's uniforms from u[0..n) in
+ *   order instead of the call-site streams.
+ * ref_samplers != 0: Poisson counts above mean 10 and binomials with np and
+ *   n(1-p) >= 10 use the reference's rounded normal (src/rand.F90:298-302,
+ *   359-361) instead of the exact samplers, as a run without GSL draws them.
+ * pmc_gpu_get_trace: the uniforms used so far (n_used) and, when recording,
+ *   the values (up to max_n); ran_out != 0 if the tape ran out.
+ * pmc_gpu_trace_off: back to the call-site streams.
+ * The parallel mode refuses to step a box with a tape (PMC_ERR_STATE). */
+int pmc_gpu_record(pmc_gpu *h, int box, int64_t capacity, int ref_samplers);
+int pmc_gpu_replay(pmc_gpu *h, int box, const double *u, int64_t n, int ref_samplers);
+int pmc_gpu_get_trace(pmc_gpu *h, int box, double *u, int64_t max_n, int64_t *n_used,
+                      int *ran_out);
+int pmc_gpu_trace_off(pmc_gpu *h, int box);
+/* Parallel mode: per box, the particles that joined a (bin, class) list
+ * within a step since the last counter reset (merged particles that landed
+ * in another list, appended particles), and how many of them found the
+ * list's reserved arrival entries full (the later pairs of that step did
+ * not see them).  No reference counterpart: the reference's lists are
+ * sequential (src/aero_sorted.F90:397-494). */
+int pmc_gpu_get_arrival_stats(pmc_gpu *h, int64_t *n_arrivals, int64_t *n_overflow);
+/* Per box, since the last reset: the samples and coagulations that the
+ * per-particle path (try_per_particle_coag, src/coagulation.F90:150-241)
+ * added to tot_n_samp and tot_n_coag (:234-235), in either mode. */
+int pmc_gpu_get_accel_counts(pmc_gpu *h, int64_t *n_samp, int64_t *n_coag);
 /* Size distribution of one box on a log-spaced diameter grid of n_bin bins
  * over [d_min, d_max] (m): bin_grid_histogram_1d (src/bin_grid.F90:225-248)
  * of the particle diameters weighted by their number concentrations, as

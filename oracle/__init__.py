@@ -112,6 +112,8 @@ def lib():
             "orc_box_dilute": (_i, [_p, _d, _u32]),
             "orc_box_particle_loss": (_i, [_p, _i, _d, _d, _u32]),
             "orc_box_rand_group": (_i, [_p, _u64, _u32, _i, _d]),
+            "orc_box_set_tape": (_i, [_p, _i, _p, _i64, _i]),
+            "orc_box_tape_state": (_i64, [_p, _p, _i64, _p]),
             "orc_box_coagulate_components": (_i, [_p, _u64, _u32, _i, _i, _p, _i, _i, _p, _p,
                                                   _p]),
         }
@@ -130,7 +132,7 @@ def _ptr(a: np.ndarray):
 class Stream(ctypes.Structure):
     _fields_ = [("key", ctypes.c_uint32 * 2), ("ctr", ctypes.c_uint32 * 4),
                 ("buf", ctypes.c_uint32 * 4), ("have", ctypes.c_int),
-                ("block", ctypes.c_uint32)]
+                ("block", ctypes.c_uint32), ("tape", ctypes.c_void_p)]
 
     def __init__(self, seed, box, step, site, ordinal, kind):
         super().__init__()
@@ -445,6 +447,28 @@ class Box:
         if err:
             raise RuntimeError(f"oracle coagulate error {err}")
         return out[: int(n[0])].copy()
+
+    def record(self, capacity, ref_samplers=False):
+        """Write every uniform the next mc_coag calls draw, in call order."""
+        if lib().orc_box_set_tape(self._h, 1, None, int(capacity), int(bool(ref_samplers))):
+            raise RuntimeError("oracle record")
+
+    def replay(self, uniforms, ref_samplers=False):
+        """Draw the next mc_coag calls' uniforms from `uniforms`, in order."""
+        u = np.ascontiguousarray(uniforms, dtype=np.float64)
+        self._tape = u
+        if lib().orc_box_set_tape(self._h, 2, _ptr(u), len(u), int(bool(ref_samplers))):
+            raise RuntimeError("oracle replay")
+
+    def trace(self, max_n=0):
+        """(uniforms used, recorded values up to max_n, ran out)."""
+        out = np.zeros(max(int(max_n), 1))
+        ro = np.zeros(1, dtype=np.int32)
+        used = lib().orc_box_tape_state(self._h, _ptr(out), int(max_n), _ptr(ro))
+        return int(used), out[: min(int(used), int(max_n))].copy(), bool(ro[0])
+
+    def trace_off(self):
+        lib().orc_box_set_tape(self._h, 0, None, 0, 0)
 
     def mc_coag(self, kernel_type, del_t, step):
         ns = np.zeros(1, dtype=np.int64)

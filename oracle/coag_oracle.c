@@ -45,8 +45,13 @@ void orc_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2],
 /* A call-site stream.  key = (seed_lo, seed_hi + box); counter =
  * (ordinal, site, step, kind << 24 | block).  Each Philox block yields two
  * uniforms, each built from 52 bits as (k + 1/2) 2^-52, so 0 < U < 1. */
+/* the tape of the box whose mc_coag is running, attached to every stream
+ * that step opens */
+static orc_tape *g_tape = NULL;
+
 void orc_stream_open(orc_stream *s, uint64_t seed, uint32_t box, uint32_t step,
                      uint32_t site, uint32_t ordinal, uint32_t kind) {
+    s->tape = g_tape;
     s->key[0] = (uint32_t)seed;
     s->key[1] = (uint32_t)(seed >> 32) + box;
     s->ctr[0] = ordinal;
@@ -62,7 +67,26 @@ static double u52(uint32_t a, uint32_t b) {
     return (double)k * 0x1p-52 + 0x1p-53;
 }
 
+static double philox_uniform(orc_stream *s);
+
 double orc_stream_uniform(orc_stream *s) {
+    orc_tape *t = s->tape;
+    if (t) {
+        if (t->mode == 2) {
+            if (t->pos < t->n) return t->u[t->pos++];
+            t->err = 1;  /* ran out: go on with the stream, fail after the step */
+            return philox_uniform(s);
+        }
+        double v = philox_uniform(s);
+        if (t->pos < t->n) t->u[t->pos] = v;
+        else t->err = 1;
+        t->pos++;
+        return v;
+    }
+    return philox_uniform(s);
+}
+
+static double philox_uniform(orc_stream *s) {
     if (s->have == 0) {
         uint32_t c[4] = {s->ctr[0], s->ctr[1], s->ctr[2],
                          s->ctr[3] | (s->block & 0xFFFFFFu)};
@@ -108,6 +132,11 @@ double orc_rand_normal(orc_stream *s, double mean, double stddev) {
  * transformed rejection method for generating Poisson random variables",
  * Insurance: Mathematics and Economics 12, 1993). */
 int orc_rand_poisson(orc_stream *s, double mean) {
+    if (mean > 10.0 && s->tape && s->tape->ref_samplers) {
+        /* the reference without GSL (:298-302): nint of a normal variate */
+        int k = (int)lround(orc_rand_normal(s, mean, sqrt(mean)));
+        return k > 0 ? k : 0;
+    }
     if (mean <= 10.0) {
         double L = exp(-mean);
         int k = 0;
@@ -145,6 +174,11 @@ int orc_rand_poisson(orc_stream *s, double mean) {
 int orc_rand_binomial(orc_stream *s, int n, double p) {
     double np = (double)n * p;
     double nomp = (double)n * (1.0 - p);
+    if (np >= 10.0 && nomp >= 10.0 && s->tape && s->tape->ref_samplers) {
+        /* the reference without GSL (:359-361) */
+        int k = (int)lround(orc_rand_normal(s, np, sqrt(np * (1.0 - p))));
+        return k < 0 ? 0 : (k > n ? n : k);
+    }
     if (np >= 10.0 && nomp >= 10.0) {
         int flip = p > 0.5;
         double pp = flip ? 1.0 - p : p;
@@ -885,6 +919,7 @@ struct orc_box {
     int64_t n_warn_kmax;
     int64_t n_accel;       /* bin pairs that took the per-particle path */
     int regridded;
+    orc_tape tape;         /* mode 0: the call-site streams */
 };
 
 orc_box *orc_box_create(int n_spec, const double *density, int n_group,
@@ -929,6 +964,7 @@ void orc_box_destroy(orc_box *b) {
     free(b->comp_time); free(b->least_ct); free(b->greatest_ct);
     free(b->fwd_bin); free(b->fwd_class); free(b->fwd_index);
     free(b->info_id); free(b->info_other); free(b->info_action);
+    free(b->tape.u);
     free(b);
 }
 
@@ -1769,8 +1805,43 @@ static int try_per_particle_coag(orc_box *b, int type, double k_max,
 }
 
 /* mc_coag, src/coagulation.F90:43-96 */
+static int mc_coag_body(orc_box *b, int type, double del_t, uint32_t step,
+                        int64_t *tot_n_samp, int64_t *tot_n_coag);
+
 int orc_mc_coag(orc_box *b, int type, double del_t, uint32_t step,
                 int64_t *tot_n_samp, int64_t *tot_n_coag) {
+    g_tape = b->tape.mode ? &b->tape : NULL;
+    int err = mc_coag_body(b, type, del_t, step, tot_n_samp, tot_n_coag);
+    g_tape = NULL;
+    if (!err && b->tape.mode && b->tape.err) err = 3;  /* the tape ran out */
+    return err;
+}
+
+int orc_box_set_tape(orc_box *b, int mode, const double *u, int64_t n, int ref_samplers) {
+    free(b->tape.u);
+    memset(&b->tape, 0, sizeof(b->tape));
+    if (mode == 0) return 0;
+    if (n < 1 || (mode != 1 && mode != 2)) return 1;
+    b->tape.u = (double *)malloc(sizeof(double) * (size_t)n);
+    if (mode == 2) memcpy(b->tape.u, u, sizeof(double) * (size_t)n);
+    b->tape.n = n;
+    b->tape.mode = mode;
+    b->tape.ref_samplers = ref_samplers ? 1 : 0;
+    return 0;
+}
+
+int64_t orc_box_tape_state(const orc_box *b, double *u, int64_t max_n, int *ran_out) {
+    if (ran_out) *ran_out = b->tape.err;
+    if (b->tape.mode == 1 && u) {
+        int64_t m = b->tape.pos < b->tape.n ? b->tape.pos : b->tape.n;
+        if (m > max_n) m = max_n;
+        if (m > 0) memcpy(u, b->tape.u, sizeof(double) * (size_t)m);
+    }
+    return b->tape.pos;
+}
+
+static int mc_coag_body(orc_box *b, int type, double del_t, uint32_t step,
+                        int64_t *tot_n_samp, int64_t *tot_n_coag) {
     *tot_n_samp = 0;
     *tot_n_coag = 0;
     int err = remake_if_needed(b);

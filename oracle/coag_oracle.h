@@ -71,12 +71,26 @@ extern "C" {
 #define ORC_ERR_UNSUPPORTED 4  /* a case the oracle does not restate */
 
 /* ---- Philox4x32-10 and the per-site uniform stream ---- */
+/* A box's uniforms in call order (the values pmc_random returns,
+ * src/rand.F90:138-166): replayed instead of the call-site streams, or
+ * recorded from them.  ref_samplers: Poisson counts above mean 10 and
+ * binomials with np, n(1-p) >= 10 by the reference's rounded normal. */
+typedef struct {
+    double *u;
+    int64_t n;             /* replay: values held; record: capacity */
+    int64_t pos;
+    int mode;              /* 1 record, 2 replay */
+    int ref_samplers;
+    int err;               /* ran out */
+} orc_tape;
+
 typedef struct {
     uint32_t key[2];
     uint32_t ctr[4];       /* ctr[3] low 24 bits = block number */
     uint32_t buf[4];
     int have;              /* uniforms left in buf (0..2) */
     uint32_t block;
+    orc_tape *tape;        /* the stepping box's tape, or NULL */
 } orc_stream;
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
@@ -215,6 +229,12 @@ int orc_loss_rate(int type, double vol, double density, double temp,
 int orc_box_dilute(orc_box *b, double sample_prob, uint32_t step);
 int orc_box_particle_loss(orc_box *b, int loss_type, double del_t,
                           double height, uint32_t step);
+
+/* record (mode 1, capacity n) or replay (mode 2, values u[0..n)) the box's
+ * uniforms in the next mc_coag calls; mode 0 stops.  orc_box_tape_state
+ * gives the values used so far and whether the tape ran out. */
+int orc_box_set_tape(orc_box *b, int mode, const double *u, int64_t n, int ref_samplers);
+int64_t orc_box_tape_state(const orc_box *b, double *u, int64_t max_n, int *ran_out);
 
 /* single draws of rand_group and of aero_particle_coagulate's components
  * on stream (seed, ordinal), for their statistics tests */

@@ -41,6 +41,12 @@ ABI = {
     "pmc_gpu_reserve": (_i, [_p, _p]),
     "pmc_gpu_get_part_steps": (_i, [_p, _p]),
     "pmc_gpu_get_path_stats": (_i, [_p, _p, _p, _p, _p]),
+    "pmc_gpu_get_arrival_stats": (_i, [_p, _p, _p]),
+    "pmc_gpu_get_accel_counts": (_i, [_p, _p, _p]),
+    "pmc_gpu_record": (_i, [_p, _i, _i64, _i]),
+    "pmc_gpu_replay": (_i, [_p, _i, _p, _i64, _i]),
+    "pmc_gpu_get_trace": (_i, [_p, _i, _p, _i64, _p, _p]),
+    "pmc_gpu_trace_off": (_i, [_p, _i]),
     "pmc_gpu_size_dist": (_i, [_p, _i, _i, _i, _d, _d, _p]),
     "pmc_gpu_size_dist_noalloc": (_i, [_p, _i, _i, _i, _d, _d, _p]),
     "pmc_gpu_set_n_part_ideal": (_i, [_p, _i, _d]),
@@ -242,6 +248,18 @@ class CoagGPU:
                     n_comp=nco[:n], comp_source=cs[:n], comp_time=ct[:n], least=le[:n],
                     greatest=gr[:n])
 
+    def download_ids_vols(self, box):
+        """The box's particle ids and species volumes only (n, n_spec)."""
+        n = self.n_part(box)
+        m = max(n, 1)
+        vol = np.zeros((m, self.n_spec))
+        ids = np.zeros(m, dtype=np.int64)
+        cnt = np.zeros(1, dtype=np.int32)
+        _check(load().pmc_gpu_download(self._h, box, m, _ptr(cnt), _ptr(vol), _ptr(ids), None,
+                                       None, None, None, None, None, None, None))
+        n = int(cnt[0])
+        return ids[:n], vol[:n]
+
     # -- the step --------------------------------------------------------
     def mc_coag(self, coag_kernel_type: int, del_t: float):
         """One coagulation timestep for all boxes; returns per-box
@@ -343,6 +361,42 @@ class CoagGPU:
         q = {"num": 0, "mass": 1, "vol": 2}[quantity]
         _check(load().pmc_gpu_size_dist_noalloc(self._h, box, q, n_bin, d_min, d_max,
                                                 ctypes.c_void_p(int(d_hist))))
+
+    def record(self, box, capacity, ref_samplers=False):
+        """Exact mode: write every uniform the box draws, in call order."""
+        _check(load().pmc_gpu_record(self._h, box, int(capacity), int(bool(ref_samplers))))
+
+    def replay(self, box, uniforms, ref_samplers=False):
+        """Exact mode: draw the box's uniforms from `uniforms`, in order."""
+        u = np.ascontiguousarray(uniforms, dtype=np.float64)
+        _check(load().pmc_gpu_replay(self._h, box, _ptr(u), len(u), int(bool(ref_samplers))))
+
+    def trace(self, box, max_n=0):
+        """(uniforms used, recorded values up to max_n, ran out)."""
+        n = np.zeros(1, dtype=np.int64)
+        ro = np.zeros(1, dtype=np.int32)
+        out = np.zeros(max(int(max_n), 1))
+        _check(load().pmc_gpu_get_trace(self._h, box, _ptr(out), int(max_n), _ptr(n), _ptr(ro)))
+        used = int(n[0])
+        return used, out[: min(used, int(max_n))].copy(), bool(ro[0])
+
+    def trace_off(self, box):
+        _check(load().pmc_gpu_trace_off(self._h, box))
+
+    def arrival_stats(self):
+        """Per box: particles that joined a list within a step, and those of
+        them the list's reserved arrival entries could not hold."""
+        arr = np.zeros(self.n_box, dtype=np.int64)
+        over = np.zeros(self.n_box, dtype=np.int64)
+        _check(load().pmc_gpu_get_arrival_stats(self._h, _ptr(arr), _ptr(over)))
+        return {"n_arrivals": arr, "n_overflow": over}
+
+    def accel_counts(self):
+        """Per box: samples and coagulations of the per-particle path."""
+        ns = np.zeros(self.n_box, dtype=np.int64)
+        nc = np.zeros(self.n_box, dtype=np.int64)
+        _check(load().pmc_gpu_get_accel_counts(self._h, _ptr(ns), _ptr(nc)))
+        return {"n_samp": ns, "n_coag": nc}
 
     def path_stats(self):
         """Per box: pairs on the per-particle path and dropped draws; for

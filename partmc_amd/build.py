@@ -64,6 +64,25 @@ def build_fortran(verbose: bool = False):
     return DRIVER
 
 
+PROBE = os.path.join(HERE, "libpmc_probe.so")
+
+
+def build_probe(verbose: bool = False) -> str:
+    """The throughput probes (csrc/pmc_probe.hip) that bench.py --probe runs."""
+    tmp = PROBE + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-Wno-unused-command-line-argument", "-o", tmp, os.path.join(CSRC, "pmc_probe.hip")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed building libpmc_probe.so")
+    os.replace(tmp, PROBE)
+    return PROBE
+
+
 def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     cmd = command(tmp)
@@ -74,6 +93,7 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libpartmc_coag.so")
     os.replace(tmp, LIB)
+    build_probe(verbose)
     build_fortran(verbose)
     return LIB
 

@@ -109,6 +109,7 @@ struct pmc_gpu {
   };
   std::vector<Timed> pending;
   std::vector<std::pair<std::string, std::pair<double, int64_t>>> totals;
+  std::vector<double *> tape_buf;  // per box: device values of its tape, or null
 };
 
 static void timing_begin(pmc_gpu *h, const char *name, hipEvent_t *start) {
@@ -443,6 +444,8 @@ int pmc_gpu_destroy(pmc_gpu *h) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   parallel_free(h->work);
+  for (double *t : h->tape_buf)
+    if (t) (void)hipFree(t);
   (void)hipStreamDestroy(h->stream);
   delete h;
   return 0;
@@ -641,6 +644,11 @@ int pmc_gpu_upload(pmc_gpu *h, int box, int n_part, int capacity, const double *
 
 static int check_box_errors(pmc_gpu *h, std::vector<BoxDesc> &hb) {
   for (int b = 0; b < h->n_box; ++b)
+    if (hb[b].tape.mode != TAPE_OFF && hb[b].tape.err)
+      return fail(ERR_CAPACITY, "box %d: the %s tape ran out after %lld uniforms", b,
+                  hb[b].tape.mode == TAPE_REPLAY ? "replayed" : "recording",
+                  (long long)hb[b].tape.n);
+  for (int b = 0; b < h->n_box; ++b)
     if (hb[b].error) {
       int code = hb[b].error;
       const char *what = "";
@@ -787,6 +795,10 @@ int pmc_gpu_mc_coag_async(pmc_gpu *h, int kernel_type, double del_t) {
   for (int b = 0; b < h->n_box; ++b)
     if (h->cap[b] == 0) return fail(ERR_STATE, "box %d has no particle storage: upload first", b);
   HIPCHECK(hipSetDevice(h->device));
+  if (h->mode != PMC_MODE_EXACT)
+    for (int b = 0; b < (int)h->tape_buf.size(); ++b)
+      if (h->tape_buf[b])
+        return fail(ERR_STATE, "box %d records or replays a tape: that is the exact mode's", b);
   if (h->mode == PMC_MODE_EXACT) {
     LAUNCH_TIMED(h, "k_prepare", k_prepare, dim3(h->n_box), dim3(SORT_THREADS), 0, h->stream,
                  h->d_box, h->A, -1);
@@ -856,6 +868,8 @@ int pmc_gpu_reset_counters(pmc_gpu *h) {
     B.max_p_err = 0.0;
     B.n_rounds = B.n_dropped = 0;
     B.n_tail_tests = 0;
+    B.n_arrivals = B.n_arr_over = 0;
+    B.n_acc_samp = B.n_acc_coag = 0;
     B.n_accel_pairs = 0;
     B.part_steps = 0;
   }
@@ -900,6 +914,92 @@ int pmc_gpu_get_path_stats(pmc_gpu *h, int64_t *n_accel_pairs, int64_t *n_droppe
   if (h->work.d.acc_samp)
     HIPCHECK(hipMemcpy(&acc_samp, h->work.d.acc_samp, sizeof(acc_samp), hipMemcpyDeviceToHost));
   rounds[3] = (int64_t)acc_samp;
+  return 0;
+}
+
+// ------------------------------------------------------------ tapes
+static int set_tape(pmc_gpu *h, int box, int mode, const double *u, int64_t n, int ref) {
+  int e = check_box(h, box);
+  if (e) return e;
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  if ((int)h->tape_buf.size() != h->n_box) h->tape_buf.assign(h->n_box, nullptr);
+  if (h->tape_buf[box]) {
+    HIPCHECK(hipFree(h->tape_buf[box]));
+    h->tape_buf[box] = nullptr;
+  }
+  Tape t{};
+  t.mode = mode;
+  t.ref_samplers = ref ? 1 : 0;
+  if (mode != TAPE_OFF) {
+    HIPCHECK(dmalloc(&h->tape_buf[box], (size_t)std::max<int64_t>(n, 1)));
+    if (mode == TAPE_REPLAY && n > 0)
+      HIPCHECK(hipMemcpy(h->tape_buf[box], u, sizeof(double) * n, hipMemcpyHostToDevice));
+    t.u = h->tape_buf[box];
+    t.n = n;
+  }
+  HIPCHECK(hipMemcpy((char *)(h->d_box + box) + offsetof(BoxDesc, tape), &t, sizeof(Tape),
+                     hipMemcpyHostToDevice));
+  return 0;
+}
+
+int pmc_gpu_record(pmc_gpu *h, int box, int64_t capacity, int ref_samplers) {
+  if (capacity < 1) return fail(ERR_ARG, "record: capacity must be >= 1");
+  return set_tape(h, box, TAPE_RECORD, nullptr, capacity, ref_samplers);
+}
+
+int pmc_gpu_replay(pmc_gpu *h, int box, const double *u, int64_t n, int ref_samplers) {
+  if (!u || n < 1) return fail(ERR_ARG, "replay: need n >= 1 uniforms");
+  for (int64_t i = 0; i < n; ++i)
+    if (!(u[i] >= 0.0 && u[i] < 1.0))
+      return fail(ERR_ARG, "replay: value %lld (%g) is not in [0, 1)", (long long)i, u[i]);
+  return set_tape(h, box, TAPE_REPLAY, u, n, ref_samplers);
+}
+
+int pmc_gpu_trace_off(pmc_gpu *h, int box) { return set_tape(h, box, TAPE_OFF, nullptr, 0, 0); }
+
+int pmc_gpu_get_trace(pmc_gpu *h, int box, double *u, int64_t max_n, int64_t *n_used,
+                      int *ran_out) {
+  int e = check_box(h, box);
+  if (e) return e;
+  if (!n_used) return fail(ERR_ARG, "get_trace: n_used is required");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  Tape t{};
+  HIPCHECK(hipMemcpy(&t, (char *)(h->d_box + box) + offsetof(BoxDesc, tape), sizeof(Tape),
+                     hipMemcpyDeviceToHost));
+  *n_used = t.pos;
+  if (ran_out) *ran_out = t.err;
+  if (t.mode == TAPE_RECORD && u && max_n > 0) {
+    const int64_t m = std::min(std::min(t.pos, t.n), max_n);
+    if (m > 0) HIPCHECK(hipMemcpy(u, t.u, sizeof(double) * m, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int pmc_gpu_get_arrival_stats(pmc_gpu *h, int64_t *n_arrivals, int64_t *n_overflow) {
+  if (!h || !n_arrivals || !n_overflow) return fail(ERR_ARG, "get_arrival_stats: bad arguments");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  std::vector<BoxDesc> hb(h->n_box);
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
+  for (int b = 0; b < h->n_box; ++b) {
+    n_arrivals[b] = hb[b].n_arrivals;
+    n_overflow[b] = hb[b].n_arr_over;
+  }
+  return 0;
+}
+
+int pmc_gpu_get_accel_counts(pmc_gpu *h, int64_t *n_samp, int64_t *n_coag) {
+  if (!h || !n_samp || !n_coag) return fail(ERR_ARG, "get_accel_counts: bad arguments");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  std::vector<BoxDesc> hb(h->n_box);
+  HIPCHECK(hipMemcpy(hb.data(), h->d_box, sizeof(BoxDesc) * h->n_box, hipMemcpyDeviceToHost));
+  for (int b = 0; b < h->n_box; ++b) {
+    n_samp[b] = hb[b].n_acc_samp;
+    n_coag[b] = hb[b].n_acc_coag;
+  }
   return 0;
 }
 

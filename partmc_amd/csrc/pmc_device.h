@@ -77,13 +77,31 @@ __host__ __device__ inline double u52(uint32_t a, uint32_t b) {
   return (double)k * 0x1p-52 + 0x1p-53;
 }
 
+// A box's uniforms in the order the sequential chain draws them, the
+// values pmc_random returns (src/rand.F90:138-166) in call order: replayed
+// from a recorded run in place of the call-site streams, or recorded from
+// this run.  ref_samplers: Poisson counts above mean 10 and binomials with
+// np and n(1-p) >= 10 use the reference's rounded normal (src/rand.F90:
+// 298-302, 359-361), as a run without GSL draws them.
+constexpr int TAPE_OFF = 0, TAPE_RECORD = 1, TAPE_REPLAY = 2;
+struct Tape {
+  double *u;         // the uniforms
+  int64_t n;         // replay: values held; record: capacity
+  int64_t pos;       // next value read or written
+  int mode;
+  int ref_samplers;
+  int err;           // the tape ran out
+};
+
 // A call-site stream: key = (seed_lo, seed_hi + box), counter =
-// (ordinal, site, step, kind << 24 | block).  Two uniforms per block.
+// (ordinal, site, step, kind << 24 | block).  Two uniforms per block.  With
+// a tape attached its uniforms come from, or go to, the tape.
 struct Stream {
   uint32_t k0, k1, c0, c1, c2, c3;
   uint32_t block;
   uint32_t buf2, buf3;
   int have;
+  Tape *tape;
 
   __host__ __device__ inline void open(uint64_t seed, uint32_t box,
                                        uint32_t step, uint32_t site,
@@ -96,8 +114,24 @@ struct Stream {
     c3 = kind << 24;
     block = 0;
     have = 0;
+    tape = nullptr;
   }
   __host__ __device__ inline double uniform() {
+    if (tape) {
+      if (tape->mode == TAPE_REPLAY) {
+        if (tape->pos < tape->n) return tape->u[tape->pos++];
+        tape->err = 1;  // ran out: the step goes on with its own stream and fails after
+        return philox_uniform();
+      }
+      const double v = philox_uniform();
+      if (tape->pos < tape->n) tape->u[tape->pos] = v;
+      else tape->err = 1;
+      tape->pos++;
+      return v;
+    }
+    return philox_uniform();
+  }
+  __host__ __device__ inline double philox_uniform() {
     if (have == 0) {
       uint32_t o[4];
       philox4x32_10(c0, c1, c2, c3 | (block & 0xFFFFFFu), k0, k1, o);
@@ -119,6 +153,16 @@ __device__ inline int rand_int(Stream &s, int n) {
   return (k % n) + 1;
 }
 
+// rand_normal without GSL (Box-Muller, the first deviate), src/rand.F90:
+// 452-470 with no probability threshold
+__device__ inline double rand_normal_ref(Stream &s, double mean, double stddev) {
+  const double u1 = s.uniform();
+  const double u2 = s.uniform();
+  const double r = sqrt(-2.0 * log(u1));
+  const double theta = 2.0 * C_PI * u2;
+  return stddev * (r * cos(theta)) + mean;
+}
+
 // prob_round, src/rand.F90:214-227
 __device__ inline int prob_round(Stream &s, double val) {
   if (s.uniform() < ceil(val) - val) return (int)floor(val);
@@ -129,6 +173,10 @@ __device__ inline int prob_round(Stream &s, double val) {
 // PTRS transformed rejection above (an exact sampler in place of the
 // reference's rounded normal, :298-302).
 __device__ inline int rand_poisson(Stream &s, double mean) {
+  if (mean > 10.0 && s.tape && s.tape->ref_samplers) {
+    const int k = (int)round(rand_normal_ref(s, mean, sqrt(mean)));
+    return k > 0 ? k : 0;
+  }
   if (mean <= 10.0) {
     double L = exp(-mean);
     int k = 0;
@@ -164,6 +212,10 @@ __device__ inline int rand_poisson(Stream &s, double mean) {
 __device__ inline int rand_binomial(Stream &s, int n, double p) {
   double np = (double)n * p;
   double nomp = (double)n * (1.0 - p);
+  if (np >= 10.0 && nomp >= 10.0 && s.tape && s.tape->ref_samplers) {
+    const int k = (int)round(rand_normal_ref(s, np, sqrt(np * (1.0 - p))));
+    return k < 0 ? 0 : (k > n ? n : k);
+  }
   if (np >= 10.0 && nomp >= 10.0) {
     bool flip = p > 0.5;
     double pp = flip ? 1.0 - p : p;

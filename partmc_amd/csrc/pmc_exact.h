@@ -508,6 +508,7 @@ struct Rec {
 struct ExactBox {
   const Arrays *A;
   BoxDesc *B;
+  Tape *tape = nullptr;  // the box's tape when one records or replays, else null
   int box;
   int64_t base;
   int nb, nc;
@@ -825,6 +826,7 @@ __device__ inline int exact_sample_source(ExactBox &E, const Weights &w,
   aps.open(E.seed, E.box, E.step, site, ordinal, RNG_ACC_POISSON);
   smp.open(E.seed, E.box, E.step, site, ordinal, RNG_ACC_SAMPLE);
   fin.open(E.seed, E.box, E.step, site, ordinal, RNG_ACC_FINISH);
+  aps.tape = smp.tape = fin.tape = E.tape;
   double nc_target = nc_at_radius(w, target.klass, E.rec_radius(target));
   int64_t target_id = target.id;
   int ct = target.klass;
@@ -1018,6 +1020,7 @@ k_exact_sweep(BoxDesc *boxes, Arrays A, int type, double del_t, uint64_t seed,
   ExactBox E;
   E.A = &A;
   E.B = B;
+  E.tape = B->tape.mode != TAPE_OFF ? &B->tape : nullptr;
   E.box = box;
   E.base = B->off;
   E.nb = B->n_bin;
@@ -1088,10 +1091,13 @@ k_exact_sweep(BoxDesc *boxes, Arrays A, int type, double del_t, uint64_t seed,
                       if (!err && nco > 0) {
                         Stream ts;
                         ts.open(seed, box, step, site, (uint32_t)e, RNG_ACC_TARGET);
+                        ts.tape = E.tape;
                         err = exact_target_with_source(E, w, ts, bt, ct, e, source, cc);
                       }
                       tot_samp += ns_;
                       tot_coag += nco;
+                      B->n_acc_samp += ns_;
+                      B->n_acc_coag += nco;
                     }
                   }
                   err = __shfl(err, 0);
@@ -1112,6 +1118,7 @@ k_exact_sweep(BoxDesc *boxes, Arrays A, int type, double del_t, uint64_t seed,
           if (lane == 0) {
             Stream ps;
             ps.open(seed, box, step, site, 0, RNG_POISSON);
+            ps.tape = E.tape;
             n_samp = rand_poisson(ps, mean);
           }
           n_samp = __shfl(n_samp, 0);
@@ -1122,15 +1129,17 @@ k_exact_sweep(BoxDesc *boxes, Arrays A, int type, double del_t, uint64_t seed,
             n1 = E.n_entry(b1, c1);
             n2 = E.n_entry(b2, c2);
             if ((n1 < 2 && same) || n1 < 1 || n2 < 1) break;
-            // speculative batch: lane j evaluates test i + j
+            // speculative batch: lane j evaluates test i + j (one test at a
+            // time on a tape, whose draws follow the call order)
             const int ti_s = i + lane;
-            bool valid = ti_s <= n_samp;
+            bool valid = ti_s <= n_samp && (!E.tape || lane == 0);
             bool accept = false, warn = false;
             int dom1 = 0, dom2 = 0;
             int lerr = 0;
             if (valid) {
               Stream ts;
               ts.open(seed, box, step, site, (uint32_t)(ti_s - 1), RNG_TEST);
+              ts.tape = E.tape;
               int i1 = rand_int(ts, n1);
               int i2;
               if (same) {
@@ -1155,7 +1164,7 @@ k_exact_sweep(BoxDesc *boxes, Arrays A, int type, double del_t, uint64_t seed,
             unsigned long long wm = __ballot(valid && warn && lane <= last);
             n_warn += __popcll(wm);
             if (first == 64) {
-              i += 64;
+              i += E.tape ? 1 : 64;
               continue;
             }
             int fe = __shfl(lerr, first);
@@ -1167,6 +1176,7 @@ k_exact_sweep(BoxDesc *boxes, Arrays A, int type, double del_t, uint64_t seed,
             if (lane == 0) {
               Stream ms;
               ms.open(seed, box, step, site, (uint32_t)(i + first - 1), RNG_MERGE);
+              ms.tape = E.tape;
               err = exact_coagulate(E, w, ms, a1, a2, c1, c2, cc);
             }
             err = __shfl(err, 0);

@@ -87,11 +87,13 @@ struct PW {  // device view of the parallel work arrays
   int *r_nsamp, *r_ncoag, *r_nrm, *r_nchain, *r_head, *r_ndup, *r_ndead, *r_dhead, *r_flags,
       *r_err;
   int *r_rdhead, *r_nrd;       // distinct sources read from small source lists
+  int *r_q1, *r_q2;            // pairs (in the box) of the target's first and last merge
   // particle pool [pool_cap]: entry r < acc_cap is the final target of
   // record r; entries from acc_cap on are the copies (aero_state_dup_particle)
   double *q_vol;               // [n_spec][pool_cap]
   int *q_group, *q_nprim, *q_ncomp, *q_next;
   int *q_pos, *q_bin, *q_dead;  // a copy's walk start (pair, bin) and whether it died
+  int *q_made;                 // the pair (in the box) whose merge made the copy
   int *q_csrc;                 // [MAX_COMP][pool_cap]
   double *q_ctime;             // [MAX_COMP][pool_cap]
   double *q_least, *q_greatest;
@@ -105,6 +107,24 @@ struct PW {  // device view of the parallel work arrays
   uint32_t *tp_in, *tpair;     // [T_cap] pair of every test of a single-batch step
   int tpair_ok;                // tpair is current for this step
   unsigned long long *tail_rounds;  // [1] claim rounds run by the tail, summed over boxes
+  // Particles that join a list within the step ("arrivals": a merged
+  // particle that lands in another (bin, class) than its slot's list, an
+  // appended particle).  The draws of every pair reserve acap entries after
+  // each list's n step-start members; the j-th reserved entry is the list's
+  // j-th arrival before the pair begins (in sequence order).
+  int *acap;          // [n_box * K_stride] reserved arrival entries of the list this step
+  int *arr_n;         // [n_box * K_stride] arrivals recorded this step
+  int *arr_head;      // [n_box * K_stride] the list's arrivals: first pool entry, -1 none
+  int *ap_p;          // [ap_cap] arrival pool: the arrival's slot
+  uint32_t *ap_t;     // [ap_cap] sequence index at which it joined the list
+  uint32_t *ap_dep;   // [ap_cap] at which it left again (UNCLAIMED: still there)
+  int *ap_next;       // [ap_cap] next arrival of the same list
+  uint32_t *ap_top;   // [1] pool entries used this step
+  uint32_t ap_cap;
+  int *arr_of;        // [total_cap] pool entry of the arrival a slot holds
+  int *key0;          // [total_cap] a touched slot's list key at step start
+  uint32_t *depart_t; // [total_cap] when a touched slot's step-start particle left that list
+  uint8_t *thin;      // [T_cap] the test's last draw was thinned (per batch)
 };
 
 constexpr int ACC_CAP = 1 << 18;
@@ -115,6 +135,9 @@ constexpr int RM_CAP = 1 << 24;
 constexpr int ACC_WARN = 1 << 30;
 constexpr int READ_TRACK = 128;  // source lists up to this size have their reads tracked
 constexpr int RF_WARN = 1, RF_DIED = 2, RF_MERGED = 4;
+constexpr int A_MAX = 64;       // arrival entries a list can reserve
+constexpr int ARR_REACH = 3;    // an empty list reserves entries when a list at most this many bins below is occupied
+constexpr int TNS_ATHIN = 1 << 29;  // tns bit of a per-set test thinned on an empty arrival entry
 
 struct ParallelWork {
   int n_box = 0;
@@ -600,13 +623,18 @@ __global__ void k_par_pairs(BoxDesc *boxes, Arrays A, PW W, double del_t, uint64
         // target slots go to the list's first accelerated pair
         // (k_par_acc_first)
         if (n1 > 0 && n2 > 0) atomicAdd((unsigned long long *)&B->n_accel_pairs, 1ull);
-      } else if (n1 > 0 && n2 > 0) {
+      } else {
+        // the bound on the lists when the loop reaches the pair: step-start
+        // members plus reserved arrival entries (draws thin to the members)
+        const int *acp = W.acap + (int64_t)box * W.K_stride;
+        const double N1 = (double)(n1 + acp[par_key(b1, c1, nb)]);
+        const double N2 = (double)(n2 + acp[par_key(b2, c2, nb)]);
         bool same = (b1 == b2) && (c1 == c2);
-        double n_possible = same ? (double)n1 * ((double)n2 - 1.0) / 2.0 : (double)n1 * (double)n2;
+        double n_possible = same ? N1 * (N2 - 1.0) / 2.0 : N1 * N2;
         double mean = k_max * del_t * n_possible;
         Stream ps;
         ps.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), 0, RNG_POISSON);
-        ns = (uint32_t)rand_poisson(ps, mean);
+        ns = n_possible > 0.0 ? (uint32_t)rand_poisson(ps, mean) : 0u;
         if (ns) {
           atomicAdd((unsigned long long *)&B->n_samp, (unsigned long long)ns);
           atomicAdd((unsigned long long *)&B->step_samp, (unsigned long long)ns);
@@ -666,6 +694,160 @@ __device__ inline bool drawable(const PW &W, int p, int key) {
   return st_get(W, p) != ST_HOLE && W.key[p] == key;
 }
 
+// ---- list membership within a step
+// In the reference a bin pair draws its test count from the lists as they
+// are when the loop reaches it (src/coagulation.F90:615-626) and each test
+// draws among the members at that moment (:766-803).  Here the count comes
+// from an upper bound, each list's n step-start members plus acap reserved
+// arrival entries, and a draw thins itself: a member that left before the
+// pair began, or a reserved entry with no arrival before then, drops the
+// test.  Poisson thinning keeps the surviving count Poisson with the mean
+// of the members present when the pair begins.  Within the pair, a member
+// that left after the pair began is drawn again among the present ones.
+
+// the first change of slot p in this step: remember its step-start key
+__device__ inline void slot_touch(const PW &W, int64_t p) {
+  if (st_get(W, p) == ST_LIVE) {
+    W.key0[p] = W.key[p];
+    W.depart_t[p] = UNCLAIMED;
+  }
+}
+
+// An arrival joins its list's pool chain.  The chain's order is that of
+// the atomics; readers rank the entries by (sequence index, slot).  The
+// entry is complete before it is linked, and the chains a round reads were
+// linked in earlier rounds (kernels, or tail rounds behind a barrier).
+__device__ inline void record_arrival(const PW &W, BoxDesc *B, int box, int key, int64_t p,
+                                      uint32_t t) {
+  const int64_t L = (int64_t)box * W.K_stride + key;
+  const uint32_t e = atomicAdd(W.ap_top, 1u);
+  if (e >= W.ap_cap) {
+    atomicExch(&B->error, ERR_CAPACITY);
+    return;
+  }
+  W.ap_p[e] = (int)p;
+  W.ap_t[e] = t;
+  W.ap_dep[e] = UNCLAIMED;
+  W.arr_of[p] = (int)e;
+  atomicAdd(&W.arr_n[L], 1);
+  atomicAdd((unsigned long long *)&B->n_arrivals, 1ull);
+  int old = __hip_atomic_load(&W.arr_head[L], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (;;) {
+    W.ap_next[e] = old;
+    __threadfence();
+    const int prev = atomicCAS(&W.arr_head[L], old, (int)e);
+    if (prev == old) break;
+    old = prev;
+  }
+}
+
+// the particle in slot p leaves the list it is in, at sequence index t
+__device__ inline void slot_leave(const PW &W, int64_t p, uint32_t t) {
+  slot_touch(W, p);
+  const int cur = W.key[p];
+  if (cur < 0) return;
+  if (cur == W.key0[p] && W.depart_t[p] == UNCLAIMED) {
+    W.depart_t[p] = t;  // the slot's step-start member
+    return;
+  }
+  const int e = W.arr_of[p];  // an arrival of list cur
+  if (e >= 0 && W.ap_p[e] == (int)p && W.ap_dep[e] == UNCLAIMED) W.ap_dep[e] = t;
+}
+
+// slot p receives a particle of list new_key in place of the one it held
+// (a merge into a parent's slot, a target merged in place): the old one
+// leaves its list at t_leave, the new one joins at t_join
+__device__ inline void slot_write(const PW &W, BoxDesc *B, int box, int64_t p, uint32_t t_leave,
+                                  uint32_t t_join, int new_key) {
+  slot_touch(W, p);
+  if (W.key[p] == new_key) return;  // changed in place, still in its list
+  slot_leave(W, p, t_leave);
+  if (new_key != W.key0[p]) record_arrival(W, B, box, new_key, p, t_join);
+}
+
+// an appended slot receives a particle of list new_key at sequence index t
+__device__ inline void slot_append(const PW &W, BoxDesc *B, int box, int64_t p, uint32_t t,
+                                   int new_key) {
+  W.key0[p] = -1;
+  W.depart_t[p] = 0;
+  record_arrival(W, B, box, new_key, p, t);
+}
+
+constexpr int DRAW_THIN = -2, DRAW_ATHIN = -3, DRAW_AGAIN = -4;
+
+// The list's first acap arrivals of the step, in order of (sequence
+// index, slot), take its reserved entries: entry j of a pair that begins
+// at tau is the j-th arrival if that came before tau.  Its pool entry, or -1.
+__device__ inline int arrival_at(const PW &W, int64_t L, int j, uint32_t tau) {
+  for (int e = W.arr_head[L]; e >= 0; e = W.ap_next[e]) {
+    const uint32_t te = W.ap_t[e];
+    const int pe = W.ap_p[e];
+    int rank = 0;
+    for (int f = W.arr_head[L]; f >= 0 && rank < j; f = W.ap_next[f]) {
+      const uint32_t tf = W.ap_t[f];
+      if (tf < te || (tf == te && W.ap_p[f] < pe)) ++rank;
+    }
+    if (rank == j - 1) return te < tau ? e : -1;
+  }
+  return -1;
+}
+
+// entry c (1-based) of list `key` (list index L, n step-start members
+// starting at s) for a pair that begins at tau: the particle, or
+// DRAW_THIN (a member that left before tau), DRAW_ATHIN (no arrival there
+// before tau), DRAW_AGAIN (a member that left after tau: draw again)
+__device__ inline int draw_entry(const PW &W, int64_t base, int64_t L, int key, int n, int s,
+                                 int c, uint32_t tau) {
+  if (c <= n) {
+    const int p = W.perm[base + s + c - 1];
+    if (drawable(W, p, key)) return p;
+    return W.depart_t[p] < tau ? DRAW_THIN : DRAW_AGAIN;
+  }
+  const int e = arrival_at(W, L, c - n, tau);
+  if (e < 0 || W.ap_dep[e] < tau) return DRAW_ATHIN;
+  const int p = W.ap_p[e];
+  return drawable(W, p, key) ? p : DRAW_AGAIN;
+}
+
+// reserved arrival entries and a fresh arrival log for every list: lists
+// at most ARR_REACH bins above an occupied one (particles only grow)
+__global__ void k_par_acap(BoxDesc *boxes, PW W) {
+  const int box = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= W.K_stride) return;
+  const BoxDesc *B = &boxes[box];
+  const int64_t L = (int64_t)box * W.K_stride + k;
+  const int nb = B->n_bin, nc = B->w.n_class;
+  int cap = 0;
+  if (!B->error && nb > 0 && k < nb * nc) {
+    const int b = k % nb + 1;
+    const int *cnt = W.lcnt + (int64_t)box * W.K_stride;
+    bool near = false;
+    for (int c = 1; c <= nc && !near; ++c)
+      for (int bb = max(1, b - ARR_REACH); bb <= b && !near; ++bb) near = cnt[par_key(bb, c, nb)] > 0;
+    if (near) cap = min(A_MAX, 2 + cnt[k] / 64);
+  }
+  W.acap[L] = cap;
+  W.arr_n[L] = 0;
+  W.arr_head[L] = -1;
+  if (box == 0 && k == 0) *W.ap_top = 0u;
+}
+
+// arrivals of the step beyond their list's reserved entries (not seen by
+// the list's later pairs)
+__global__ void k_par_arr_over(BoxDesc *boxes, PW W) {
+  const int box = blockIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  int over = 0;
+  if (k < W.K_stride) {
+    const int64_t L = (int64_t)box * W.K_stride + k;
+    over = max(0, W.arr_n[L] - W.acap[L]);
+  }
+  for (int off = 32; off > 0; off >>= 1) over += __shfl_xor(over, off);
+  if ((threadIdx.x & 63) == 0 && over)
+    atomicAdd((unsigned long long *)&boxes[box].n_arr_over, (unsigned long long)over);
+}
+
 // find_rand_pair + maybe_coag_pair (src/coagulation.F90:712-803) for test t
 // of pair k (global pair index): the two entries, drawn from the step's
 // lists skipping slots whose particle is gone or has left the list's
@@ -677,6 +859,7 @@ struct TestOut {
   int p1, p2;
   bool accept;
   bool warn;
+  int thin;  // 0: drawn; 1: thinned on a member that left before the pair; 2: on an empty arrival entry
 };
 
 __device__ inline void perset_test(BoxDesc *B, const Arrays &A, const PW &W, int box, int64_t k,
@@ -685,9 +868,11 @@ __device__ inline void perset_test(BoxDesc *B, const Arrays &A, const PW &W, int
   o.p1 = o.p2 = -1;
   o.accept = false;
   o.warn = false;
+  o.thin = 0;
   int c1, c2, b1, b2;
   decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
   const uint32_t j = t - W.pair_exc[k];
+  const uint32_t tau = W.pair_exc[k];  // where the pair begins in the sequence
   const int nb = B->n_bin, nc = B->w.n_class;
   const int64_t ti = table_index(A, box, nb, nc, b1, b2, c1, c2);
   const int cc = A.cc[ti];
@@ -697,33 +882,44 @@ __device__ inline void perset_test(BoxDesc *B, const Arrays &A, const PW &W, int
   const int key1 = par_key(b1, c1, nb), key2 = par_key(b2, c2, nb);
   const int n1 = W.lcnt[lb + key1], n2 = W.lcnt[lb + key2];
   const int s1 = W.lstart[lb + key1], s2 = W.lstart[lb + key2];
+  const int N1 = n1 + W.acap[lb + key1], N2 = n2 + W.acap[lb + key2];
   const int64_t base = B->off;
   Stream ts;
   ts.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), j, RNG_TEST);
+  // each side's first draw thins the test or stands; a member that left
+  // within the pair is drawn again among those present
   int p1 = -1, p2 = -1;
   int i1 = 0;
   for (int a = 0; a < MAX_REDRAW; ++a) {
-    int c = rand_int(ts, n1);
-    int p = W.perm[base + s1 + c - 1];
-    if (drawable(W, p, key1)) {
+    const int c = rand_int(ts, N1);
+    const int p = draw_entry(W, base, lb + key1, key1, n1, s1, c, tau);
+    if (p >= 0) {
       i1 = c;
       p1 = p;
       break;
     }
+    if (a == 0 && p != DRAW_AGAIN) {
+      o.thin = p == DRAW_THIN ? 1 : 2;
+      return;
+    }
   }
-  if (p1 >= 0 && (!same || n2 > 1)) {
+  if (p1 >= 0 && (!same || N2 > 1)) {
     for (int a = 0; a < MAX_REDRAW; ++a) {
       int c;
       if (same) {
-        c = rand_int(ts, n2 - 1);
-        if (c == i1) c = n2;
+        c = rand_int(ts, N2 - 1);
+        if (c == i1) c = N2;
       } else {
-        c = rand_int(ts, n2);
+        c = rand_int(ts, N2);
       }
-      int p = W.perm[base + s2 + c - 1];
-      if (drawable(W, p, key2) && p != p1) {
+      const int p = draw_entry(W, base, lb + key2, key2, n2, s2, c, tau);
+      if (p >= 0 && p != p1) {
         p2 = p;
         break;
+      }
+      if (a == 0 && p < 0 && p != DRAW_AGAIN) {
+        o.thin = p == DRAW_THIN ? 1 : 2;
+        return;
       }
     }
   }
@@ -894,7 +1090,10 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
   perset_test(B, A, W, box, k, t, type, seed, step, precision, band, validation, o);
   W.tp1[i] = o.p1;
   W.tp2[i] = o.p2;
-  W.tns[i] = o.warn ? ACC_WARN : 0;  // counted once the test is final
+  W.thin[t - t0] = o.thin ? 1 : 0;
+  // counted once the test is final; a test thinned on an arrival entry waits
+  // for the tail, where earlier events are settled first
+  W.tns[i] = (o.warn ? ACC_WARN : 0) | (o.thin == 2 ? TNS_ATHIN : 0);
   if (o.accept) {
     W.acc[i] = 1;
     Stream ms;
@@ -1187,7 +1386,7 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
   const int n_remove = rand_poisson(aps, prm * n_spt);
   const int n_extra = rand_poisson(aps, (1.0 - prm) * n_spt);
   const int n_total = n_remove + n_extra;
-  int n_avg = 0, n_samp = 0, n_rm_here = 0;
+  int n_avg = 0, n_samp = 0, n_rm_here = 0, n_thin = 0, n_samp_rm = 0;
   double prob_coag_tot = 0.0;
   double vol_sq[MAX_SPEC], vol_cv[MAX_SPEC];
   for (int q = 0; q < ns; ++q) {
@@ -1205,9 +1404,11 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
   Rec pi;
   TargetSide ts;  // formed at the first sample: many pairs draw none
   bool have_ts = false;
-  for (int i_samp = 1; i_samp <= n_total; ++i_samp) {
+  int i_samp = 1;
+  for (; i_samp <= n_total; ++i_samp) {
     if (n_rm_here >= n_s) break;  // every source removed
-    if (n_samp > n_remove && n_avg >= 2) {
+    // the reference's n_samp > n_samp_remove (:413) over the samples kept
+    if (i_samp > n_remove && n_samp > n_samp_rm && n_avg >= 2) {
       double cvmax = 0.0;
       for (int q = 0; q < ns; ++q) {
         double vm = source.vol[q] / (double)n_avg;
@@ -1217,8 +1418,14 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
       double m95 = cvmax / sqrt((double)n_avg) * student_t_95_coeff(n_avg);
       if (m95 < 0.1) break;
     }
-    // a uniform draw among the sources still present
+    // a uniform draw among the sources still present.  The counts above
+    // come from the step-start list; a source an earlier event took away
+    // before this target's turn was not in the reference's count
+    // (compute_n_source, src/coagulation.F90:288-328, reads the list as it
+    // is), so a first draw that lands on one thins the sample.  Sources
+    // this target removed are drawn again, as in the reference's loop.
     int ps = -1;
+    bool thinned = false;
     for (int a = 0; a < MAX_REDRAW; ++a) {
       int c = rand_int(smp, n_s);
       int q = W.perm[sbase + c - 1];
@@ -1226,6 +1433,14 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
         ps = q;
         break;
       }
+      if (a == 0 && !drawable(W, q, skey) && W.depart_t[q] < t) {
+        thinned = true;
+        break;
+      }
+    }
+    if (thinned) {
+      n_thin++;
+      continue;
     }
     if (ps < 0) {
       atomicAdd(&B->n_dropped, 1);
@@ -1235,6 +1450,7 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
     // target's draws: remember what was read
     if (n_s <= READ_TRACK && !rm_has(W, reads, ps) && !rm_push(W, reads, ps)) return ERR_CAPACITY;
     n_samp++;
+    if (i_samp <= n_remove) n_samp_rm++;
     load_rec_head(A, ps, pi);
     double kk;
     if (!have_ts) {
@@ -1264,9 +1480,14 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
   }
   *n_samp_out = n_samp;
   if (n_avg == 0) return 0;
+  // the samples the early exit left undrawn thin as the drawn ones did
+  for (int i = i_samp; i <= n_total; ++i) {
+    const int q = W.perm[sbase + rand_int(smp, n_s) - 1];
+    if (!drawable(W, q, skey) && W.depart_t[q] < t) n_thin++;
+  }
   double prob_coag_mean = prob_coag_tot / (double)n_samp;
   if (!(prob_coag_mean <= 1.0)) *warn = 1;
-  int n_coag = rand_binomial(fin, n_total, prob_coag_mean);
+  int n_coag = rand_binomial(fin, n_total - n_thin, prob_coag_mean);
   for (int q = 0; q < ns; ++q) source.vol[q] = source.vol[q] * ((double)n_coag / (double)n_avg);
   source.nprim = prob_round(fin, (double)source.nprim * ((double)n_coag / (double)n_avg));
   const int n_comp = source.ncomp;
@@ -1362,6 +1583,7 @@ __device__ __noinline__ int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &
   RmChain reads{-1, -1, RM_CHUNK, 0, 0ull};
   int n_samp_tot = 0, n_coag_tot = 0, warn = 0, flags = 0, err_code = 0, n_rm = 0;
   int n_dup = 0, n_dead = 0, dup_head = -1, dup_tail = -1;
+  int q_first = -1, q_last = -1;  // pairs of the target's first and last merge
   // The target walks first; each copy made on the way (in the reference it
   // joins the list of the target's new bin) then walks on from the pair
   // that made it, in creation order.  Removals are preceded in the chain
@@ -1425,7 +1647,11 @@ __device__ __noinline__ int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &
         np.group = rand_group_w(ft, w, cc, r_new);
         np.klass = cc;
         cur = np;
-        if (walker < 0) flags |= RF_MERGED;
+        if (walker < 0) {
+          flags |= RF_MERGED;
+          if (q_first < 0) q_first = (int)q;
+          q_last = (int)q;
+        }
         const int new_bin = bin_find_log(nb, B->log_lo, B->log_hi, r_new);
         if (new_bin < 1 || new_bin > nb) {
           err_code = 765620746;
@@ -1448,6 +1674,7 @@ __device__ __noinline__ int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &
           pool_store(W, ns, d, cp);
           W.q_next[d] = -1;
           W.q_pos[d] = (int)q_next;
+          W.q_made[d] = (int)q;
           W.q_bin[d] = new_bin;
           W.q_dead[d] = 0;
           if (dup_tail >= 0) W.q_next[dup_tail] = d;
@@ -1497,6 +1724,8 @@ __device__ __noinline__ int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &
   W.r_dhead[rec] = dup_head;
   W.r_flags[rec] = flags;
   W.r_err[rec] = err_code;
+  W.r_q1[rec] = q_first;
+  W.r_q2[rec] = q_last;
   if (claim) claim_write(W, p, t);
   return 1;
 }
@@ -1630,8 +1859,10 @@ __global__ void k_par_box_first(PW W, int E) {
   if (e == 0 || W.ev_box[e - 1] != W.ev_box[e]) W.box_first[W.ev_box[e]] = e;
 }
 
-__device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc *B, int64_t slot,
-                                          const Rec &r, int nb, int type, int want_terms) {
+__device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc *B, int box,
+                                          uint32_t t_leave, uint32_t t_join, int64_t slot,
+                                          bool appended, const Rec &r, int nb, int type,
+                                          int want_terms) {
   double v = 0.0;
   double m = 0.0;
   for (int s = 0; s < A.n_spec; ++s) {
@@ -1651,14 +1882,17 @@ __device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc 
   }
   A.least[slot] = r.least;
   A.greatest[slot] = r.greatest;
-  st_set(W, slot, ST_NEW);
   int b = bin_find_log(nb, B->log_lo, B->log_hi, vol2rad(v));
   if (b < 1 || b > nb) {
     atomicExch(&B->error, 134572570);
+    st_set(W, slot, ST_NEW);
     W.key[slot] = 0;
     return;
   }
   int k = par_key(b, r.klass, nb);
+  if (appended) slot_append(W, B, box, slot, t_join, k);
+  else slot_write(W, B, box, slot, t_leave, t_join, k);
+  st_set(W, slot, ST_NEW);
   W.key[slot] = k;
   if (want_terms) W.terms[slot] = particle_terms(type, v, m / v, B->env);
 }
@@ -1860,9 +2094,9 @@ __device__ inline void resample_components(const Arrays &A, Stream &s, int64_t p
 // aero_particle_coagulate (src/aero_particle.F90:917-1033).  Components
 // are concatenated, or resampled on `rs` (the merge stream after the merge
 // decisions) when the parents hold more than MAX_COMP together.
-__device__ inline int direct_merge(const Arrays &A, const PW &W, BoxDesc *B, int64_t p1,
-                                   int64_t p2, int64_t dst, int64_t id, int group, int klass,
-                                   int type, int want_terms, Stream *rs) {
+__device__ inline int direct_merge(const Arrays &A, const PW &W, BoxDesc *B, int box, uint32_t t,
+                                   int64_t p1, int64_t p2, int64_t dst, bool appended, int64_t id,
+                                   int group, int klass, int type, int want_terms, Stream *rs) {
   const int64_t tc = A.total_cap;
   const int n1 = A.ncomp[p1], n2 = A.ncomp[p2];
   const int np1 = A.nprim[p1], np2 = A.nprim[p2];
@@ -1914,14 +2148,18 @@ __device__ inline int direct_merge(const Arrays &A, const PW &W, BoxDesc *B, int
   A.ncomp[dst] = n_out;
   A.least[dst] = least;
   A.greatest[dst] = greatest;
-  st_set(W, dst, ST_NEW);
   const int nb = B->n_bin;
   const int b = bin_find_log(nb, B->log_lo, B->log_hi, vol2rad(v));
   if (b < 1 || b > nb) {
+    st_set(W, dst, ST_NEW);
     W.key[dst] = 0;
     return 134572570;
   }
-  W.key[dst] = par_key(b, klass, nb);
+  const int k_new = par_key(b, klass, nb);
+  if (appended) slot_append(W, B, box, dst, t, k_new);
+  else slot_write(W, B, box, dst, t, t, k_new);
+  st_set(W, dst, ST_NEW);
+  W.key[dst] = k_new;
   if (want_terms) W.terms[dst] = particle_terms(type, v, m / v, B->env);
   return 0;
 }
@@ -1972,7 +2210,9 @@ __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, in
         return;
       }
     }
-    const int err = direct_merge(A, W, B, p1, p2, slot, nid, new_group, cc, type, want_terms, &s);
+    const bool appended = !(o & (O_SLOT1 | O_SLOT2));
+    const int err = direct_merge(A, W, B, box, t, p1, p2, slot, appended, nid, new_group, cc, type,
+                                 want_terms, &s);
     if (err) atomicExch(&B->error, err);
   }
   if (o & O_RM1) {
@@ -1980,6 +2220,7 @@ __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, in
     W.dead_r[p1] = rid;
     al.live_add(key1, -1);
     if (!(o & O_SLOT1)) {
+      slot_leave(W, p1, t);
       st_set(W, p1, ST_HOLE);
       W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(p1 - B->off);
     }
@@ -1989,6 +2230,7 @@ __device__ inline void perset_merge(BoxDesc *B, const Arrays &A, const PW &W, in
     W.dead_r[p2] = rid;
     al.live_add(key2, -1);
     if (!(o & O_SLOT2)) {
+      slot_leave(W, p2, t);
       st_set(W, p2, ST_HOLE);
       W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(p2 - B->off);
     }
@@ -2036,6 +2278,7 @@ __device__ __noinline__ void acc_apply(BoxDesc *B, const Arrays &A, const PW &W,
       W.dead_t[q] = t;
       W.dead_r[q] = rid;
       al.live_add(W.key[q], -1);
+      slot_leave(W, q, t);
       st_set(W, q, ST_HOLE);
       W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(q - B->off);
       put_info(A, B, al, A.id[q], 2, owner);
@@ -2044,6 +2287,8 @@ __device__ __noinline__ void acc_apply(BoxDesc *B, const Arrays &A, const PW &W,
   const unsigned long long n_samp = (unsigned long long)W.r_nsamp[r];
   const unsigned long long n_coag = (unsigned long long)W.r_ncoag[r];
   al.count((int64_t)n_samp, (int64_t)n_coag, (flags & RF_WARN) ? 1 : 0);
+  atomicAdd((unsigned long long *)&B->n_acc_samp, n_samp);
+  if (n_coag) atomicAdd((unsigned long long *)&B->n_acc_coag, n_coag);
   if (W.r_err[r]) atomicExch(&B->error, W.r_err[r]);
   if (!(flags & RF_MERGED)) return;
   Rec np;
@@ -2051,8 +2296,14 @@ __device__ __noinline__ void acc_apply(BoxDesc *B, const Arrays &A, const PW &W,
   W.dead_t[p] = t;
   W.dead_r[p] = rid;
   al.live_add(W.key[p], -1);
-  write_new_particle(A, W, B, p, np, nb, type, want_terms);
+  // in the reference the target changes at the pairs where it merges: it
+  // leaves its list at the first and joins its final one at the last
+  const int64_t pb = (int64_t)box * W.P_stride;
+  const uint32_t t_leave = W.r_q1[r] >= 0 ? W.pair_exc[pb + W.r_q1[r]] : t;
+  const uint32_t t_join = W.r_q2[r] >= 0 ? W.pair_exc[pb + W.r_q2[r]] : t;
+  write_new_particle(A, W, B, box, t_leave, t_join, p, false, np, nb, type, want_terms);
   if (flags & RF_DIED) {
+    slot_leave(W, p, t_join);
     st_set(W, p, ST_HOLE);
     W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(p - B->off);
     put_info(A, B, al, target_id, 4, 0);
@@ -2072,7 +2323,8 @@ __device__ __noinline__ void acc_apply(BoxDesc *B, const Arrays &A, const PW &W,
       atomicExch(&B->error, ERR_CAPACITY);
       return;
     }
-    write_new_particle(A, W, B, slot, np, nb, type, want_terms);
+    const uint32_t t_made = W.pair_exc[pb + W.q_made[d]];
+    write_new_particle(A, W, B, box, t_made, t_made, slot, true, np, nb, type, want_terms);
     al.live_add(W.key[slot], 1);
   }
 }
@@ -2166,6 +2418,7 @@ k_par_apply_chains(BoxDesc *boxes, Arrays A, PW W, int E, int64_t hole_base, int
       W.dead_t[q] = t;
       W.dead_r[q] = rid;
       atomicAdd(&live[W.key[q]], -1);
+      slot_leave(W, q, t);
       st_set(W, q, ST_HOLE);
       W.holes[hole0 + ord] = ((uint64_t)box << 32) | (uint64_t)(q - B->off);
       const int64_t ki = info0 + ord;
@@ -2255,10 +2508,13 @@ __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
       unsigned long long ns = (unsigned long long)(v & (ACC_WARN - 1));
       atomicAdd((unsigned long long *)&B->n_samp, ns);
       atomicAdd((unsigned long long *)&B->step_samp, ns);
+      atomicAdd((unsigned long long *)&B->n_acc_samp, ns);
       if (v & ACC_WARN) atomicAdd((unsigned long long *)&B->n_warn, 1ull);
     }
   } else if (W.tp1[i] >= 0) {
     f = (consumed_before(W, W.tp1[i], t, rid) || consumed_before(W, W.tp2[i], t, rid)) ? 1 : 0;
+  } else if (W.tns[i] & TNS_ATHIN) {
+    f = 1;  // an arrival before the pair may still come from an event not yet settled
   }
   if (!f && W.trec[i] < 0 && (W.tns[i] & ACC_WARN)) {
     // a per-set test's k_max warning (src/coagulation.F90:749-751), once final
@@ -2285,7 +2541,7 @@ __device__ inline int lower_bound_u32(const uint32_t *v, int n, uint32_t x) {
 }
 
 constexpr int TAIL_THREADS = 512;
-constexpr int TF_EFFECT = 1, TF_ACC = 2, TF_RUN = 4, TF_WARN = 8;
+constexpr int TF_EFFECT = 1, TF_ACC = 2, TF_RUN = 4, TF_WARN = 8, TF_ATHIN = 16;
 
 // claims inside one kernel are read and released with device-scope atomics,
 // so they need no cache maintenance between the phases of a round
@@ -2331,7 +2587,8 @@ __device__ inline void tail_barrier() {
 __global__ void __launch_bounds__(TAIL_THREADS)
 k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pend, int type,
            uint64_t seed, uint32_t step, int precision, double band, double del_t,
-           uint32_t acc_base, int want_terms, int rid0, unsigned long long *rounds_out) {
+           uint32_t acc_base, int want_terms, int rid0, unsigned long long *rounds_out,
+           uint32_t batch_t0) {
   (void)n_box;
   const int box = blockIdx.x;
   const int tid = threadIdx.x;
@@ -2391,7 +2648,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     const int rec = is_acc ? (int)(W.acc_exc[k] + (t - W.pair_exc[k]) - acc_base) : -1;
     // 1. run the tests whose inputs are new or changed
     if (active && (fl & TF_RUN)) {
-      fl &= ~(TF_RUN | TF_EFFECT | TF_WARN);
+      fl &= ~(TF_RUN | TF_EFFECT | TF_WARN | TF_ATHIN);
       if (is_acc) {
         const int r = acc_walk(boxes, A, W, k, t, rec, type, seed, step, del_t, false, &p1, &tns);
         if (r == 1) fl |= TF_EFFECT;
@@ -2401,8 +2658,10 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
         perset_test(B, A, W, box, k, t, type, seed, step, precision, band, 0, o);
         p1 = o.p1;
         p2 = o.p2;
+        W.thin[t - batch_t0] = o.thin ? 1 : 0;
         if (o.accept) fl |= TF_EFFECT;
         if (o.warn) fl |= TF_WARN;
+        if (o.thin == 2) fl |= TF_ATHIN;
       }
     }
     const bool effect = fl & TF_EFFECT;
@@ -2511,6 +2770,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     // 6. what stays in the window: losers (run again only if an earlier
     // winner changed what they read) and tests whose inputs changed
     int keep = 0;
+    bool athin = false;
     if (active && !win) {
       bool changed = consumed_before(W, p1, t, rid) || consumed_before(W, p2, t, rid);
       if (is_acc && effect && !changed)
@@ -2520,16 +2780,30 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
       if (effect || changed) {
         keep = 1;
         if (changed) fl |= TF_RUN;
+      } else if (fl & TF_ATHIN) {
+        athin = true;  // decided below
       } else {
         // final without effect: its counts
         if (is_acc && tns) {
           c_samp += tns & (ACC_WARN - 1);
+          atomicAdd((unsigned long long *)&B->n_acc_samp,
+                    (unsigned long long)(tns & (ACC_WARN - 1)));
           if (tns & ACC_WARN) c_warn += 1;
         }
         if (!is_acc && (fl & TF_WARN)) c_warn += 1;
       }
     }
     if (win && !is_acc && (fl & TF_WARN)) c_warn += 1;
+    // A test thinned on an empty arrival entry is final once no earlier
+    // test of the window won this round (its arrivals show from the next
+    // round on) or waits for another round; until then it runs again.
+    int unsettled = (win || keep) ? 1 : 0, n_before;
+    BS(scan_tmp).ExclusiveSum(unsettled, n_before);
+    __syncthreads();
+    if (athin && n_before > 0) {
+      keep = 1;
+      fl |= TF_RUN;
+    }
     int r_keep;
     BS(scan_tmp).ExclusiveSum(keep, r_keep, tot);
     __syncthreads();
@@ -2692,6 +2966,17 @@ __global__ void k_par_end(BoxDesc *boxes, int n_box) {
 }
 
 // ------------------------------------------------------------ host side
+// The reference's tot_n_samp counts each pair's Poisson draw over the
+// members present when the loop reaches it (src/coagulation.F90:627): the
+// batch's thinned tests come off the counts drawn over the bound.
+__global__ void __launch_bounds__(256) k_thin_counts(BoxDesc *boxes, PW W, int64_t NP, uint32_t t0,
+                                                     int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int box = -1;
+  if (i < n && W.thin[i]) box = (int)(test_pair(W, NP, t0 + (uint32_t)i) / W.P_stride);
+  wave_flush_counts(boxes, box, -1, 0, 0);
+}
+
 inline int parallel_init(ParallelWork &W, int n_box, int max_bin) {
   W.n_box = n_box;
   W.d.max_bin = max_bin;
@@ -2708,7 +2993,7 @@ inline int parallel_init(ParallelWork &W, int n_box, int max_bin) {
 // the accelerated path's records, allocated when a step first has target slots
 inline void parallel_free_records(ParallelWork &W) {
   PW &d = W.d;
-  void *ptrs[] = {d.r_rdhead, d.r_nrd, d.r_nsamp, d.r_ncoag, d.r_nrm, d.r_nchain, d.r_head, d.r_ndup, d.r_ndead,
+  void *ptrs[] = {d.r_q1, d.r_q2, d.q_made, d.r_rdhead, d.r_nrd, d.r_nsamp, d.r_ncoag, d.r_nrm, d.r_nchain, d.r_head, d.r_ndup, d.r_ndead,
                   d.r_dhead, d.r_flags, d.r_err, d.q_vol, d.q_group, d.q_nprim, d.q_ncomp,
                   d.q_next, d.q_pos, d.q_bin, d.q_dead, d.q_csrc, d.q_ctime, d.q_least,
                   d.q_greatest, d.dup_top, d.rm_buf, d.rm_top};
@@ -2716,6 +3001,7 @@ inline void parallel_free_records(ParallelWork &W) {
     if (p) (void)hipFree(p);
   d.r_nsamp = d.r_ncoag = d.r_nrm = d.r_nchain = d.r_head = d.r_ndup = d.r_ndead = nullptr;
   d.r_dhead = d.r_flags = d.r_err = d.r_rdhead = d.r_nrd = nullptr;
+  d.r_q1 = d.r_q2 = d.q_made = nullptr;
   d.q_pos = d.q_bin = d.q_dead = nullptr;
   d.q_vol = d.q_ctime = d.q_least = d.q_greatest = nullptr;
   d.q_group = d.q_nprim = d.q_ncomp = d.q_next = d.q_csrc = nullptr;
@@ -2727,12 +3013,13 @@ inline int parallel_alloc_records(ParallelWork &W, int n_spec) {
   PW &d = W.d;
   const int64_t c = ACC_CAP, pc = (int64_t)ACC_CAP + DUP_CAP;
   int **rv[] = {&d.r_nsamp, &d.r_ncoag, &d.r_nrm, &d.r_nchain, &d.r_head, &d.r_ndup, &d.r_ndead,
-                &d.r_dhead, &d.r_flags, &d.r_err, &d.r_rdhead, &d.r_nrd};
+                &d.r_dhead, &d.r_flags, &d.r_err, &d.r_rdhead, &d.r_nrd, &d.r_q1, &d.r_q2};
   for (int **q : rv) PCHECK(pmalloc(q, c));
   PCHECK(pmalloc(&d.q_vol, (size_t)n_spec * pc));
   PCHECK(pmalloc(&d.q_csrc, (size_t)MAX_COMP * pc));
   PCHECK(pmalloc(&d.q_ctime, (size_t)MAX_COMP * pc));
-  int **qv[] = {&d.q_group, &d.q_nprim, &d.q_ncomp, &d.q_next, &d.q_pos, &d.q_bin, &d.q_dead};
+  int **qv[] = {&d.q_group, &d.q_nprim, &d.q_ncomp, &d.q_next, &d.q_pos, &d.q_bin, &d.q_dead,
+                &d.q_made};
   for (int **q : qv) PCHECK(pmalloc(q, pc));
   PCHECK(pmalloc(&d.q_least, pc));
   PCHECK(pmalloc(&d.q_greatest, pc));
@@ -2756,7 +3043,9 @@ inline void parallel_free_arrays(ParallelWork &W) {
                   d.holes, d.holes_sorted, d.cmp_lo, d.cmp_hi, d.cmp_head, d.acc_samp,
                   W.scan_tmp, W.sort_tmp,
                   d.pair_acc, d.pair_accn, d.acc_exc, d.trec, d.tns, d.ev_rec, d.misc,
-                  d.hole_top, d.tail_rounds, d.pend_k, d.tp_in, d.tpair};
+                  d.hole_top, d.tail_rounds, d.pend_k, d.tp_in, d.tpair,
+                  d.acap, d.arr_n, d.arr_head, d.ap_p, d.ap_t, d.ap_dep, d.ap_next, d.ap_top,
+                  d.arr_of, d.key0, d.depart_t, d.thin};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   parallel_free_records(W);
@@ -2804,6 +3093,9 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.dead_r, tc));
   PCHECK(pmalloc(&d.terms, tc));
   PCHECK(pmalloc(&d.perm, tc));
+  PCHECK(pmalloc(&d.key0, tc));
+  PCHECK(pmalloc(&d.depart_t, tc));
+  PCHECK(pmalloc(&d.arr_of, tc));
   // tiles over each box's capacity
   std::vector<int> tb, ts;
   std::vector<int64_t> hoff(n_box);
@@ -2837,6 +3129,9 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.lcnt, (size_t)n_box * d.K_stride));
   PCHECK(pmalloc(&d.live, (size_t)n_box * d.K_stride));
   PCHECK(hipMemset(d.live, 0, sizeof(int) * (size_t)n_box * d.K_stride));
+  PCHECK(pmalloc(&d.acap, (size_t)n_box * d.K_stride));
+  PCHECK(pmalloc(&d.arr_n, (size_t)n_box * d.K_stride));
+  PCHECK(pmalloc(&d.arr_head, (size_t)n_box * d.K_stride));
   W.NP = (int64_t)n_box * d.P_stride;
   PCHECK(pmalloc(&d.pair_n, W.NP + 1));
   PCHECK(pmalloc(&d.pair_exc, W.NP + 1));
@@ -2865,6 +3160,14 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.tp_in, T));
   PCHECK(pmalloc(&d.tpair, T));
   PCHECK(pmalloc(&d.tns, T));
+  PCHECK(pmalloc(&d.thin, T));
+  // arrivals of one step: at most one per merge or copy, over every batch
+  d.ap_cap = (uint32_t)std::min<int64_t>(2 * T + DUP_CAP, (int64_t)UINT32_MAX - 1);
+  PCHECK(pmalloc(&d.ap_p, d.ap_cap));
+  PCHECK(pmalloc(&d.ap_t, d.ap_cap));
+  PCHECK(pmalloc(&d.ap_dep, d.ap_cap));
+  PCHECK(pmalloc(&d.ap_next, d.ap_cap));
+  PCHECK(pmalloc(&d.ap_top, 1));
   PCHECK(pmalloc(&d.ev_rec, T));
   int **fl[] = {&d.f_app, &d.f_id, &d.f_info, &d.f_hole, &d.f_lose, &d.f_pend,
                 &d.s_app, &d.s_id, &d.s_info, &d.s_hole, &d.s_lose, &d.s_pend};
@@ -2966,6 +3269,7 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
                                           (int)((int64_t)d.K_hist * W.n_tiles), st));
   hipLaunchKernelGGL(k_par_lists, dim3((d.K_stride + 255) / 256, n_box), dim3(256), 0, st, d_box, d,
                      n_box);
+  hipLaunchKernelGGL(k_par_acap, dim3((d.K_stride + 255) / 256, n_box), dim3(256), 0, st, d_box, d);
   // keys are below mb_step * classes; the sentinel 2^key_bits - 1 above them
   int key_bits = 1;
   while ((1 << key_bits) <= mb_step * d.tab_classes) key_bits++;
@@ -3028,6 +3332,9 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
     t0 = t1;
     const uint32_t *pend = nullptr;
     const int n_batch = n;
+    if (n_batch > 0)
+      hipLaunchKernelGGL(k_fill_u8, dim3((n_batch + 255) / 256), dim3(256), 0, st, d.thin,
+                         (int64_t)n_batch, (uint8_t)0);
     for (int round = 0; n > 0; ++round) {
       // more device-wide rounds only while they settle most of the batch:
       // when over half of it is still pending, chains dominate and the tail
@@ -3045,7 +3352,7 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         PLAUNCH(W, "k_par_tail", k_par_tail, dim3(n_box), dim3(TAIL_THREADS),
                 d.K_stride * sizeof(int), st, d_box, A, d,
                 n_box, n, pend, type, seed, step, precision, band, del_t, a0,
-                want_terms, (int)W.rid + 1, d.tail_rounds);
+                want_terms, (int)W.rid + 1, d.tail_rounds, batch_t0);
         PCHECK(hipMemcpyAsync(W.h_u64, d.hole_top, sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, st));
         PCHECK(hipMemcpyAsync(W.h_u64 + 1, d.tail_rounds, sizeof(unsigned long long),
@@ -3120,6 +3427,9 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         // k_par_tests overwrites tid[i] with the same value it reads
       }
     }
+    if (n_batch > 0)
+      hipLaunchKernelGGL(k_thin_counts, dim3((n_batch + 255) / 256), dim3(256), 0, st, d_box, d,
+                         W.NP, batch_t0, n_batch);
   }
   // 6. compaction
   if (n_holes > 0) {
@@ -3130,6 +3440,8 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
             0, st, d_box, A, d, (int)n_holes);
     hipLaunchKernelGGL(k_compact_finish, gb, bb, 0, st, d_box, d, n_box);
   }
+  hipLaunchKernelGGL(k_par_arr_over, dim3((d.K_stride + 255) / 256, n_box), dim3(256), 0, st, d_box,
+                     d);
   hipLaunchKernelGGL(k_par_end, gb, bb, 0, st, d_box, n_box);
   PCHECK(hipGetLastError());
   return 0;

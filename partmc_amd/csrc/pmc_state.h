@@ -41,7 +41,11 @@ struct BoxDesc {
   int error;
   int n_rounds, n_dropped;   // parallel mode: tail claim rounds, dropped samples
   int64_t n_tail_tests;      // parallel mode: tests finished in the tail
+  int64_t n_arrivals;        // parallel mode: particles that joined a list within a step
+  int64_t n_arr_over;        // ... of them beyond the entries their list reserved
+  int64_t n_acc_samp, n_acc_coag;  // samples and coagulations of the per-particle path
   double log_lo, log_hi;     // log(edges[0]), log(edges[n_bin])
+  Tape tape;                 // exact mode: uniforms recorded or replayed in call order
   Weights w;
   Env env;
 };
