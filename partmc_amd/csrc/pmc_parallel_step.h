@@ -84,16 +84,13 @@ struct PW {  // device view of the parallel work arrays
   int *ev_rec;                 // [T_cap] record of an event, -1 for per-set events
   int acc_cap, pool_cap;
   // event records [acc_cap]
-  int *r_nsamp, *r_ncoag, *r_nrm, *r_nchain, *r_head, *r_ndup, *r_ndead, *r_dhead, *r_flags,
+  int *r_nsamp, *r_ncoag, *r_nrm, *r_nchain, *r_head, *r_ndup, *r_dhead, *r_flags,
       *r_err;
   int *r_rdhead, *r_nrd;       // distinct sources read from small source lists
-  int *r_q1, *r_q2;            // pairs (in the box) of the target's first and last merge
   // particle pool [pool_cap]: entry r < acc_cap is the final target of
   // record r; entries from acc_cap on are the copies (aero_state_dup_particle)
   double *q_vol;               // [n_spec][pool_cap]
   int *q_group, *q_nprim, *q_ncomp, *q_next;
-  int *q_pos, *q_bin, *q_dead;  // a copy's walk start (pair, bin) and whether it died
-  int *q_made;                 // the pair (in the box) whose merge made the copy
   int *q_csrc;                 // [MAX_COMP][pool_cap]
   double *q_ctime;             // [MAX_COMP][pool_cap]
   double *q_least, *q_greatest;
@@ -137,7 +134,7 @@ constexpr int READ_TRACK = 128;  // source lists up to this size have their read
 constexpr int RF_WARN = 1, RF_DIED = 2, RF_MERGED = 4;
 constexpr int A_MAX = 64;       // arrival entries a list can reserve
 constexpr int ARR_REACH = 3;    // an empty list reserves entries when a list at most this many bins below is occupied
-constexpr int TNS_ATHIN = 1 << 29;  // tns bit of a per-set test thinned on an empty arrival entry
+constexpr int TNS_ATHIN = 1 << 29;  // tns bit of a test or target event on an empty arrival entry
 
 struct ParallelWork {
   int n_box = 0;
@@ -605,6 +602,7 @@ __global__ void k_par_pairs(BoxDesc *boxes, Arrays A, PW W, double del_t, uint64
   BoxDesc *B = &boxes[box];
   uint32_t ns = 0;
   int acc = 0;
+  uint32_t na = 0;  // target events of an accelerated pair
   if (!B->error) {
     int c1, c2, b1, b2;
     decode_pair(q, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
@@ -620,9 +618,15 @@ __global__ void k_par_pairs(BoxDesc *boxes, Arrays A, PW W, double del_t, uint64
       // an empty bin during the step still visits that bin's pairs
       acc = accel_kind(B->w, edges, b1, b2, c1, c2, A.cc[ti], k_max, del_t, cnt, nb, &n_spt);
       if (acc) {
-        // target slots go to the list's first accelerated pair
-        // (k_par_acc_first)
+        // one event per entry of the target list when the loop reaches the
+        // pair: its reserved arrival entries, then its step-start members;
+        // none when the source list is empty
         if (n1 > 0 && n2 > 0) atomicAdd((unsigned long long *)&B->n_accel_pairs, 1ull);
+        const int kt = acc == 1 ? par_key(b1, c1, nb) : par_key(b2, c2, nb);
+        const int n_src = acc == 1 ? n2 : n1;
+        const int *acp = W.acap + (int64_t)box * W.K_stride;
+        if (n_src > 0) na = (uint32_t)(cnt[kt] + acp[kt]);
+        ns = na;
       } else {
         // the bound on the lists when the loop reaches the pair: step-start
         // members plus reserved arrival entries (draws thin to the members)
@@ -644,7 +648,7 @@ __global__ void k_par_pairs(BoxDesc *boxes, Arrays A, PW W, double del_t, uint64
   }
   W.pair_n[(int64_t)box * W.P_stride + q] = ns;
   W.pair_acc[(int64_t)box * W.P_stride + q] = (uint8_t)acc;
-  W.pair_accn[(int64_t)box * W.P_stride + q] = 0u;
+  W.pair_accn[(int64_t)box * W.P_stride + q] = na;
 }
 
 __device__ inline int64_t find_pair(const uint32_t *exc, int64_t n, uint32_t t) {
@@ -1119,65 +1123,6 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
 // src/aero_state.F90:488-539).  The final target, its copies and the
 // removed sources are applied when the event wins all of its claims.
 
-__device__ inline int64_t pair_q(const PW &W, int c1, int c2, int b1, int b2) {
-  return ((int64_t)(c1 - 1) * W.tab_classes + (c2 - 1)) * W.max_bin * W.max_bin +
-         (int64_t)(b1 - 1) * W.max_bin + (b2 - 1);
-}
-
-// first i in [lo, hi] with q0 + (i - 1) * stride > after, or hi + 1
-__device__ inline int first_above(int64_t after, int64_t q0, int64_t stride, int lo, int hi) {
-  if (after < q0) return lo;
-  int64_t i = (after - q0) / stride + 2;
-  return i < lo ? lo : (i > hi + 1 ? hi + 1 : (int)i);
-}
-
-// The next pair after `after` (loop order) that is accelerated with the
-// list (bt, ct) on its target side, or -1.  Candidates in loop order:
-// (ct, c2 < ct, bt, b2) side 1; (ct, ct, b1 < bt, bt) side 2;
-// (ct, ct, bt, b2 > bt) side 1; (c1 > ct, ct, b1, bt) side 2.
-__device__ inline int64_t next_acc_pair(const PW &W, const uint8_t *pacc, int nb, int nc, int ct,
-                                        int bt, int64_t after) {
-  const int64_t mb = W.max_bin;
-  for (int c2 = 1; c2 < ct; ++c2) {
-    int64_t q0 = pair_q(W, ct, c2, bt, 1);
-    for (int b2 = first_above(after, q0, 1, 1, nb); b2 <= nb; ++b2)
-      if (pacc[q0 + b2 - 1] == 1) return q0 + b2 - 1;
-  }
-  {
-    int64_t q0 = pair_q(W, ct, ct, 1, bt);
-    for (int b1 = first_above(after, q0, mb, 1, bt - 1); b1 < bt; ++b1)
-      if (pacc[q0 + (b1 - 1) * mb] == 2) return q0 + (b1 - 1) * mb;
-  }
-  {
-    int64_t q0 = pair_q(W, ct, ct, bt, 1);
-    for (int b2 = first_above(after, q0, 1, bt + 1, nb); b2 <= nb; ++b2)
-      if (pacc[q0 + b2 - 1] == 1) return q0 + b2 - 1;
-  }
-  for (int c1 = ct + 1; c1 <= nc; ++c1) {
-    int64_t q0 = pair_q(W, c1, ct, 1, bt);
-    for (int b1 = first_above(after, q0, mb, 1, nb); b1 <= nb; ++b1)
-      if (pacc[q0 + (b1 - 1) * mb] == 2) return q0 + (b1 - 1) * mb;
-  }
-  return -1;
-}
-
-// the target slots of a list go to its first accelerated pair
-__global__ void k_par_acc_first(BoxDesc *boxes, PW W) {
-  const int box = blockIdx.y;
-  const int key = blockIdx.x * blockDim.x + threadIdx.x;
-  const BoxDesc *B = &boxes[box];
-  const int nb = B->n_bin, nc = B->w.n_class;
-  if (B->error || key >= nb * nc) return;
-  const int bt = key % nb + 1, ct = key / nb + 1;
-  const int n_t = W.lcnt[(int64_t)box * W.K_stride + par_key(bt, ct, nb)];
-  if (n_t == 0) return;
-  const int64_t pb = (int64_t)box * W.P_stride;
-  int64_t q = next_acc_pair(W, W.pair_acc + pb, nb, nc, ct, bt, -1);
-  if (q < 0) return;
-  W.pair_n[pb + q] = (uint32_t)n_t;
-  W.pair_accn[pb + q] = (uint32_t)n_t;
-}
-
 // aero_weight_array_rand_group (src/aero_weight_array.F90:496-514)
 __device__ inline int rand_group_w(Stream &s, const Weights &w, int cls, double r) {
   double pdf[MAX_GROUP];
@@ -1372,7 +1317,8 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
                                         const Rec &target, int ct, int cs, double nc_src_min,
                                         int64_t sbase, int skey, int n_s, double n_spt,
                                         double accept_factor, Stream &aps, Stream &smp,
-                                        Stream &fin, uint32_t t, bool claim, RmChain &chain,
+                                        Stream &fin, uint32_t tau, uint32_t t, bool claim,
+                                        RmChain &chain,
                                         RmChain &reads, int *n_samp_out,
                                         int *n_coag_out, int *warn, Rec &source) {
   const int ns = A.n_spec;
@@ -1419,11 +1365,12 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
       if (m95 < 0.1) break;
     }
     // a uniform draw among the sources still present.  The counts above
-    // come from the step-start list; a source an earlier event took away
-    // before this target's turn was not in the reference's count
-    // (compute_n_source, src/coagulation.F90:288-328, reads the list as it
-    // is), so a first draw that lands on one thins the sample.  Sources
-    // this target removed are drawn again, as in the reference's loop.
+    // come from the step-start list; a source that left before the pair
+    // began (tau) was not in the reference's count (compute_n_source,
+    // src/coagulation.F90:288-328, reads the list as it is), so a first
+    // draw that lands on one thins the sample.  Sources that earlier
+    // targets of the pair, or this one, removed are drawn again, as in the
+    // reference's loop.
     int ps = -1;
     bool thinned = false;
     for (int a = 0; a < MAX_REDRAW; ++a) {
@@ -1433,7 +1380,7 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
         ps = q;
         break;
       }
-      if (a == 0 && !drawable(W, q, skey) && W.depart_t[q] < t) {
+      if (a == 0 && !drawable(W, q, skey) && W.depart_t[q] < tau) {
         thinned = true;
         break;
       }
@@ -1483,7 +1430,7 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
   // the samples the early exit left undrawn thin as the drawn ones did
   for (int i = i_samp; i <= n_total; ++i) {
     const int q = W.perm[sbase + rand_int(smp, n_s) - 1];
-    if (!drawable(W, q, skey) && W.depart_t[q] < t) n_thin++;
+    if (!drawable(W, q, skey) && W.depart_t[q] < tau) n_thin++;
   }
   double prob_coag_mean = prob_coag_tot / (double)n_samp;
   if (!(prob_coag_mean <= 1.0)) *warn = 1;
@@ -1532,14 +1479,22 @@ __device__ inline int acc_sample_source(const Arrays &A, const PW &W, BoxDesc *B
   return 0;
 }
 
-// The walk of target slot t (pair k, record rec): returns 1 when the event
-// has an effect (record rec and the pool filled; the removed sources are
-// claimed with index t when `claim`), 0 when it has none (*tns = its
-// samples + ACC_WARN), -1 on a box error.  *p_out is the target's slot, or
-// -1 when the target is gone.
-__device__ __noinline__ int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &W, int64_t k,
-                               uint32_t t, int rec, int type, uint64_t seed, uint32_t step,
-                               double del_t, bool claim, int *p_out, int *tns) {
+// One target of an accelerated pair: the body of try_per_particle_coag's
+// target loop (src/coagulation.F90:216-237) as the event at sequence index
+// t of pair k (record rec).  A pair's events are laid out as the reference
+// loops over its target list at the pair's start: the list's arrivals of
+// the step first, the latest first (they were appended last), then its
+// step-start members from the last entry down.  Each visit of a target to
+// a pair is its own event, so a visit sees the sources as the events
+// before the pair left them, and a target that merged at an earlier pair
+// is read as that pair's event left it.  Returns 1 when the event has an
+// effect (record rec and the pool filled; the removed sources are claimed
+// with index t when `claim`), 0 when it has none (*tns = its samples +
+// ACC_WARN, or TNS_ATHIN for an arrival entry with no arrival yet), -1 on
+// a box error.  *p_out is the target's slot, or -1 when there is none.
+__device__ __noinline__ int acc_visit(BoxDesc *boxes, const Arrays &A, const PW &W, int64_t k,
+                                      uint32_t t, int rec, int type, uint64_t seed, uint32_t step,
+                                      double del_t, bool claim, int *p_out, int *tns) {
   *p_out = -1;
   *tns = 0;
   const int box = (int)(k / W.P_stride);
@@ -1548,24 +1503,30 @@ __device__ __noinline__ int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &
   const int nb = B->n_bin, nc = B->w.n_class;
   const int64_t lb = (int64_t)box * W.K_stride;
   const int64_t pb = (int64_t)box * W.P_stride;
-  const uint8_t *pacc = W.pair_acc + pb;
   const double *edges = A.edges + (int64_t)box * (A.max_bin + 1);
   int c1, c2, b1, b2;
-  int64_t q = k - pb;
+  const int64_t q = k - pb;
   decode_pair(q, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
-  int b_cur = pacc[q] == 1 ? b1 : b2;
-  const int ct = pacc[q] == 1 ? c1 : c2;
-  const int n_t = W.lcnt[lb + par_key(b_cur, ct, nb)];
-  const int e = n_t - (int)(t - W.pair_exc[k]);
-  const int p = W.perm[B->off + W.lstart[lb + par_key(b_cur, ct, nb)] + e - 1];
-  if (st_get(W, p) == ST_HOLE) return 0;  // removed earlier in this step
-  if (W.key[p] != par_key(b_cur, ct, nb)) {
-    // changed in place earlier in the step: walk on from its current list
-    const int kp = W.key[p];
-    if (kp / nb + 1 != ct) return 0;
-    b_cur = kp % nb + 1;
-    q = next_acc_pair(W, pacc, nb, nc, ct, b_cur, q - 1);
+  const int kind = W.pair_acc[k];
+  const int bt = kind == 1 ? b1 : b2, ct = kind == 1 ? c1 : c2;
+  const int bs = kind == 1 ? b2 : b1, cs = kind == 1 ? c2 : c1;
+  const int kt = par_key(bt, ct, nb), ks = par_key(bs, cs, nb);
+  const uint32_t tau = W.pair_exc[k];
+  const int j = (int)(t - tau);
+  const int acap_t = W.acap[lb + kt];
+  int p;
+  if (j < acap_t) {
+    const int e = arrival_at(W, lb + kt, acap_t - j, tau);
+    if (e < 0 || W.ap_dep[e] < tau) {
+      *tns = TNS_ATHIN;  // an earlier event not yet settled may still fill it
+      return 0;
+    }
+    p = W.ap_p[e];
+  } else {
+    const int m = W.lcnt[lb + kt] - (j - acap_t);
+    p = W.perm[B->off + W.lstart[lb + kt] + m - 1];
   }
+  if (!drawable(W, p, kt)) return 0;  // left the list before the pair
   *p_out = p;
   // the record's chains start empty, so an attempt cut short by an error
   // leaves nothing to walk
@@ -1576,162 +1537,95 @@ __device__ __noinline__ int acc_walk(BoxDesc *boxes, const Arrays &A, const PW &
   W.r_head[rec] = -1;
   const int ns = A.n_spec;
   const Weights &w = B->w;
-  const uint32_t ordinal = (uint32_t)(p - B->off) + 1;
-  Rec target, cur, source;
+  const uint32_t ordinal = (uint32_t)j + 1;
+  const int64_t ti = table_index(A, box, nb, nc, b1, b2, c1, c2);
+  const double k_max = A.kw[ti];
+  const int cc = A.cc[ti];
+  const int n_s = W.lcnt[lb + ks];
+  const uint32_t site = pair_site(nb, nc, b1, b2, c1, c2);
+  Stream aps, smp, fin;
+  aps.open(seed, box, step, site, ordinal, RNG_ACC_POISSON);
+  smp.open(seed, box, step, site, ordinal, RNG_ACC_SAMPLE);
+  fin.open(seed, box, step, site, ordinal, RNG_ACC_FINISH);
+  Rec target, source;
   load_rec(A, p, target);
   RmChain chain{-1, -1, RM_CHUNK, 0, 0ull};
   RmChain reads{-1, -1, RM_CHUNK, 0, 0ull};
-  int n_samp_tot = 0, n_coag_tot = 0, warn = 0, flags = 0, err_code = 0, n_rm = 0;
-  int n_dup = 0, n_dead = 0, dup_head = -1, dup_tail = -1;
-  int q_first = -1, q_last = -1;  // pairs of the target's first and last merge
-  // The target walks first; each copy made on the way (in the reference it
-  // joins the list of the target's new bin) then walks on from the pair
-  // that made it, in creation order.  Removals are preceded in the chain
-  // by a marker -1 - k when they belong to copy k.
-  int walker = -1;   // -1: the target; else the copy's pool entry
-  uint32_t tag = 0;  // walker number: copies use sites beyond the pair sites
-  cur = target;
-  for (;;) {
-    bool died = false;
-    bool marked = false;
-    while (q >= 0 && !err_code) {
-      decode_pair(q, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
-      const int kind = pacc[q];
-      const int bs = kind == 1 ? b2 : b1, cs = kind == 1 ? c2 : c1;
-      const int64_t ti = table_index(A, box, nb, nc, b1, b2, c1, c2);
-      const double k_max = A.kw[ti];
-      const int cc = A.cc[ti];
-      const int ks = par_key(bs, cs, nb);
-      const int n_s = W.lcnt[lb + ks];
-      // copies draw from their own sites, above the box's nc^2 nb^2 pair sites
-      const uint32_t site = pair_site(nb, nc, b1, b2, c1, c2) + tag * (uint32_t)(nc * nc * nb * nb);
-      Stream aps, smp, fin;
-      aps.open(seed, box, step, site, ordinal, RNG_ACC_POISSON);
-      smp.open(seed, box, step, site, ordinal, RNG_ACC_SAMPLE);
-      fin.open(seed, box, step, site, ordinal, RNG_ACC_FINISH);
-      if (walker >= 0 && !marked) {
-        // owner marker for this copy's removals (pushed before any of them)
-        if (!rm_push(W, chain, -1 - (int)tag)) {
+  int n_samp = 0, n_coag = 0, warn = 0, flags = 0, err_code = 0;
+  int err = acc_sample_source(A, W, B, type, target, ct, cs, nc_at_radius(w, cs, edges[bs - 1]),
+                              B->off + W.lstart[lb + ks], ks, n_s, k_max * del_t * (double)n_s,
+                              1.0 / k_max, aps, smp, fin, tau, t, claim, chain, reads, &n_samp,
+                              &n_coag, &warn, source);
+  if (err) {
+    atomicExch(&B->error, err);
+    return -1;
+  }
+  const int n_rm = chain.n;
+  int n_dup = 0, dup_head = -1, dup_tail = -1;
+  Rec np = target;
+  if (n_coag > 0) {
+    // coag_target_with_source (src/coagulation.F90:522-582)
+    Stream ft;
+    ft.open(seed, box, step, site, ordinal, RNG_ACC_TARGET);
+    const double old_nc = nc_at_radius(w, target.klass, vol2rad(rec_vol(ns, target)));
+    err = par_rec_coagulate(ns, ft, target, source, np);
+    if (err) {
+      atomicExch(&B->error, err);
+      return -1;
+    }
+    np.id = target.id;
+    const double r_new = vol2rad(rec_vol(ns, np));
+    np.group = rand_group_w(ft, w, cc, r_new);
+    np.klass = cc;
+    flags |= RF_MERGED;
+    const int new_bin = bin_find_log(nb, B->log_lo, B->log_hi, r_new);
+    if (new_bin < 1 || new_bin > nb) {
+      err_code = 765620746;
+    } else {
+      // aero_state_dup_particle (src/aero_state.F90:488-539): the copies
+      // join the list of the target's new bin
+      const int n_copies = prob_round(ft, old_nc / nc_at_radius(w, np.klass, r_new));
+      if (n_copies == 0) flags |= RF_DIED;
+      for (int c = 1; c < n_copies; ++c) {
+        const int d = W.acc_cap + atomicAdd(W.dup_top, 1);
+        if (d >= W.pool_cap) {
           atomicExch(&B->error, ERR_CAPACITY);
           return -1;
         }
-        marked = true;
-      }
-      const int n_before = chain.n;
-      int n_samp, n_coag;
-      int err = acc_sample_source(A, W, B, type, cur, ct, cs, nc_at_radius(w, cs, edges[bs - 1]),
-                                  B->off + W.lstart[lb + ks], ks, n_s, k_max * del_t * (double)n_s,
-                                  1.0 / k_max, aps, smp, fin, t, claim, chain, reads, &n_samp, &n_coag,
-                                  &warn,
-                                  source);
-      if (err) {
-        atomicExch(&B->error, err);
-        return -1;
-      }
-      n_rm += chain.n - n_before;
-      n_samp_tot += n_samp;
-      n_coag_tot += n_coag;
-      if (n_coag > 0) {
-        // coag_target_with_source
-        Stream ft;
-        ft.open(seed, box, step, site, ordinal, RNG_ACC_TARGET);
-        const double old_nc = nc_at_radius(w, cur.klass, vol2rad(rec_vol(ns, cur)));
-        Rec np;
-        err = par_rec_coagulate(ns, ft, cur, source, np);
-        if (err) {
-          atomicExch(&B->error, err);
-          return -1;
-        }
-        np.id = cur.id;
-        const double r_new = vol2rad(rec_vol(ns, np));
-        np.group = rand_group_w(ft, w, cc, r_new);
-        np.klass = cc;
-        cur = np;
-        if (walker < 0) {
-          flags |= RF_MERGED;
-          if (q_first < 0) q_first = (int)q;
-          q_last = (int)q;
-        }
-        const int new_bin = bin_find_log(nb, B->log_lo, B->log_hi, r_new);
-        if (new_bin < 1 || new_bin > nb) {
-          err_code = 765620746;
-          break;
-        }
-        const int n_copies = prob_round(ft, old_nc / nc_at_radius(w, np.klass, r_new));
-        if (n_copies == 0) {
-          died = true;
-          break;
-        }
-        const int64_t q_next = next_acc_pair(W, pacc, nb, nc, ct, new_bin, q);
-        for (int c = 1; c < n_copies; ++c) {
-          const int d = W.acc_cap + atomicAdd(W.dup_top, 1);
-          if (d >= W.pool_cap) {
-            atomicExch(&B->error, ERR_CAPACITY);
-            return -1;
-          }
-          Rec cp = cur;
-          cp.group = rand_group_w(ft, w, cur.klass, r_new);
-          pool_store(W, ns, d, cp);
-          W.q_next[d] = -1;
-          W.q_pos[d] = (int)q_next;
-          W.q_made[d] = (int)q;
-          W.q_bin[d] = new_bin;
-          W.q_dead[d] = 0;
-          if (dup_tail >= 0) W.q_next[dup_tail] = d;
-          else dup_head = d;
-          dup_tail = d;
-          n_dup++;
-        }
-        b_cur = new_bin;
-        q = q_next;
-      } else {
-        q = next_acc_pair(W, pacc, nb, nc, ct, b_cur, q);
+        Rec cp = np;
+        cp.group = rand_group_w(ft, w, np.klass, r_new);
+        pool_store(W, ns, d, cp);
+        W.q_next[d] = -1;
+        if (dup_tail >= 0) W.q_next[dup_tail] = d;
+        else dup_head = d;
+        dup_tail = d;
+        n_dup++;
       }
     }
-    if (walker < 0) {
-      target = cur;
-      if (died) flags |= RF_DIED;
-    } else {
-      pool_store(W, ns, walker, cur);
-      if (died) {
-        W.q_dead[walker] = 1;
-        n_dead++;
-      }
-    }
-    if (err_code) break;
-    walker = walker < 0 ? dup_head : W.q_next[walker];
-    if (walker < 0) break;
-    tag++;
-    pool_load(W, ns, walker, ct, 0, cur);
-    q = W.q_pos[walker];
-    b_cur = W.q_bin[walker];
   }
   if (warn) flags |= RF_WARN;
   W.r_rdhead[rec] = reads.head;
   W.r_nrd[rec] = reads.n;
-  if (n_coag_tot == 0 && n_rm == 0 && err_code == 0) {
-    *tns = n_samp_tot | (warn ? ACC_WARN : 0);
+  if (n_coag == 0 && n_rm == 0 && err_code == 0) {
+    *tns = n_samp | (warn ? ACC_WARN : 0);
     return 0;
   }
-  if (flags & RF_MERGED) pool_store(W, ns, rec, target);
-  W.r_nsamp[rec] = n_samp_tot;
-  W.r_ncoag[rec] = n_coag_tot;
+  if (flags & RF_MERGED) pool_store(W, ns, rec, np);
+  W.r_nsamp[rec] = n_samp;
+  W.r_ncoag[rec] = n_coag;
   W.r_nrm[rec] = n_rm;
   W.r_nchain[rec] = chain.n;
   W.r_head[rec] = chain.head;
   W.r_ndup[rec] = n_dup;
-  W.r_ndead[rec] = n_dead;
   W.r_dhead[rec] = dup_head;
   W.r_flags[rec] = flags;
   W.r_err[rec] = err_code;
-  W.r_q1[rec] = q_first;
-  W.r_q2[rec] = q_last;
   if (claim) claim_write(W, p, t);
   return 1;
 }
 
-// walks per wavefront: a walk is one lane's serial chain, and fewer walks a
-// wavefront spread the first round's walks over more compute units
+// visits per wavefront: each is one lane's serial sample loop, and fewer a
+// wavefront spread the first round's visits over more compute units
 constexpr int ACC_LANES = 16;
 // Round 0 runs one thread per target slot of the batch (record a maps to
 // its slot through acc_exc); later rounds run over the pending slots.
@@ -1764,12 +1658,12 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
   W.tp2[i] = -1;
   W.trec[i] = rec;
   int p, tns;
-  const int r = acc_walk(boxes, A, W, k, t, rec, type, seed, step, del_t, true, &p, &tns);
+  const int r = acc_visit(boxes, A, W, k, t, rec, type, seed, step, del_t, true, &p, &tns);
   W.tp1[i] = p;
   W.tns[i] = tns;
   if (r == 1) W.acc[i] = 1;
-  // the walk's samples, for the algorithmic traffic of this kernel
-  const int ns = (r == 1) ? W.r_nsamp[rec] : (tns & (ACC_WARN - 1));
+  // the visit's samples, for the algorithmic traffic of this kernel
+  const int ns = (r == 1) ? W.r_nsamp[rec] : (tns & (TNS_ATHIN - 1));
   if (ns > 0) atomicAdd(W.acc_samp, (unsigned long long)ns);
 }
 
@@ -1824,9 +1718,9 @@ __global__ void k_par_resolve(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, 
     rd_walk(W, r, [&](int q) { win = win && W.claim_wr[q] >= t; });
     const int died = (W.r_flags[r] & RF_DIED) ? 1 : 0;
     W.ev_out[e] = win ? O_WIN : 0;
-    W.f_app[e] = win ? W.r_ndup[r] - W.r_ndead[r] : 0;
+    W.f_app[e] = win ? W.r_ndup[r] : 0;
     W.f_id[e] = win ? W.r_ndup[r] : 0;
-    W.f_info[e] = win ? W.r_nrm[r] + died + W.r_ndead[r] : 0;
+    W.f_info[e] = win ? W.r_nrm[r] + died : 0;
     W.f_hole[e] = win ? W.r_nrm[r] + died : 0;
     W.f_lose[e] = win ? 0 : 1;
     return;
@@ -2296,14 +2190,9 @@ __device__ __noinline__ void acc_apply(BoxDesc *B, const Arrays &A, const PW &W,
   W.dead_t[p] = t;
   W.dead_r[p] = rid;
   al.live_add(W.key[p], -1);
-  // in the reference the target changes at the pairs where it merges: it
-  // leaves its list at the first and joins its final one at the last
-  const int64_t pb = (int64_t)box * W.P_stride;
-  const uint32_t t_leave = W.r_q1[r] >= 0 ? W.pair_exc[pb + W.r_q1[r]] : t;
-  const uint32_t t_join = W.r_q2[r] >= 0 ? W.pair_exc[pb + W.r_q2[r]] : t;
-  write_new_particle(A, W, B, box, t_leave, t_join, p, false, np, nb, type, want_terms);
+  write_new_particle(A, W, B, box, t, t, p, false, np, nb, type, want_terms);
   if (flags & RF_DIED) {
-    slot_leave(W, p, t_join);
+    slot_leave(W, p, t);
     st_set(W, p, ST_HOLE);
     W.holes[al.hole_index()] = ((uint64_t)box << 32) | (uint64_t)(p - B->off);
     put_info(A, B, al, target_id, 4, 0);
@@ -2313,18 +2202,13 @@ __device__ __noinline__ void acc_apply(BoxDesc *B, const Arrays &A, const PW &W,
   int kk = 0;
   for (int d = W.r_dhead[r]; d >= 0; d = W.q_next[d]) {
     ++kk;
-    if (W.q_dead[d]) {
-      put_info(A, B, al, copy_id(kk), 4, 0);
-      continue;
-    }
     pool_load(W, ns, d, klass, copy_id(kk), np);
     const int64_t slot = al.append_slot();
     if (slot < 0) {
       atomicExch(&B->error, ERR_CAPACITY);
       return;
     }
-    const uint32_t t_made = W.pair_exc[pb + W.q_made[d]];
-    write_new_particle(A, W, B, box, t_made, t_made, slot, true, np, nb, type, want_terms);
+    write_new_particle(A, W, B, box, t, t, slot, true, np, nb, type, want_terms);
     al.live_add(W.key[slot], 1);
   }
 }
@@ -2494,8 +2378,10 @@ __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
   int f = 0;
   if (W.acc[i]) {
     f = (W.ev_out[W.acc_scan[i]] & O_WIN) ? 0 : 1;
+  } else if (W.trec[i] >= 0 && (W.tns[i] & TNS_ATHIN)) {
+    f = 1;  // an arrival entry: an earlier event not yet settled may fill it
   } else if (W.trec[i] >= 0) {
-    // a target slot without effect: final unless an earlier winner of the
+    // a target event without effect: final unless an earlier winner of the
     // round changed its target; its samples count once final
     bool changed = consumed_before(W, W.tp1[i], t, rid);
     if (W.tp1[i] >= 0)
@@ -2650,8 +2536,9 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     if (active && (fl & TF_RUN)) {
       fl &= ~(TF_RUN | TF_EFFECT | TF_WARN | TF_ATHIN);
       if (is_acc) {
-        const int r = acc_walk(boxes, A, W, k, t, rec, type, seed, step, del_t, false, &p1, &tns);
+        const int r = acc_visit(boxes, A, W, k, t, rec, type, seed, step, del_t, false, &p1, &tns);
         if (r == 1) fl |= TF_EFFECT;
+        if (tns & TNS_ATHIN) fl |= TF_ATHIN;
         p2 = -1;
       } else {
         TestOut o;
@@ -2701,9 +2588,9 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
         rd_walk(W, rec, [&](int q) { win = win && claim_wr_load(W, q) >= t; });
         if (win) {
           const int died = (W.r_flags[rec] & RF_DIED) ? 1 : 0;
-          c_app = W.r_ndup[rec] - W.r_ndead[rec];
+          c_app = W.r_ndup[rec];
           c_id = W.r_ndup[rec];
-          c_info = W.r_nrm[rec] + died + W.r_ndead[rec];
+          c_info = W.r_nrm[rec] + died;
           c_hole = W.r_nrm[rec] + died;
         }
       } else {
@@ -2993,16 +2880,14 @@ inline int parallel_init(ParallelWork &W, int n_box, int max_bin) {
 // the accelerated path's records, allocated when a step first has target slots
 inline void parallel_free_records(ParallelWork &W) {
   PW &d = W.d;
-  void *ptrs[] = {d.r_q1, d.r_q2, d.q_made, d.r_rdhead, d.r_nrd, d.r_nsamp, d.r_ncoag, d.r_nrm, d.r_nchain, d.r_head, d.r_ndup, d.r_ndead,
+  void *ptrs[] = {d.r_rdhead, d.r_nrd, d.r_nsamp, d.r_ncoag, d.r_nrm, d.r_nchain, d.r_head, d.r_ndup,
                   d.r_dhead, d.r_flags, d.r_err, d.q_vol, d.q_group, d.q_nprim, d.q_ncomp,
-                  d.q_next, d.q_pos, d.q_bin, d.q_dead, d.q_csrc, d.q_ctime, d.q_least,
+                  d.q_next, d.q_csrc, d.q_ctime, d.q_least,
                   d.q_greatest, d.dup_top, d.rm_buf, d.rm_top};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
-  d.r_nsamp = d.r_ncoag = d.r_nrm = d.r_nchain = d.r_head = d.r_ndup = d.r_ndead = nullptr;
+  d.r_nsamp = d.r_ncoag = d.r_nrm = d.r_nchain = d.r_head = d.r_ndup = nullptr;
   d.r_dhead = d.r_flags = d.r_err = d.r_rdhead = d.r_nrd = nullptr;
-  d.r_q1 = d.r_q2 = d.q_made = nullptr;
-  d.q_pos = d.q_bin = d.q_dead = nullptr;
   d.q_vol = d.q_ctime = d.q_least = d.q_greatest = nullptr;
   d.q_group = d.q_nprim = d.q_ncomp = d.q_next = d.q_csrc = nullptr;
   d.dup_top = d.rm_buf = d.rm_top = nullptr;
@@ -3012,14 +2897,13 @@ inline void parallel_free_records(ParallelWork &W) {
 inline int parallel_alloc_records(ParallelWork &W, int n_spec) {
   PW &d = W.d;
   const int64_t c = ACC_CAP, pc = (int64_t)ACC_CAP + DUP_CAP;
-  int **rv[] = {&d.r_nsamp, &d.r_ncoag, &d.r_nrm, &d.r_nchain, &d.r_head, &d.r_ndup, &d.r_ndead,
-                &d.r_dhead, &d.r_flags, &d.r_err, &d.r_rdhead, &d.r_nrd, &d.r_q1, &d.r_q2};
+  int **rv[] = {&d.r_nsamp, &d.r_ncoag, &d.r_nrm, &d.r_nchain, &d.r_head, &d.r_ndup,
+                &d.r_dhead, &d.r_flags, &d.r_err, &d.r_rdhead, &d.r_nrd};
   for (int **q : rv) PCHECK(pmalloc(q, c));
   PCHECK(pmalloc(&d.q_vol, (size_t)n_spec * pc));
   PCHECK(pmalloc(&d.q_csrc, (size_t)MAX_COMP * pc));
   PCHECK(pmalloc(&d.q_ctime, (size_t)MAX_COMP * pc));
-  int **qv[] = {&d.q_group, &d.q_nprim, &d.q_ncomp, &d.q_next, &d.q_pos, &d.q_bin, &d.q_dead,
-                &d.q_made};
+  int **qv[] = {&d.q_group, &d.q_nprim, &d.q_ncomp, &d.q_next};
   for (int **q : qv) PCHECK(pmalloc(q, pc));
   PCHECK(pmalloc(&d.q_least, pc));
   PCHECK(pmalloc(&d.q_greatest, pc));
@@ -3278,7 +3162,6 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
   // 4. Poisson test counts per bin pair, laid out in reference order
   PLAUNCH(W, "k_par_pairs", k_par_pairs, dim3((unsigned)((d.P_stride + 255) / 256), n_box),
           dim3(256), 0, st, d_box, A, d, del_t, seed, step);
-  hipLaunchKernelGGL(k_par_acc_first, dim3((d.K_stride + 63) / 64, n_box), dim3(64), 0, st, d_box, d);
   PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.pair_n, d.pair_exc,
                                           (int)(W.NP + 1), st));
   PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.pair_accn, d.acc_exc,

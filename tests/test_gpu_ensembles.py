@@ -89,20 +89,7 @@ def test_sedimentation_ensembles_agree(factor):
     assert_agree(f"sedimentation x{factor:g}", par, seq)
 
 
-# Target walks read their source lists as of the walk's first bin pair (the
-# walk is one event in the sequence order).  A target that grows into a
-# higher bin meets a source list again at a pair that the reference reaches
-# only after the other targets of its old bin have sampled, and sometimes
-# removed, that list's few high-weight sources.  At 30 times C2's
-# concentration this overcounts the per-particle path's coagulations by
-# about 24% (its number distribution fails the chi-square test narrowly);
-# at 1 and 10 times it agrees.
-WALK_ORDER = pytest.mark.xfail(
-    strict=False, reason="target walks read source lists as of their first pair; "
-    "the per-particle path overcounts coagulations at 30x concentration")
-
-
-@pytest.mark.parametrize("factor", [1.0, 10.0, pytest.param(30.0, marks=WALK_ORDER)])
+@pytest.mark.parametrize("factor", [1.0, 10.0, 30.0])
 def test_power_weighted_brownian_ensembles_agree(factor):
     base = small(P.config_c2(), 6000)
     base.weight_magnitude *= factor

@@ -179,9 +179,18 @@ def conc_metrics(vol, pop):
     return np.array([len(r), nc.sum(), nc[r > 2e-7].sum(), nc[r < 5e-8].sum()])
 
 
-@pytest.mark.parametrize("boost", [10.0, pytest.param(30.0, marks=pytest.mark.xfail(
-    strict=False, reason="target walks read source lists as of their first pair; the "
-    "per-particle path overcounts coagulations at 30x concentration"))])
+# At 30 times C2's concentration the per-particle path's coagulation count
+# agrees at the first step (+2%, 0.8 standard errors over 128 runs) and then
+# drifts up, to +18% after 7 steps: the few high-weight sources are removed
+# more slowly than in the sequential chain, by a mechanism that needs the
+# medium particles of bins 15-25 beside them (a box without them agrees).
+# The number concentrations and size distributions agree at every factor.
+COUNT_DRIFT = pytest.mark.xfail(
+    strict=False, reason="the per-particle coagulation count drifts up over steps at 30x "
+    "concentration; concentrations agree")
+
+
+@pytest.mark.parametrize("boost", [10.0, pytest.param(30.0, marks=COUNT_DRIFT)])
 def test_accelerated_ensemble_matches_oracle(boost):
     """Power weighting (trimodal, exponent -3): bin pairs with a target bin of
     lower weight take the per-particle path (try_per_particle_coag,
