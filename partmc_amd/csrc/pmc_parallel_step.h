@@ -111,17 +111,26 @@ struct PW {  // device view of the parallel work arrays
   // j-th arrival before the pair begins (in sequence order).
   int *acap;          // [n_box * K_stride] reserved arrival entries of the list this step
   int *arr_n;         // [n_box * K_stride] arrivals recorded this step
-  int *arr_head;      // [n_box * K_stride] the list's arrivals: first pool entry, -1 none
-  int *ap_p;          // [ap_cap] arrival pool: the arrival's slot
-  uint32_t *ap_t;     // [ap_cap] sequence index at which it joined the list
-  uint32_t *ap_dep;   // [ap_cap] at which it left again (UNCLAIMED: still there)
-  int *ap_next;       // [ap_cap] next arrival of the same list
-  uint32_t *ap_top;   // [1] pool entries used this step
-  uint32_t ap_cap;
+  int *arr_head;      // [n_box * K_stride] the list's arrivals beyond A_MAX: first entry, -1 none
+  // arrival pool: list L's first A_MAX arrivals at L * A_MAX + i, the rest
+  // from ap_ovf on, chained from arr_head[L]
+  int *ap_p;          // the arrival's slot
+  uint32_t *ap_t;     // sequence index at which it joined the list
+  uint32_t *ap_dep;   // at which it left again (UNCLAIMED: still there)
+  int *ap_next;       // next overflow arrival of the same list
+  uint32_t *ap_top;   // [1] overflow entries used this step
+  int64_t ap_ovf;     // n_box * K_stride * A_MAX
+  uint32_t ap_cap;    // overflow entries
   int *arr_of;        // [total_cap] pool entry of the arrival a slot holds
   int *key0;          // [total_cap] a touched slot's list key at step start
   uint32_t *depart_t; // [total_cap] when a touched slot's step-start particle left that list
   uint8_t *thin;      // [T_cap] the test's last draw was thinned (per batch)
+  // Per (box, bin): the lowest index of an event pending at the start of
+  // this round whose results land in that bin or a lower one (tests on an
+  // empty arrival entry excluded; particles only grow, so an event on bins
+  // b1, b2 adds particles to bins from max(b1, b2) up), as a running
+  // minimum over the bins; and the same, per bin, for the next round.
+  uint32_t *minpend, *minpend_next;  // [n_box * K_stride] each
 };
 
 constexpr int ACC_CAP = 1 << 18;
@@ -135,6 +144,7 @@ constexpr int RF_WARN = 1, RF_DIED = 2, RF_MERGED = 4;
 constexpr int A_MAX = 64;       // arrival entries a list can reserve
 constexpr int ARR_REACH = 3;    // an empty list reserves entries when a list at most this many bins below is occupied
 constexpr int TNS_ATHIN = 1 << 29;  // tns bit of a test or target event on an empty arrival entry
+constexpr int TNS_SIDE2 = 1 << 28;  // ... of the pair's second list
 
 struct ParallelWork {
   int n_box = 0;
@@ -717,24 +727,33 @@ __device__ inline void slot_touch(const PW &W, int64_t p) {
   }
 }
 
-// An arrival joins its list's pool chain.  The chain's order is that of
-// the atomics; readers rank the entries by (sequence index, slot).  The
-// entry is complete before it is linked, and the chains a round reads were
-// linked in earlier rounds (kernels, or tail rounds behind a barrier).
+// An arrival takes the next entry of its list's array, or past A_MAX of
+// them an overflow entry linked into the list's chain.  Which entries the
+// arrivals take depends on the order of the atomics, so readers use only
+// the set: counts and choices by (sequence index, slot) or by a hash of the
+// slot.  The entry is complete before the readers of a later round (a
+// later kernel, or a tail round behind a barrier) look at it.
 __device__ inline void record_arrival(const PW &W, BoxDesc *B, int box, int key, int64_t p,
                                       uint32_t t) {
   const int64_t L = (int64_t)box * W.K_stride + key;
-  const uint32_t e = atomicAdd(W.ap_top, 1u);
-  if (e >= W.ap_cap) {
-    atomicExch(&B->error, ERR_CAPACITY);
-    return;
+  const int i = atomicAdd(&W.arr_n[L], 1);
+  int64_t e;
+  if (i < A_MAX) {
+    e = L * A_MAX + i;
+  } else {
+    const uint32_t o = atomicAdd(W.ap_top, 1u);
+    if (o >= W.ap_cap) {
+      atomicExch(&B->error, ERR_CAPACITY);
+      return;
+    }
+    e = W.ap_ovf + o;
   }
   W.ap_p[e] = (int)p;
   W.ap_t[e] = t;
   W.ap_dep[e] = UNCLAIMED;
   W.arr_of[p] = (int)e;
-  atomicAdd(&W.arr_n[L], 1);
   atomicAdd((unsigned long long *)&B->n_arrivals, 1ull);
+  if (i < A_MAX) return;
   int old = __hip_atomic_load(&W.arr_head[L], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (;;) {
     W.ap_next[e] = old;
@@ -743,6 +762,16 @@ __device__ inline void record_arrival(const PW &W, BoxDesc *B, int box, int key,
     if (prev == old) break;
     old = prev;
   }
+}
+
+// visit every arrival entry of list L: the array, then the overflow chain
+template <class F>
+__device__ inline void arrivals_each(const PW &W, int64_t L, F f) {
+  const int n = W.arr_n[L];
+  const int na = n < A_MAX ? n : A_MAX;
+  for (int i = 0; i < na; ++i) f(L * A_MAX + i);
+  if (n > A_MAX)
+    for (int e = W.arr_head[L]; e >= 0; e = W.ap_next[e]) f((int64_t)e);
 }
 
 // the particle in slot p leaves the list it is in, at sequence index t
@@ -755,7 +784,8 @@ __device__ inline void slot_leave(const PW &W, int64_t p, uint32_t t) {
     return;
   }
   const int e = W.arr_of[p];  // an arrival of list cur
-  if (e >= 0 && W.ap_p[e] == (int)p && W.ap_dep[e] == UNCLAIMED) W.ap_dep[e] = t;
+  if (e >= 0 && e < W.ap_ovf + W.ap_cap && W.ap_p[e] == (int)p && W.ap_dep[e] == UNCLAIMED)
+    W.ap_dep[e] = t;
 }
 
 // slot p receives a particle of list new_key in place of the one it held
@@ -779,35 +809,75 @@ __device__ inline void slot_append(const PW &W, BoxDesc *B, int box, int64_t p, 
 
 constexpr int DRAW_THIN = -2, DRAW_ATHIN = -3, DRAW_AGAIN = -4;
 
-// The list's first acap arrivals of the step, in order of (sequence
-// index, slot), take its reserved entries: entry j of a pair that begins
-// at tau is the j-th arrival if that came before tau.  Its pool entry, or -1.
-__device__ inline int arrival_at(const PW &W, int64_t L, int j, uint32_t tau) {
-  for (int e = W.arr_head[L]; e >= 0; e = W.ap_next[e]) {
+// 32-bit mixing (the finalizer of MurmurHash3)
+__device__ inline uint32_t mix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+// A draw on reserved entry j of list L for a pair that begins at tau.  The
+// entry holds an arrival when the list had at least j arrivals before tau;
+// which one is drawn uniformly among them (not `exclude`, the first draw of
+// a test on the same list) as the one whose slot hashes lowest under u.  So
+// each arrival present at tau is drawn at the rate of a member.  Returns
+// the pool entry, or -1.
+__device__ inline int64_t arrival_draw(const PW &W, int64_t L, int j, uint32_t tau, uint32_t u,
+                                       int exclude) {
+  int n_before = 0;
+  int64_t best = -1;
+  uint32_t best_h = 0;
+  int best_p = 0;
+  arrivals_each(W, L, [&](int64_t e) {
+    if (W.ap_t[e] >= tau) return;
+    ++n_before;
+    const int pe = W.ap_p[e];
+    if (pe == exclude) return;
+    const uint32_t h = mix32(u ^ ((uint32_t)pe * 0x9E3779B9u));
+    if (best < 0 || h < best_h || (h == best_h && pe < best_p)) {
+      best = e;
+      best_h = h;
+      best_p = pe;
+    }
+  });
+  return j <= n_before ? best : -1;
+}
+
+// The arrival of rank r (1-based, in (sequence index, slot) order) among
+// list L's arrivals, if it came before tau: the pool entry, or -1.
+__device__ inline int64_t arrival_rank(const PW &W, int64_t L, int r, uint32_t tau) {
+  int64_t found = -1;
+  arrivals_each(W, L, [&](int64_t e) {
+    if (found >= 0) return;
     const uint32_t te = W.ap_t[e];
     const int pe = W.ap_p[e];
     int rank = 0;
-    for (int f = W.arr_head[L]; f >= 0 && rank < j; f = W.ap_next[f]) {
+    arrivals_each(W, L, [&](int64_t f) {
       const uint32_t tf = W.ap_t[f];
       if (tf < te || (tf == te && W.ap_p[f] < pe)) ++rank;
-    }
-    if (rank == j - 1) return te < tau ? e : -1;
-  }
-  return -1;
+    });
+    if (rank == r - 1) found = te < tau ? e : -2;
+  });
+  return found >= 0 ? found : -1;
 }
 
 // entry c (1-based) of list `key` (list index L, n step-start members
 // starting at s) for a pair that begins at tau: the particle, or
 // DRAW_THIN (a member that left before tau), DRAW_ATHIN (no arrival there
-// before tau), DRAW_AGAIN (a member that left after tau: draw again)
+// before tau), DRAW_AGAIN (a member that left after tau: draw again).  A
+// reserved entry takes one more draw from `ts` to pick its arrival.
 __device__ inline int draw_entry(const PW &W, int64_t base, int64_t L, int key, int n, int s,
-                                 int c, uint32_t tau) {
+                                 int c, uint32_t tau, Stream &ts, int exclude) {
   if (c <= n) {
     const int p = W.perm[base + s + c - 1];
     if (drawable(W, p, key)) return p;
     return W.depart_t[p] < tau ? DRAW_THIN : DRAW_AGAIN;
   }
-  const int e = arrival_at(W, L, c - n, tau);
+  const uint32_t u = (uint32_t)(ts.uniform() * 4294967296.0);
+  const int64_t e = arrival_draw(W, L, c - n, tau, u, exclude);
   if (e < 0 || W.ap_dep[e] < tau) return DRAW_ATHIN;
   const int p = W.ap_p[e];
   return drawable(W, p, key) ? p : DRAW_AGAIN;
@@ -863,7 +933,8 @@ struct TestOut {
   int p1, p2;
   bool accept;
   bool warn;
-  int thin;  // 0: drawn; 1: thinned on a member that left before the pair; 2: on an empty arrival entry
+  int thin;  // 0: drawn; 1: thinned on a member that left before the pair; 2 (3): on an empty
+            // arrival entry of the first (second) list
 };
 
 __device__ inline void perset_test(BoxDesc *B, const Arrays &A, const PW &W, int box, int64_t k,
@@ -896,7 +967,7 @@ __device__ inline void perset_test(BoxDesc *B, const Arrays &A, const PW &W, int
   int i1 = 0;
   for (int a = 0; a < MAX_REDRAW; ++a) {
     const int c = rand_int(ts, N1);
-    const int p = draw_entry(W, base, lb + key1, key1, n1, s1, c, tau);
+    const int p = draw_entry(W, base, lb + key1, key1, n1, s1, c, tau, ts, -1);
     if (p >= 0) {
       i1 = c;
       p1 = p;
@@ -916,13 +987,13 @@ __device__ inline void perset_test(BoxDesc *B, const Arrays &A, const PW &W, int
       } else {
         c = rand_int(ts, N2);
       }
-      const int p = draw_entry(W, base, lb + key2, key2, n2, s2, c, tau);
+      const int p = draw_entry(W, base, lb + key2, key2, n2, s2, c, tau, ts, same ? p1 : -1);
       if (p >= 0 && p != p1) {
         p2 = p;
         break;
       }
       if (a == 0 && p < 0 && p != DRAW_AGAIN) {
-        o.thin = p == DRAW_THIN ? 1 : 2;
+        o.thin = p == DRAW_THIN ? 1 : 3;
         return;
       }
     }
@@ -1097,7 +1168,7 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
   W.thin[t - t0] = o.thin ? 1 : 0;
   // counted once the test is final; a test thinned on an arrival entry waits
   // for the tail, where earlier events are settled first
-  W.tns[i] = (o.warn ? ACC_WARN : 0) | (o.thin == 2 ? TNS_ATHIN : 0);
+  W.tns[i] = (o.warn ? ACC_WARN : 0) | (o.thin >= 2 ? TNS_ATHIN : 0) | (o.thin == 3 ? TNS_SIDE2 : 0);
   if (o.accept) {
     W.acc[i] = 1;
     Stream ms;
@@ -1516,7 +1587,7 @@ __device__ __noinline__ int acc_visit(BoxDesc *boxes, const Arrays &A, const PW 
   const int acap_t = W.acap[lb + kt];
   int p;
   if (j < acap_t) {
-    const int e = arrival_at(W, lb + kt, acap_t - j, tau);
+    const int64_t e = arrival_rank(W, lb + kt, acap_t - j, tau);
     if (e < 0 || W.ap_dep[e] < tau) {
       *tns = TNS_ATHIN;  // an earlier event not yet settled may still fill it
       return 0;
@@ -1663,7 +1734,7 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
   W.tns[i] = tns;
   if (r == 1) W.acc[i] = 1;
   // the visit's samples, for the algorithmic traffic of this kernel
-  const int ns = (r == 1) ? W.r_nsamp[rec] : (tns & (TNS_ATHIN - 1));
+  const int ns = (r == 1) ? W.r_nsamp[rec] : (tns & (TNS_SIDE2 - 1));
   if (ns > 0) atomicAdd(W.acc_samp, (unsigned long long)ns);
 }
 
@@ -2371,15 +2442,37 @@ __device__ inline bool consumed_before(const PW &W, int p, uint32_t t, int rid) 
   return p >= 0 && st_get(W, p) != ST_LIVE && W.dead_t[p] < t && W.dead_r[p] == rid;
 }
 
+// A test on an empty arrival entry of list L (bin bL) of a pair that
+// begins at tau is final once no event with a smaller index than tau that
+// could add a particle to L was pending at the start of the round that ran
+// it: every arrival into L before tau had been applied then.  On the
+// pair's second list the bound is the test's own index, since earlier
+// tests of the pair can change the first list's draws (a member they
+// removed is drawn again).  Such tests change no list until an earlier
+// event fills their entry, so they do not hold back others.
 __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t t = W.tid[i];
+  const int64_t NP = (int64_t)n_box * W.P_stride;
+  const int64_t k = test_pair(W, NP, t);
+  const int box = (int)(k / W.P_stride);
+  int c1, c2, b1, b2;
+  decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
+  const int kind = W.pair_acc[k];
+  // the lowest bin this event's results can land in
+  const int b_lo = kind ? (kind == 1 ? b1 : b2) : max(b1, b2);
+  const int64_t mb = (int64_t)box * W.K_stride;
+  const bool athin = (W.tns[i] & TNS_ATHIN) && !W.acc[i];
   int f = 0;
-  if (W.acc[i]) {
+  if (athin) {
+    // an empty entry of the second list was reached after the first
+    // list's draws, which earlier tests of the same pair can change
+    const bool side2 = !kind && (W.tns[i] & TNS_SIDE2);
+    const int b_list = kind ? b_lo : (side2 ? b2 : b1);
+    f = (side2 ? t : W.pair_exc[k]) <= W.minpend[mb + b_list - 1] ? 0 : 1;
+  } else if (W.acc[i]) {
     f = (W.ev_out[W.acc_scan[i]] & O_WIN) ? 0 : 1;
-  } else if (W.trec[i] >= 0 && (W.tns[i] & TNS_ATHIN)) {
-    f = 1;  // an arrival entry: an earlier event not yet settled may fill it
   } else if (W.trec[i] >= 0) {
     // a target event without effect: final unless an earlier winner of the
     // round changed its target; its samples count once final
@@ -2389,8 +2482,7 @@ __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
     f = changed ? 1 : 0;
     int v = W.tns[i];
     if (!f && v) {
-      const int64_t k = test_pair(W, (int64_t)n_box * W.P_stride, t);
-      BoxDesc *B = &boxes[k / W.P_stride];
+      BoxDesc *B = &boxes[box];
       unsigned long long ns = (unsigned long long)(v & (ACC_WARN - 1));
       atomicAdd((unsigned long long *)&B->n_samp, ns);
       atomicAdd((unsigned long long *)&B->step_samp, ns);
@@ -2399,15 +2491,25 @@ __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
     }
   } else if (W.tp1[i] >= 0) {
     f = (consumed_before(W, W.tp1[i], t, rid) || consumed_before(W, W.tp2[i], t, rid)) ? 1 : 0;
-  } else if (W.tns[i] & TNS_ATHIN) {
-    f = 1;  // an arrival before the pair may still come from an event not yet settled
   }
   if (!f && W.trec[i] < 0 && (W.tns[i] & ACC_WARN)) {
     // a per-set test's k_max warning (src/coagulation.F90:749-751), once final
-    const int64_t k = test_pair(W, (int64_t)n_box * W.P_stride, t);
-    atomicAdd((unsigned long long *)&boxes[k / W.P_stride].n_warn, 1ull);
+    atomicAdd((unsigned long long *)&boxes[box].n_warn, 1ull);
   }
+  if (f && !athin) atomicMin(&W.minpend_next[mb + b_lo - 1], t);
   W.f_pend[i] = f;
+}
+
+// the running minimum over each box's bins (one thread per box)
+__global__ void k_minpend_prefix(const BoxDesc *boxes, PW W, int n_box) {
+  const int box = blockIdx.x * blockDim.x + threadIdx.x;
+  if (box >= n_box) return;
+  uint32_t *m = W.minpend_next + (int64_t)box * W.K_stride;
+  uint32_t run = UNCLAIMED;
+  for (int b = 0; b < W.K_stride; ++b) {
+    run = min(run, m[b]);
+    m[b] = run;
+  }
 }
 
 __global__ void k_par_gather_pending(PW W, int n) {
@@ -2427,7 +2529,7 @@ __device__ inline int lower_bound_u32(const uint32_t *v, int n, uint32_t x) {
 }
 
 constexpr int TAIL_THREADS = 512;
-constexpr int TF_EFFECT = 1, TF_ACC = 2, TF_RUN = 4, TF_WARN = 8, TF_ATHIN = 16;
+constexpr int TF_EFFECT = 1, TF_ACC = 2, TF_RUN = 4, TF_WARN = 8, TF_ATHIN = 16, TF_SIDE2 = 32;
 
 // claims inside one kernel are read and released with device-scope atomics,
 // so they need no cache maintenance between the phases of a round
@@ -2494,6 +2596,9 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
   __shared__ long long s_hole_base;
   __shared__ int s_err;
   extern __shared__ int s_live[];  // live-count changes of the box's lists
+  // then, per bin, the lowest index of an unsettled test of the window whose
+  // results land in that bin or a lower one (a running minimum)
+  uint32_t *s_pm = (uint32_t *)(s_live + W.K_stride);
   for (int q = tid; q < W.K_stride; q += TAIL_THREADS) s_live[q] = 0;
   int64_t c_samp = 0, c_coag = 0, c_warn = 0;  // this lane's counter changes
   int wn = 0, head = lo;
@@ -2534,7 +2639,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     const int rec = is_acc ? (int)(W.acc_exc[k] + (t - W.pair_exc[k]) - acc_base) : -1;
     // 1. run the tests whose inputs are new or changed
     if (active && (fl & TF_RUN)) {
-      fl &= ~(TF_RUN | TF_EFFECT | TF_WARN | TF_ATHIN);
+      fl &= ~(TF_RUN | TF_EFFECT | TF_WARN | TF_ATHIN | TF_SIDE2);
       if (is_acc) {
         const int r = acc_visit(boxes, A, W, k, t, rec, type, seed, step, del_t, false, &p1, &tns);
         if (r == 1) fl |= TF_EFFECT;
@@ -2548,7 +2653,8 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
         W.thin[t - batch_t0] = o.thin ? 1 : 0;
         if (o.accept) fl |= TF_EFFECT;
         if (o.warn) fl |= TF_WARN;
-        if (o.thin == 2) fl |= TF_ATHIN;
+        if (o.thin >= 2) fl |= TF_ATHIN;
+        if (o.thin == 3) fl |= TF_SIDE2;
       }
     }
     const bool effect = fl & TF_EFFECT;
@@ -2681,15 +2787,40 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
       }
     }
     if (win && !is_acc && (fl & TF_WARN)) c_warn += 1;
-    // A test thinned on an empty arrival entry is final once no earlier
-    // test of the window won this round (its arrivals show from the next
-    // round on) or waits for another round; until then it runs again.
-    int unsettled = (win || keep) ? 1 : 0, n_before;
-    BS(scan_tmp).ExclusiveSum(unsettled, n_before);
+    // A test thinned on an empty arrival entry of list L (bin bL) is final
+    // once no earlier test of the window that won this round (its arrivals
+    // show from the next round on) or waits for another round could add a
+    // particle to L: particles only grow, so a test on bins b1, b2 adds
+    // them to bins from max(b1, b2) up, and a target event to bins from its
+    // target's.  On the pair's second list the bound is the test's own
+    // index (earlier tests of the pair can change the first list's draws).
+    // Until then it runs again.
+    for (int q = tid; q < W.K_stride; q += TAIL_THREADS) s_pm[q] = UNCLAIMED;
     __syncthreads();
-    if (athin && n_before > 0) {
-      keep = 1;
-      fl |= TF_RUN;
+    int b1 = 1, b2 = 1, kind = 0;
+    if (active) {
+      int c1, c2;
+      decode_pair(k - pb, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
+      kind = W.pair_acc[k];
+    }
+    const int b_lo = kind ? (kind == 1 ? b1 : b2) : max(b1, b2);
+    if (active && (win || keep)) atomicMin(&s_pm[b_lo - 1], t);
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t run = UNCLAIMED;
+      for (int q = 0; q < B->n_bin; ++q) {
+        run = min(run, s_pm[q]);
+        s_pm[q] = run;
+      }
+    }
+    __syncthreads();
+    if (athin) {
+      const bool side2 = !kind && (fl & TF_SIDE2);
+      const int b_list = kind ? b_lo : (side2 ? b2 : b1);
+      if ((side2 ? t : W.pair_exc[k]) > s_pm[b_list - 1]) {
+        keep = 1;
+        fl |= TF_RUN;
+      }
     }
     int r_keep;
     BS(scan_tmp).ExclusiveSum(keep, r_keep, tot);
@@ -2929,7 +3060,7 @@ inline void parallel_free_arrays(ParallelWork &W) {
                   d.pair_acc, d.pair_accn, d.acc_exc, d.trec, d.tns, d.ev_rec, d.misc,
                   d.hole_top, d.tail_rounds, d.pend_k, d.tp_in, d.tpair,
                   d.acap, d.arr_n, d.arr_head, d.ap_p, d.ap_t, d.ap_dep, d.ap_next, d.ap_top,
-                  d.arr_of, d.key0, d.depart_t, d.thin};
+                  d.arr_of, d.key0, d.depart_t, d.thin, d.minpend, d.minpend_next};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   parallel_free_records(W);
@@ -3045,12 +3176,19 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.tpair, T));
   PCHECK(pmalloc(&d.tns, T));
   PCHECK(pmalloc(&d.thin, T));
+  PCHECK(pmalloc(&d.minpend, (size_t)n_box * d.K_stride));
+  PCHECK(pmalloc(&d.minpend_next, (size_t)n_box * d.K_stride));
   // arrivals of one step: at most one per merge or copy, over every batch
-  d.ap_cap = (uint32_t)std::min<int64_t>(2 * T + DUP_CAP, (int64_t)UINT32_MAX - 1);
-  PCHECK(pmalloc(&d.ap_p, d.ap_cap));
-  PCHECK(pmalloc(&d.ap_t, d.ap_cap));
-  PCHECK(pmalloc(&d.ap_dep, d.ap_cap));
-  PCHECK(pmalloc(&d.ap_next, d.ap_cap));
+  d.ap_cap = (uint32_t)std::min<int64_t>(2 * T + DUP_CAP, (int64_t)INT32_MAX / 2);
+  d.ap_ovf = (int64_t)n_box * d.K_stride * A_MAX;
+  if (d.ap_ovf + d.ap_cap > (int64_t)INT32_MAX) {
+    g_parallel_error = "arrival pool larger than 2^31 entries";
+    return ERR_CAPACITY;
+  }
+  PCHECK(pmalloc(&d.ap_p, d.ap_ovf + d.ap_cap));
+  PCHECK(pmalloc(&d.ap_t, d.ap_ovf + d.ap_cap));
+  PCHECK(pmalloc(&d.ap_dep, d.ap_ovf + d.ap_cap));
+  PCHECK(pmalloc(&d.ap_next, d.ap_ovf + d.ap_cap));
   PCHECK(pmalloc(&d.ap_top, 1));
   PCHECK(pmalloc(&d.ev_rec, T));
   int **fl[] = {&d.f_app, &d.f_id, &d.f_info, &d.f_hole, &d.f_lose, &d.f_pend,
@@ -3218,6 +3356,9 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
     if (n_batch > 0)
       hipLaunchKernelGGL(k_fill_u8, dim3((n_batch + 255) / 256), dim3(256), 0, st, d.thin,
                          (int64_t)n_batch, (uint8_t)0);
+    // at the start of the first round every event of the batch is pending
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)(((int64_t)n_box * d.K_stride + 255) / 256)),
+                       dim3(256), 0, st, d.minpend, (int64_t)n_box * d.K_stride, 0u);
     for (int round = 0; n > 0; ++round) {
       // more device-wide rounds only while they settle most of the batch:
       // when over half of it is still pending, chains dominate and the tail
@@ -3233,7 +3374,7 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, d.tail_rounds, 0ull);
         hipLaunchKernelGGL(k_pend_pairs, dim3((n + 255) / 256), dim3(256), 0, st, d, W.NP, pend, n);
         PLAUNCH(W, "k_par_tail", k_par_tail, dim3(n_box), dim3(TAIL_THREADS),
-                d.K_stride * sizeof(int), st, d_box, A, d,
+                2 * d.K_stride * sizeof(int), st, d_box, A, d,
                 n_box, n, pend, type, seed, step, precision, band, del_t, a0,
                 want_terms, (int)W.rid + 1, d.tail_rounds, batch_t0);
         PCHECK(hipMemcpyAsync(W.h_u64, d.hole_top, sizeof(unsigned long long),
@@ -3292,8 +3433,13 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
               n_box, E, seed, step, type, want_terms, n_holes, (int)W.rid);
       hipLaunchKernelGGL(k_par_box_totals, gb, bb, 0, st, d_box, d, n_box, E);
       hipLaunchKernelGGL(k_par_release, dim3((E + 255) / 256), dim3(256), 0, st, d, E);
+      hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)(((int64_t)n_box * d.K_stride + 255) / 256)),
+                         dim3(256), 0, st, d.minpend_next, (int64_t)n_box * d.K_stride, UNCLAIMED);
       hipLaunchKernelGGL(k_par_pending, dim3((n + 255) / 256), dim3(256), 0, st, d_box, d, n_box, n,
                          (int)W.rid);
+      hipLaunchKernelGGL(k_minpend_prefix, dim3((n_box + 63) / 64), dim3(64), 0, st, d_box, d,
+                         n_box);
+      std::swap(d.minpend, d.minpend_next);
       hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.f_pend + n, 1, 0);
       PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.f_pend, d.s_pend, n + 1, st));
       hipLaunchKernelGGL(k_par_gather_pending, dim3((n + 255) / 256), dim3(256), 0, st, d, n);
