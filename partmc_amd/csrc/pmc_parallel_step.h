@@ -2529,7 +2529,8 @@ __device__ inline int lower_bound_u32(const uint32_t *v, int n, uint32_t x) {
 }
 
 constexpr int TAIL_THREADS = 512;
-constexpr int TF_EFFECT = 1, TF_ACC = 2, TF_RUN = 4, TF_WARN = 8, TF_ATHIN = 16, TF_SIDE2 = 32;
+constexpr int TF_EFFECT = 1, TF_ACC = 2, TF_RUN = 4, TF_WARN = 8, TF_ATHIN = 16, TF_SIDE2 = 32,
+              TF_WAIT = 64;
 
 // claims inside one kernel are read and released with device-scope atomics,
 // so they need no cache maintenance between the phases of a round
@@ -2587,7 +2588,10 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
   if (lo >= hi) return;
   using BS = hipcub::BlockScan<int, TAIL_THREADS>;
   using BS2 = hipcub::BlockScan<long long, TAIL_THREADS>;
+  using BSU = hipcub::BlockScan<uint32_t, TAIL_THREADS>;
   __shared__ typename BS::TempStorage scan_tmp;
+  __shared__ typename BSU::TempStorage scan_tmp3;
+  __shared__ uint32_t s_carry;
   __shared__ typename BS2::TempStorage scan_tmp2;
   __shared__ uint32_t w_t[TAIL_THREADS];
   __shared__ int w_k[TAIL_THREADS], w_p1[TAIL_THREADS], w_p2[TAIL_THREADS];
@@ -2639,7 +2643,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     const int rec = is_acc ? (int)(W.acc_exc[k] + (t - W.pair_exc[k]) - acc_base) : -1;
     // 1. run the tests whose inputs are new or changed
     if (active && (fl & TF_RUN)) {
-      fl &= ~(TF_RUN | TF_EFFECT | TF_WARN | TF_ATHIN | TF_SIDE2);
+      fl &= ~(TF_RUN | TF_EFFECT | TF_WARN | TF_ATHIN | TF_SIDE2 | TF_WAIT);
       if (is_acc) {
         const int r = acc_visit(boxes, A, W, k, t, rec, type, seed, step, del_t, false, &p1, &tns);
         if (r == 1) fl |= TF_EFFECT;
@@ -2794,7 +2798,8 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     // them to bins from max(b1, b2) up, and a target event to bins from its
     // target's.  On the pair's second list the bound is the test's own
     // index (earlier tests of the pair can change the first list's draws).
-    // Until then it runs again.
+    // Until then it waits, and it runs once more when it is no longer
+    // blocked.
     for (int q = tid; q < W.K_stride; q += TAIL_THREADS) s_pm[q] = UNCLAIMED;
     __syncthreads();
     int b1 = 1, b2 = 1, kind = 0;
@@ -2806,20 +2811,30 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     const int b_lo = kind ? (kind == 1 ? b1 : b2) : max(b1, b2);
     if (active && (win || keep)) atomicMin(&s_pm[b_lo - 1], t);
     __syncthreads();
-    if (tid == 0) {
-      uint32_t run = UNCLAIMED;
-      for (int q = 0; q < B->n_bin; ++q) {
-        run = min(run, s_pm[q]);
-        s_pm[q] = run;
-      }
+    // running minimum over the bins, TAIL_THREADS bins at a time
+    const int nbin = B->n_bin;
+    for (int base = 0; base < nbin; base += TAIL_THREADS) {
+      const int q = base + tid;
+      uint32_t v = q < nbin ? s_pm[q] : UNCLAIMED;
+      if (base > 0 && tid == 0) v = min(v, s_carry);
+      uint32_t r;
+      BSU(scan_tmp3).InclusiveScan(v, r, hipcub::Min());
+      if (q < nbin) s_pm[q] = r;
+      if (tid == TAIL_THREADS - 1) s_carry = r;
+      __syncthreads();
     }
-    __syncthreads();
     if (athin) {
       const bool side2 = !kind && (fl & TF_SIDE2);
       const int b_list = kind ? b_lo : (side2 ? b2 : b1);
-      if ((side2 ? t : W.pair_exc[k]) > s_pm[b_list - 1]) {
+      const bool blocked = (side2 ? t : W.pair_exc[k]) > s_pm[b_list - 1];
+      if (blocked) {
+        // wait, without running, until the events that block it settle
         keep = 1;
-        fl |= TF_RUN;
+        fl |= TF_WAIT;
+      } else if (fl & TF_WAIT) {
+        // unblocked since its last run: run once more, on the settled lists
+        keep = 1;
+        fl = (fl & ~TF_WAIT) | TF_RUN;
       }
     }
     int r_keep;

@@ -3,10 +3,12 @@
 The exact mode reproduces the oracle's sequential chain bit for bit
 (tests/test_gpu_exact.py), so its ensemble stands for the reference's.  Over
 100 independent runs of each mode (their own seeds), the mean particle
-count must agree within its 95% confidence interval, and the mean number
+count must agree within its 99% confidence interval, and the mean number
 and volume size distributions within theirs: the per-bin differences of the
 means, in units of their standard errors, must pass a joint chi-square
-test at the 95% level.
+test at the 99% level.  The file makes 15 such comparisons; at 95% each, a
+correct parallel mode would fail one of them in about half of the runs
+whenever a change reshuffles its chain, at 99% in about one run in seven.
 
 Cases: a C5-type sedimentation box (flat weights, cloud and drizzle drops)
 at its own concentration and at 30 times it, where one drizzle drop
@@ -65,14 +67,14 @@ def assert_agree(name, par, seq):
     d, se = z_scores(cp, cs)
     print(f"{name}: particles {cp.mean():.1f} vs {cs.mean():.1f} (diff {d:.1f}, se {se:.1f})",
           flush=True)
-    assert abs(d) <= stats.norm.ppf(0.975) * se, (name, "count", d, se)
+    assert abs(d) <= stats.norm.ppf(0.995) * se, (name, "count", d, se)
     for label, a, b in (("number", np_, ns), ("volume", vp, vs)):
         d, se = z_scores(a, b)
         use = se > 0
         z = d[use] / se[use]
         chi2 = float(np.sum(z ** 2))
-        lim = stats.chi2.ppf(0.95, use.sum())
-        print(f"  {label}: {use.sum()} bins, chi2 {chi2:.1f} (95% limit {lim:.1f}), "
+        lim = stats.chi2.ppf(0.99, use.sum())
+        print(f"  {label}: {use.sum()} bins, chi2 {chi2:.1f} (99% limit {lim:.1f}), "
               f"max |z| {np.abs(z).max():.2f}", flush=True)
         assert chi2 <= lim, (name, label, chi2, lim, z)
 
