@@ -68,11 +68,11 @@ def kernel_bytes(name, n_spec, precision, part_steps_per_launch, tests_per_launc
 
 
 # What limits each kernel: the bandwidth roofline bounds the streaming
-# passes; the walks and the tail are chains of dependent loads and FP64
-# special functions, one walk or chain per lane.
+# passes; the target events and the tail are chains of dependent loads and
+# FP64 special functions, one event or chain per lane.
 LIMITER = {
-    "k_par_acc_tests": "latency: each target walk is one lane's chain of dependent loads "
-                       "and FP64 special functions",
+    "k_par_acc_tests": "latency: each target event is one lane's sample loop of dependent "
+                       "loads and FP64 special functions",
     "k_par_tail": "latency: one workgroup per box runs the remaining claim rounds, one link "
                   "of a chain per round",
     "k_exact_sweep": "latency: one wavefront per box follows the sequential chain",
@@ -119,6 +119,7 @@ def run_gpu(pops, kernel, dt, mode, precision, steps, warmup, seed=20261202, reb
     elapsed = time.perf_counter() - t0
     c = g.counters()
     st = g.path_stats()
+    ar = g.arrival_stats()
     out = {
         "claim_rounds": st["n_rounds"],
         "tail_tests": st["n_tail"],
@@ -126,6 +127,8 @@ def run_gpu(pops, kernel, dt, mode, precision, steps, warmup, seed=20261202, reb
         "accel_pairs": int(st["n_accel_pairs"].sum()),
         "acc_samples": st["acc_samples"],
         "n_dropped": int(st["n_dropped"].sum()),
+        "arrivals": int(ar["n_arrivals"].sum()),
+        "arrival_overflow": int(ar["n_overflow"].sum()),
         "elapsed": elapsed,
         "n_samp": int(c["n_samp"].sum()),
         "n_coag": int(c["n_coag"].sum()),
@@ -405,6 +408,8 @@ def main():
         "acc_samples_per_step": r["acc_samples"] / args.steps,
         "fp64_rechecks": r["n_recheck"],
         "dropped_draws": r["n_dropped"],
+        "arrivals_per_step": r["arrivals"] / args.steps,
+        "arrival_overflow_per_step": r["arrival_overflow"] / args.steps,
         "precision_check": None if rv is None else {
             "decision_flips": rv["n_flip"],
             "max_rel_kernel_err": rv["max_rel_err"],

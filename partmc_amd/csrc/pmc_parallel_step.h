@@ -2496,19 +2496,33 @@ __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
     // a per-set test's k_max warning (src/coagulation.F90:749-751), once final
     atomicAdd((unsigned long long *)&boxes[box].n_warn, 1ull);
   }
-  if (f && !athin) atomicMin(&W.minpend_next[mb + b_lo - 1], t);
+  // values only fall: most tests see a smaller one already and skip the atomic
+  if (f && !athin && W.minpend_next[mb + b_lo - 1] > t) atomicMin(&W.minpend_next[mb + b_lo - 1], t);
   W.f_pend[i] = f;
 }
 
-// the running minimum over each box's bins (one thread per box)
-__global__ void k_minpend_prefix(const BoxDesc *boxes, PW W, int n_box) {
-  const int box = blockIdx.x * blockDim.x + threadIdx.x;
-  if (box >= n_box) return;
-  uint32_t *m = W.minpend_next + (int64_t)box * W.K_stride;
-  uint32_t run = UNCLAIMED;
-  for (int b = 0; b < W.K_stride; ++b) {
-    run = min(run, m[b]);
-    m[b] = run;
+// the running minimum over each box's bins of the next round's minima
+// (one workgroup per box), and this round's cleared for the round after
+constexpr int PREFIX_THREADS = 256;
+__global__ void __launch_bounds__(PREFIX_THREADS) k_minpend_prefix(PW W) {
+  using BSU = hipcub::BlockScan<uint32_t, PREFIX_THREADS>;
+  __shared__ typename BSU::TempStorage tmp;
+  __shared__ uint32_t carry;
+  const int tid = threadIdx.x;
+  uint32_t *m = W.minpend_next + (int64_t)blockIdx.x * W.K_stride;
+  uint32_t *c = W.minpend + (int64_t)blockIdx.x * W.K_stride;
+  for (int base = 0; base < W.K_stride; base += PREFIX_THREADS) {
+    const int q = base + tid;
+    uint32_t v = q < W.K_stride ? m[q] : UNCLAIMED;
+    if (base > 0 && tid == 0) v = min(v, carry);
+    uint32_t r;
+    BSU(tmp).InclusiveScan(v, r, hipcub::Min());
+    if (q < W.K_stride) {
+      m[q] = r;
+      c[q] = UNCLAIMED;
+    }
+    if (tid == PREFIX_THREADS - 1) carry = r;
+    __syncthreads();
   }
 }
 
@@ -3374,6 +3388,8 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
     // at the start of the first round every event of the batch is pending
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)(((int64_t)n_box * d.K_stride + 255) / 256)),
                        dim3(256), 0, st, d.minpend, (int64_t)n_box * d.K_stride, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)(((int64_t)n_box * d.K_stride + 255) / 256)),
+                       dim3(256), 0, st, d.minpend_next, (int64_t)n_box * d.K_stride, UNCLAIMED);
     for (int round = 0; n > 0; ++round) {
       // more device-wide rounds only while they settle most of the batch:
       // when over half of it is still pending, chains dominate and the tail
@@ -3448,12 +3464,9 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
               n_box, E, seed, step, type, want_terms, n_holes, (int)W.rid);
       hipLaunchKernelGGL(k_par_box_totals, gb, bb, 0, st, d_box, d, n_box, E);
       hipLaunchKernelGGL(k_par_release, dim3((E + 255) / 256), dim3(256), 0, st, d, E);
-      hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)(((int64_t)n_box * d.K_stride + 255) / 256)),
-                         dim3(256), 0, st, d.minpend_next, (int64_t)n_box * d.K_stride, UNCLAIMED);
       hipLaunchKernelGGL(k_par_pending, dim3((n + 255) / 256), dim3(256), 0, st, d_box, d, n_box, n,
                          (int)W.rid);
-      hipLaunchKernelGGL(k_minpend_prefix, dim3((n_box + 63) / 64), dim3(64), 0, st, d_box, d,
-                         n_box);
+      hipLaunchKernelGGL(k_minpend_prefix, dim3(n_box), dim3(PREFIX_THREADS), 0, st, d);
       std::swap(d.minpend, d.minpend_next);
       hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.f_pend + n, 1, 0);
       PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.f_pend, d.s_pend, n + 1, st));
