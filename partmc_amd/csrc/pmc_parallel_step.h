@@ -2070,7 +2070,7 @@ __device__ inline int direct_merge(const Arrays &A, const PW &W, BoxDesc *B, int
   const double least = fmin(A.least[p1], A.least[p2]);
   const double greatest = fmax(A.greatest[p1], A.greatest[p2]);
   // dst may be a parent's slot: each species' sum depends only on its own
-  // column, so it is written as it is formed, with no private staging
+  // column, so it is written as it is formed
   double v = 0.0, m = 0.0;
   for (int s = 0; s < A.n_spec; ++s) {
     const double x = A.vol[(int64_t)s * tc + p1] + A.vol[(int64_t)s * tc + p2];
@@ -2078,30 +2078,26 @@ __device__ inline int direct_merge(const Arrays &A, const PW &W, BoxDesc *B, int
     v += x;
     m += x * A.density[s];
   }
-  // components: the first parent's, then the second's.  In the first
-  // parent's slot only the second's are appended; in the second's, its own
-  // move up by n1 from the top down (each is read before it is overwritten)
-  // and the first's go below them.
+  // components: the first parent's, then the second's, all loaded before
+  // any is written (dst may be a parent's slot), so the loads go out
+  // together
   int n_out = n1 + n2;
   if (n1 + n2 > MAX_COMP) {
     resample_components(A, *rs, p1, p2, dst, n1, n2, np1, np2);
     n_out = MAX_COMP;
   } else {
-    if (dst == p2) {
-      for (int c = n2 - 1; c >= 0; --c) {
-        A.csrc[(int64_t)(n1 + c) * tc + dst] = A.csrc[(int64_t)c * tc + p2];
-        A.ctime[(int64_t)(n1 + c) * tc + dst] = A.ctime[(int64_t)c * tc + p2];
+    int s1[MAX_COMP], s2[MAX_COMP];
+    double t1[MAX_COMP], t2[MAX_COMP];
+    load_comps2(A, p1, p2, n1, n2, s1, t1, s2, t2);
+#pragma unroll
+    for (int c = 0; c < MAX_COMP; ++c) {
+      if (c < n1) {
+        A.csrc[(int64_t)c * tc + dst] = s1[c];
+        A.ctime[(int64_t)c * tc + dst] = t1[c];
       }
-    } else {
-      for (int c = 0; c < n2; ++c) {
-        A.csrc[(int64_t)(n1 + c) * tc + dst] = A.csrc[(int64_t)c * tc + p2];
-        A.ctime[(int64_t)(n1 + c) * tc + dst] = A.ctime[(int64_t)c * tc + p2];
-      }
-    }
-    if (dst != p1) {
-      for (int c = 0; c < n1; ++c) {
-        A.csrc[(int64_t)c * tc + dst] = A.csrc[(int64_t)c * tc + p1];
-        A.ctime[(int64_t)c * tc + dst] = A.ctime[(int64_t)c * tc + p1];
+      if (c < n2) {
+        A.csrc[(int64_t)(n1 + c) * tc + dst] = s2[c];
+        A.ctime[(int64_t)(n1 + c) * tc + dst] = t2[c];
       }
     }
   }
