@@ -174,6 +174,11 @@ int pmc_gpu_trace_off(pmc_gpu *h, int box);
  * not see them).  No reference counterpart: the reference's lists are
  * sequential (src/aero_sorted.F90:397-494). */
 int pmc_gpu_get_arrival_stats(pmc_gpu *h, int64_t *n_arrivals, int64_t *n_overflow);
+/* The parallel mode's list sizes of one box at the start of its last step:
+ * counts[(c - 1) * n_bin + (b - 1)] particles in bin b of weight class c,
+ * the integer_varray_n_entry of aero_sorted's size_class lists
+ * (src/aero_sorted.F90:40-56) after the sort of that step. */
+int pmc_gpu_get_step_lists(pmc_gpu *h, int box, int *counts);
 /* Per box, since the last reset: the samples and coagulations that the
  * per-particle path (try_per_particle_coag, src/coagulation.F90:150-241)
  * added to tot_n_samp and tot_n_coag (:234-235), in either mode. */

@@ -43,6 +43,7 @@ ABI = {
     "pmc_gpu_get_path_stats": (_i, [_p, _p, _p, _p, _p]),
     "pmc_gpu_get_arrival_stats": (_i, [_p, _p, _p]),
     "pmc_gpu_get_accel_counts": (_i, [_p, _p, _p]),
+    "pmc_gpu_get_step_lists": (_i, [_p, _i, _p]),
     "pmc_gpu_record": (_i, [_p, _i, _i64, _i]),
     "pmc_gpu_replay": (_i, [_p, _i, _p, _i64, _i]),
     "pmc_gpu_get_trace": (_i, [_p, _i, _p, _i64, _p, _p]),
@@ -390,6 +391,15 @@ class CoagGPU:
         over = np.zeros(self.n_box, dtype=np.int64)
         _check(load().pmc_gpu_get_arrival_stats(self._h, _ptr(arr), _ptr(over)))
         return {"n_arrivals": arr, "n_overflow": over}
+
+    def step_lists(self, box, n_class=1):
+        """The parallel mode's list sizes at the start of its last step,
+        (n_class, n_bin)."""
+        e, _, _ = self.grid(box)
+        nb = len(e) - 1
+        counts = np.zeros(max(nb * n_class, 1), dtype=np.int32)
+        _check(load().pmc_gpu_get_step_lists(self._h, box, _ptr(counts)))
+        return counts[: nb * n_class].reshape(n_class, nb)
 
     def accel_counts(self):
         """Per box: samples and coagulations of the per-particle path."""

@@ -977,6 +977,25 @@ int pmc_gpu_get_trace(pmc_gpu *h, int box, double *u, int64_t max_n, int64_t *n_
   return 0;
 }
 
+int pmc_gpu_get_step_lists(pmc_gpu *h, int box, int *counts) {
+  int e = check_box(h, box);
+  if (e) return e;
+  if (!counts) return fail(ERR_ARG, "get_step_lists: null counts");
+  HIPCHECK(hipSetDevice(h->device));
+  HIPCHECK(hipStreamSynchronize(h->stream));
+  const PW &d = h->work.d;
+  if (d.lcnt == nullptr) return fail(ERR_STATE, "get_step_lists: no parallel-mode step has run");
+  BoxDesc hb;
+  HIPCHECK(hipMemcpy(&hb, h->d_box + box, sizeof(BoxDesc), hipMemcpyDeviceToHost));
+  const int K = hb.n_bin * hb.w.n_class;
+  if (K <= 0) return 0;
+  if (K > d.K_stride) return fail(ERR_STATE, "get_step_lists: lists not laid out for this grid");
+  // list (bin b, class c) sits at (b - 1) + n_bin * (c - 1), as in counts
+  HIPCHECK(hipMemcpy(counts, d.lcnt + (size_t)box * d.K_stride, K * sizeof(int),
+                     hipMemcpyDeviceToHost));
+  return 0;
+}
+
 int pmc_gpu_get_arrival_stats(pmc_gpu *h, int64_t *n_arrivals, int64_t *n_overflow) {
   if (!h || !n_arrivals || !n_overflow) return fail(ERR_ARG, "get_arrival_stats: bad arguments");
   HIPCHECK(hipSetDevice(h->device));

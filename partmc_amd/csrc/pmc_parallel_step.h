@@ -3124,6 +3124,25 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   d.max_bin = max_bin;
   d.K_stride = max_bin * d.tab_classes;
   d.P_stride = (int64_t)d.tab_classes * d.tab_classes * max_bin * max_bin;
+  // the sort and the tail keep two ints per list of a box in LDS beside
+  // their static LDS: refuse grids and class counts that do not fit
+  {
+    int dev = 0, lds_max = 0;
+    PCHECK(hipGetDevice(&dev));
+    PCHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    hipFuncAttributes fa_tail, fa_scatter;
+    PCHECK(hipFuncGetAttributes(&fa_tail, reinterpret_cast<const void *>(&k_par_tail)));
+    PCHECK(hipFuncGetAttributes(&fa_scatter, reinterpret_cast<const void *>(&k_par_scatter)));
+    const size_t dyn = 2 * (size_t)d.K_stride * sizeof(int);
+    if (fa_tail.sharedSizeBytes + dyn > (size_t)lds_max ||
+        fa_scatter.sharedSizeBytes + dyn > (size_t)lds_max) {
+      g_parallel_error = "parallel mode: " + std::to_string(max_bin) + " bins x " +
+                         std::to_string(d.tab_classes) + " weight classes need " +
+                         std::to_string(dyn) + " bytes of LDS per box list set, above the " +
+                         std::to_string(lds_max) + " a workgroup has beside the kernels' own";
+      return ERR_CAPACITY;
+    }
+  }
   const int64_t tc = A.total_cap;
   PCHECK(pmalloc(&d.key, tc));
   PCHECK(pmalloc(&d.status, tc));
