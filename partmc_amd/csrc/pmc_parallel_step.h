@@ -141,7 +141,7 @@ constexpr int RM_CAP = 1 << 24;
 constexpr int ACC_WARN = 1 << 30;
 constexpr int READ_TRACK = 128;  // source lists up to this size have their reads tracked
 constexpr int RF_WARN = 1, RF_DIED = 2, RF_MERGED = 4;
-constexpr int A_MAX = 64;       // arrival entries a list can reserve
+constexpr int A_MAX = 128;      // arrival entries a list can reserve
 constexpr int ARR_REACH = 3;    // an empty list reserves entries when a list at most this many bins below is occupied
 constexpr int TNS_ATHIN = 1 << 29;  // tns bit of a test or target event on an empty arrival entry
 constexpr int TNS_SIDE2 = 1 << 28;  // ... of the pair's second list
@@ -885,6 +885,8 @@ __device__ inline int draw_entry(const PW &W, int64_t base, int64_t L, int key, 
 
 // reserved arrival entries and a fresh arrival log for every list: lists
 // at most ARR_REACH bins above an occupied one (particles only grow)
+// reserve 2 + n/64, and a list that received arrivals in the previous step
+// (on the same grid) at least twice as many as it received then
 __global__ void k_par_acap(BoxDesc *boxes, PW W) {
   const int box = blockIdx.y;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -892,6 +894,7 @@ __global__ void k_par_acap(BoxDesc *boxes, PW W) {
   const BoxDesc *B = &boxes[box];
   const int64_t L = (int64_t)box * W.K_stride + k;
   const int nb = B->n_bin, nc = B->w.n_class;
+  const int prev = (B->regridded || B->rekey) ? 0 : W.arr_n[L];
   int cap = 0;
   if (!B->error && nb > 0 && k < nb * nc) {
     const int b = k % nb + 1;
@@ -899,7 +902,9 @@ __global__ void k_par_acap(BoxDesc *boxes, PW W) {
     bool near = false;
     for (int c = 1; c <= nc && !near; ++c)
       for (int bb = max(1, b - ARR_REACH); bb <= b && !near; ++bb) near = cnt[par_key(bb, c, nb)] > 0;
-    if (near) cap = min(A_MAX, 2 + cnt[k] / 64);
+    if (near) cap = 2 + cnt[k] / 64;
+    if (prev > 0) cap = max(cap, 2 * prev + 2);
+    cap = min(A_MAX, cap);
   }
   W.acap[L] = cap;
   W.arr_n[L] = 0;
@@ -3190,6 +3195,7 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(hipMemset(d.live, 0, sizeof(int) * (size_t)n_box * d.K_stride));
   PCHECK(pmalloc(&d.acap, (size_t)n_box * d.K_stride));
   PCHECK(pmalloc(&d.arr_n, (size_t)n_box * d.K_stride));
+  PCHECK(hipMemset(d.arr_n, 0, sizeof(int) * (size_t)n_box * d.K_stride));
   PCHECK(pmalloc(&d.arr_head, (size_t)n_box * d.K_stride));
   W.NP = (int64_t)n_box * d.P_stride;
   PCHECK(pmalloc(&d.pair_n, W.NP + 1));
