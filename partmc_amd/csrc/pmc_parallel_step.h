@@ -1709,8 +1709,12 @@ __global__ void __launch_bounds__(64)
 k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed,
                 uint32_t step, uint32_t t0, int n, const uint32_t *pend, double del_t,
                 uint32_t acc_base, int n_acc) {
-  if ((int)threadIdx.x >= ACC_LANES) return;
-  const int idx = blockIdx.x * ACC_LANES + threadIdx.x;
+  // over a pending list most slots are per-set tests that return at once:
+  // every lane takes one; over the first round's target slots, ACC_LANES a
+  // wavefront
+  const int lanes = pend ? 64 : ACC_LANES;
+  if ((int)threadIdx.x >= lanes) return;
+  const int idx = blockIdx.x * lanes + threadIdx.x;
   const int64_t NP = (int64_t)n_box * W.P_stride;
   uint32_t t;
   int i;
@@ -3453,7 +3457,8 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
         PCHECK(hipStreamWaitEvent(W.st2, W.ev_fork, 0));
         const int na = pend ? n : n_acc;
         W.tst = W.st2;
-        PLAUNCH(W, "k_par_acc_tests", k_par_acc_tests, dim3((na + ACC_LANES - 1) / ACC_LANES), dim3(64), 0, W.st2,
+        const int lanes = pend ? 64 : ACC_LANES;
+        PLAUNCH(W, "k_par_acc_tests", k_par_acc_tests, dim3((na + lanes - 1) / lanes), dim3(64), 0, W.st2,
                 d_box, A, d, n_box, type, seed, step, batch_t0, n, pend, del_t, a0, n_acc);
         W.tst = nullptr;
         PCHECK(hipEventRecord(W.ev_join, W.st2));
