@@ -18,7 +18,8 @@ LIB = os.path.join(HERE, "libpartmc_coag.so")
 SOURCES = [os.path.join(CSRC, "pmc_coag.hip")]
 DEPS = SOURCES + [os.path.join(CSRC, f) for f in
                   ("pmc_device.h", "pmc_state.h", "pmc_exact.h", "pmc_parallel.h",
-                                   "pmc_parallel_step.h", "pmc_stats.h", "pmc_rebalance.h")] + [
+                                   "pmc_parallel_step.h", "pmc_stats.h", "pmc_rebalance.h",
+                                   "pmc_scan.h")] + [
     os.path.join(HERE, "..", "include", "partmc_coag.h")]
 
 

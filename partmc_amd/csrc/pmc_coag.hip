@@ -873,8 +873,7 @@ int pmc_gpu_reset_counters(pmc_gpu *h) {
     B.n_accel_pairs = 0;
     B.part_steps = 0;
   }
-  h->work.n_rounds = 0;
-  h->work.n_tail = 0;
+  if (h->work.pstat) HIPCHECK(hipMemset(h->work.pstat, 0, 2 * sizeof(unsigned long long)));
   if (h->work.d.acc_samp) HIPCHECK(hipMemset(h->work.d.acc_samp, 0, sizeof(unsigned long long)));
   h->work.n_tail_rounds = 0;
   HIPCHECK(hipMemcpy(h->d_box, hb.data(), sizeof(BoxDesc) * h->n_box, hipMemcpyHostToDevice));
@@ -907,8 +906,11 @@ int pmc_gpu_get_path_stats(pmc_gpu *h, int64_t *n_accel_pairs, int64_t *n_droppe
       box_tail[2 * b + 1] = hb[b].n_rounds;
     }
   }
-  rounds[0] = h->work.n_rounds;
-  rounds[1] = h->work.n_tail;
+  unsigned long long ps[2] = {0, 0};
+  if (h->work.pstat)
+    HIPCHECK(hipMemcpy(ps, h->work.pstat, sizeof(ps), hipMemcpyDeviceToHost));
+  rounds[0] = (int64_t)ps[0];
+  rounds[1] = (int64_t)ps[1];
   rounds[2] = h->work.n_tail_rounds;
   unsigned long long acc_samp = 0;
   if (h->work.d.acc_samp)

@@ -29,7 +29,11 @@
 // ordinal) and every conflict is resolved by sequential index.
 #pragma once
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/block/block_load.hpp>
+#include <rocprim/block/block_radix_sort.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "pmc_scan.h"
 
 #include <string>
 #include <vector>
@@ -98,8 +102,8 @@ struct PW {  // device view of the parallel work arrays
   int *rm_buf;                 // chunks: [0] next chunk or -1, [1..RM_CHUNK) removed slots
   int *rm_top;                 // [1] chunk allocator of the round
   int rm_cap;
-  uint32_t *misc;              // [4] batch bounds from k_batch_end
-  unsigned long long *hole_top;  // [1] hole counter of the tail
+  uint32_t *misc;              // [8] batch bounds from k_batch_end
+  unsigned long long *hole_top;  // [8] hole counter; then tail_rounds and rstat
   int *pend_k;                 // [T_cap] pair of each pending test entering the tail
   uint32_t *tp_in, *tpair;     // [T_cap] pair of every test of a single-batch step
   int tpair_ok;                // tpair is current for this step
@@ -131,7 +135,21 @@ struct PW {  // device view of the parallel work arrays
   // b1, b2 adds particles to bins from max(b1, b2) up), as a running
   // minimum over the bins; and the same, per bin, for the next round.
   uint32_t *minpend, *minpend_next;  // [n_box * K_stride] each
+  uint32_t *scan_part;  // [SCAN_COLS * SCAN_MAX_BLOCKS] chunk totals of the device-wide scans
+  // The counts that drive a batch's claim rounds, kept on the device so the
+  // host launches every round without reading them back: RC_N tests of the
+  // current device-wide round (0: the round does not run), RC_E its events,
+  // RC_TAIL the tests left to the tail.
+  int *rc;                      // [RC_SIZE]
+  unsigned long long *pstat;    // [2] device-wide rounds run, tests left to the tail
+  // the batch's round sizes, read back with the hole count after the tail to
+  // size the next step's grids: round 0's events, round 1's tests and
+  // events, the tail's tests (in the allocation of hole_top, after
+  // tail_rounds)
+  unsigned long long *rstat;    // [4]
 };
+
+constexpr int RC_N = 0, RC_E = 1, RC_TAIL = 2, RC_SIZE = 4;
 
 constexpr int ACC_CAP = 1 << 18;
 constexpr int DUP_CAP = 1 << 18;
@@ -153,8 +171,6 @@ struct ParallelWork {
   int64_t H = 0, NP = 0;
   int n_tiles = 0;
   PW d{};
-  void *scan_tmp = nullptr;
-  size_t scan_tmp_bytes = 0;
   void *sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   std::vector<int> tiles_in_box;
@@ -171,10 +187,12 @@ struct ParallelWork {
   // beside the per-set tests of the same round
   hipStream_t st2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  int64_t n_tail = 0;     // tests finished in the tail
   int64_t n_tail_rounds = 0;  // the tail's claim rounds, summed over boxes
   int64_t rid = 1;        // round identifiers: a fresh one for every claim round
-  int64_t n_rounds = 0;   // claim rounds run
+  unsigned long long *pstat = nullptr;  // [2] device-wide rounds run, tests left to the tail
+  // the last batch's round 0 events, round 1 tests and events, tail tests
+  // (-1: none yet): the grids of the next step's rounds are sized from them
+  int64_t est[4] = {-1, -1, -1, -1};
 };
 
 #define PLAUNCH(W, name, ...)                                   \
@@ -443,11 +461,12 @@ __global__ void k_par_lists(BoxDesc *boxes, PW W, int n_box) {
 // written by the first item of each run.
 __global__ void __launch_bounds__(TILE_THREADS)
 k_par_scatter(BoxDesc *boxes, PW W, int key_bits) {
-  using BLD = hipcub::BlockLoad<int, TILE_THREADS, TILE_ITEMS, hipcub::BLOCK_LOAD_TRANSPOSE>;
-  using BRS = hipcub::BlockRadixSort<int, TILE_THREADS, TILE_ITEMS, int>;
+  using BLD = rocprim::block_load<int, TILE_THREADS, TILE_ITEMS,
+                                  rocprim::block_load_method::block_load_transpose>;
+  using BRS = rocprim::block_radix_sort<int, TILE_THREADS, TILE_ITEMS, int>;
   __shared__ union {
-    typename BLD::TempStorage load;
-    typename BRS::TempStorage sort;
+    typename BLD::storage_type load;
+    typename BRS::storage_type sort;
   } tmp;
   __shared__ int s_keys[TILE];
   extern __shared__ int s_run[];  // [0, K): first sorted position; [K, 2K): list offset
@@ -460,10 +479,13 @@ k_par_scatter(BoxDesc *boxes, PW W, int key_bits) {
   const int start = W.tile_start[tile];
   const int valid = max(0, min(TILE, n - start));
   int keys[TILE_ITEMS], vals[TILE_ITEMS];
-  BLD(tmp.load).Load(W.key + base + start, keys, valid, sentinel);
+  // coalesced loads, exchanged so that each thread holds TILE_ITEMS
+  // consecutive keys
+  BLD().load(W.key + base + start, keys, (unsigned)valid, sentinel, tmp.load);
   for (int it = 0; it < TILE_ITEMS; ++it) vals[it] = start + threadIdx.x * TILE_ITEMS + it;
   __syncthreads();
-  BRS(tmp.sort).Sort(keys, vals, 0, key_bits);
+  BRS().sort(keys, vals, tmp.sort, 0, key_bits);
+  __syncthreads();
   for (int it = 0; it < TILE_ITEMS; ++it) s_keys[threadIdx.x * TILE_ITEMS + it] = keys[it];
   __syncthreads();
   const int K = W.K_stride;
@@ -1146,13 +1168,16 @@ __device__ inline int perset_decide(BoxDesc *B, const Arrays &A, const PW &W, in
   return merge_decide(A, W, B, ms, p1, p2, c1, c2, cc, &g) | (g << 16);
 }
 
-// one thread per test: draw, evaluate, accept, claim
-__global__ void __launch_bounds__(256)
-k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, uint32_t step,
-            uint32_t t0, int n, const uint32_t *pend, int precision, double band,
-            int validation) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// The kernels of a claim round read their item counts from device memory
+// (d.rc), and their grids are sized on the host from a bound or an
+// estimate: every thread strides over the items.
+#define PAR_FOR(i, n) \
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
+
+// test i of the round: draw, evaluate, accept, claim
+__device__ inline void par_test(BoxDesc *boxes, const Arrays &A, const PW &W, int n_box, int type,
+                                uint64_t seed, uint32_t step, uint32_t t0, const uint32_t *pend,
+                                int precision, double band, int validation, int i) {
   const uint32_t t = pend ? pend[i] : t0 + (uint32_t)i;
   const int64_t NP = (int64_t)n_box * W.P_stride;
   const int64_t k = test_pair(W, NP, t);
@@ -1182,6 +1207,22 @@ k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, 
     else atomicMin(&W.claim[o.p1], t);
     if (d & O_RM2) claim_write(W, o.p2, t);
     else atomicMin(&W.claim[o.p2], t);
+  }
+}
+
+// STRIDE: the grid may be smaller than the count (a later round, sized from
+// an estimate).  The first round's grid covers its tests exactly, and
+// without the loop the kernel keeps half the registers.
+template <bool STRIDE>
+__global__ void __launch_bounds__(256)
+k_par_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed, uint32_t step,
+            uint32_t t0, const uint32_t *pend, int precision, double band, int validation) {
+  const int n = W.rc[RC_N];
+  if (STRIDE) {
+    PAR_FOR(i, n) par_test(boxes, A, W, n_box, type, seed, step, t0, pend, precision, band, validation, i);
+  } else {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) par_test(boxes, A, W, n_box, type, seed, step, t0, pend, precision, band, validation, i);
   }
 }
 
@@ -1705,28 +1746,19 @@ __device__ __noinline__ int acc_visit(BoxDesc *boxes, const Arrays &A, const PW 
 constexpr int ACC_LANES = 16;
 // Round 0 runs one thread per target slot of the batch (record a maps to
 // its slot through acc_exc); later rounds run over the pending slots.
-__global__ void __launch_bounds__(64)
-k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed,
-                uint32_t step, uint32_t t0, int n, const uint32_t *pend, double del_t,
-                uint32_t acc_base, int n_acc) {
-  // over a pending list most slots are per-set tests that return at once:
-  // every lane takes one; over the first round's target slots, ACC_LANES a
-  // wavefront
-  const int lanes = pend ? 64 : ACC_LANES;
-  if ((int)threadIdx.x >= lanes) return;
-  const int idx = blockIdx.x * lanes + threadIdx.x;
+__device__ inline void acc_test(BoxDesc *boxes, const Arrays &A, const PW &W, int n_box, int type,
+                                uint64_t seed, uint32_t step, uint32_t t0, const uint32_t *pend,
+                                double del_t, uint32_t acc_base, int idx) {
   const int64_t NP = (int64_t)n_box * W.P_stride;
   uint32_t t;
   int i;
   int64_t k;
   if (pend) {
-    if (idx >= n) return;
     i = idx;
     t = pend[i];
     k = test_pair(W, NP, t);
     if (!W.pair_acc[k]) return;
   } else {
-    if (idx >= n_acc) return;
     const uint32_t a = acc_base + (uint32_t)idx;
     k = find_pair(W.acc_exc, NP, a);
     t = W.pair_exc[k] + (a - W.acc_exc[k]);
@@ -1747,6 +1779,20 @@ k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t se
   if (ns > 0) atomicAdd(W.acc_samp, (unsigned long long)ns);
 }
 
+__global__ void __launch_bounds__(64)
+k_par_acc_tests(BoxDesc *boxes, Arrays A, PW W, int n_box, int type, uint64_t seed,
+                uint32_t step, uint32_t t0, const uint32_t *pend, double del_t,
+                uint32_t acc_base, int n_acc) {
+  // over a pending list most slots are per-set tests that return at once:
+  // every lane takes one; over the first round's target slots, ACC_LANES a
+  // wavefront
+  const int lanes = pend ? 64 : ACC_LANES;
+  if ((int)threadIdx.x >= lanes) return;
+  const int count = pend ? W.rc[RC_N] : n_acc;
+  for (int idx = blockIdx.x * lanes + threadIdx.x; idx < count; idx += gridDim.x * lanes)
+    acc_test(boxes, A, W, n_box, type, seed, step, t0, pend, del_t, acc_base, idx);
+}
+
 // the largest batch end t1 <= min(T, t0 + T_cap) whose target slots fit the
 // records; out[0] = t1, out[1] = acc_count(t0), out[2] = acc_count(t1)
 __device__ inline uint32_t acc_count(const PW &W, int64_t NP, uint32_t T, uint32_t t) {
@@ -1755,9 +1801,10 @@ __device__ inline uint32_t acc_count(const PW &W, int64_t NP, uint32_t T, uint32
   return W.acc_exc[k] + (W.pair_acc[k] ? t - W.pair_exc[k] : 0u);
 }
 
-__global__ void k_batch_end(PW W, int64_t NP, uint32_t T, uint32_t t0, uint32_t t_cap,
-                            uint32_t a_cap, uint32_t *out) {
+__global__ void k_batch_end(PW W, int64_t NP, uint32_t t0, uint32_t t_cap, uint32_t a_cap,
+                            uint32_t *out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const uint32_t T = W.pair_exc[NP];
   const uint32_t a0 = acc_count(W, NP, T, t0);
   uint32_t lo = t0 + 1, hi = (T - t0 < t_cap) ? T : t0 + t_cap;
   if (acc_count(W, NP, T, hi) - a0 <= a_cap) lo = hi;  // the usual case: all of it fits
@@ -1768,23 +1815,25 @@ __global__ void k_batch_end(PW W, int64_t NP, uint32_t T, uint32_t t0, uint32_t 
   out[0] = lo;
   out[1] = a0;
   out[2] = acc_count(W, NP, T, lo);
+  out[3] = T;
+  out[4] = W.acc_exc[NP];
 }
 
 // compact accepted tests into the event list (in test order)
-__global__ void k_par_gather(PW W, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || !W.acc[i]) return;
-  int e = W.acc_scan[i];
-  W.ev_t[e] = W.tid[i];
-  W.ev_p1[e] = W.tp1[i];
-  W.ev_p2[e] = W.tp2[i];
-  W.ev_rec[e] = W.trec[i];
+__global__ void k_par_gather(PW W) {
+  const int n = W.rc[RC_N];
+  PAR_FOR(i, n) {
+    if (!W.acc[i]) continue;
+    int e = W.acc_scan[i];
+    W.ev_t[e] = W.tid[i];
+    W.ev_p1[e] = W.tp1[i];
+    W.ev_p2[e] = W.tp2[i];
+    W.ev_rec[e] = W.trec[i];
+  }
 }
 
-__global__ void k_par_resolve(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, uint64_t seed,
-                              uint32_t step) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
+__device__ inline void resolve_event(BoxDesc *boxes, const Arrays &A, const PW &W, int n_box,
+                                     uint64_t seed, uint32_t step, int e) {
   uint32_t t = W.ev_t[e];
   int p1 = W.ev_p1[e], p2 = W.ev_p2[e];
   MergeCtx m = merge_ctx(boxes, W, n_box, t);
@@ -1826,11 +1875,18 @@ __global__ void k_par_resolve(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, 
   W.f_lose[e] = win ? 0 : 1;
 }
 
+__global__ void k_par_resolve(BoxDesc *boxes, Arrays A, PW W, int n_box, uint64_t seed,
+                              uint32_t step) {
+  const int E = W.rc[RC_E];
+  PAR_FOR(e, E) resolve_event(boxes, A, W, n_box, seed, step, e);
+}
+
 // first event of every box (events are in test order, so box-major)
-__global__ void k_par_box_first(PW W, int E) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
-  if (e == 0 || W.ev_box[e - 1] != W.ev_box[e]) W.box_first[W.ev_box[e]] = e;
+__global__ void k_par_box_first(PW W) {
+  const int E = W.rc[RC_E];
+  PAR_FOR(e, E) {
+    if (e == 0 || W.ev_box[e - 1] != W.ev_box[e]) W.box_first[W.ev_box[e]] = e;
+  }
 }
 
 __device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc *B, int box,
@@ -2289,11 +2345,9 @@ __device__ __noinline__ void acc_apply(BoxDesc *B, const Arrays &A, const PW &W,
   }
 }
 
-// apply the round's winners, each at the positions its ranks give
-__global__ void __launch_bounds__(256)
-k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, uint64_t seed, uint32_t step,
-            int type, int want_terms, int64_t hole_base, int rid) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void apply_event(BoxDesc *boxes, const Arrays &A, const PW &W, int n_box,
+                                   uint64_t seed, uint32_t step, int type, int want_terms, int rid,
+                                   int E, int64_t hole_base, int e) {
   int box = -1;
   int64_t cs = 0, cc = 0, cw = 0;
   int nl = 0, lk[3] = {0, 0, 0}, ld[3] = {0, 0, 0};
@@ -2334,17 +2388,27 @@ k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, int E, uint64_t seed, uin
   for (int i = 0; i < 3; ++i) wave_flush_live(live, i < nl ? lk[i] : -1, i < nl ? ld[i] : 0);
 }
 
+// apply the round's winners, each at the positions its ranks give
+__global__ void __launch_bounds__(256)
+k_par_apply(BoxDesc *boxes, Arrays A, PW W, int n_box, uint64_t seed, uint32_t step,
+            int type, int want_terms, int rid) {
+  const int E = W.rc[RC_E];
+  const int64_t hole_base = (int64_t)*W.hole_top;
+  // the loop bound is uniform over the workgroup: every lane reaches the
+  // wavefront flushes of each pass
+  for (int base = blockIdx.x * blockDim.x; base < E; base += gridDim.x * blockDim.x)
+    apply_event(boxes, A, W, n_box, seed, step, type, want_terms, rid, E, hole_base,
+                base + (int)threadIdx.x);
+}
+
 // The removed sources of the round's winning target events with long
 // removal chains, one wavefront per event: each chunk's entries go to
 // lanes, the ranks of holes and removal records follow the chain order (a
 // ballot count of the entries before a lane), and each record names the
 // particle that took the source: the target, or the copy named by the last
 // copy marker before it in the chain (a ballot of the markers).
-__global__ void __launch_bounds__(256)
-k_par_apply_chains(BoxDesc *boxes, Arrays A, PW W, int E, int64_t hole_base, int rid) {
-  const int lane = threadIdx.x & 63;
-  const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (e >= E) return;
+__device__ inline void apply_chain(BoxDesc *boxes, const Arrays &A, const PW &W, int e, int lane,
+                                   int64_t hole_base, int rid) {
   if (!(W.ev_out[e] & O_WIN)) return;
   const int r = W.ev_rec[e];
   if (r < 0 || W.r_nchain[r] <= CHAIN_WAVE_MIN) return;
@@ -2396,10 +2460,21 @@ k_par_apply_chains(BoxDesc *boxes, Arrays A, PW W, int E, int64_t hole_base, int
   }
 }
 
+__global__ void __launch_bounds__(256)
+k_par_apply_chains(BoxDesc *boxes, Arrays A, PW W, int rid) {
+  const int lane = threadIdx.x & 63;
+  const int E = W.rc[RC_E];
+  const int64_t hole_base = (int64_t)*W.hole_top;
+  // one wavefront per event, striding over the round's events
+  for (int e = blockIdx.x * 4 + (threadIdx.x >> 6); e < E; e += gridDim.x * 4)
+    apply_chain(boxes, A, W, e, lane, hole_base, rid);
+}
+
 // per-box totals of a round: appended slots, new ids, records, winners
-__global__ void k_par_box_totals(BoxDesc *boxes, PW W, int n_box, int E) {
+__global__ void k_par_box_totals(BoxDesc *boxes, PW W, int n_box) {
   int box = blockIdx.x * blockDim.x + threadIdx.x;
   if (box >= n_box) return;
+  const int E = W.rc[RC_E];
   int first = W.box_first[box];
   if (first < 0) return;
   int last = first;
@@ -2420,9 +2495,7 @@ __global__ void k_par_box_totals(BoxDesc *boxes, PW W, int n_box, int E) {
   B->n_info += info;
 }
 
-__global__ void k_par_release(PW W, int E) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= E) return;
+__device__ inline void release_event(const PW &W, int e) {
   W.claim[W.ev_p1[e]] = UNCLAIMED;
   W.claim_wr[W.ev_p1[e]] = UNCLAIMED;
   const int r = W.ev_rec[e];
@@ -2435,6 +2508,11 @@ __global__ void k_par_release(PW W, int E) {
     W.claim[W.ev_p2[e]] = UNCLAIMED;
     W.claim_wr[W.ev_p2[e]] = UNCLAIMED;
   }
+}
+
+__global__ void k_par_release(PW W) {
+  const int E = W.rc[RC_E];
+  PAR_FOR(e, E) release_event(W, e);
 }
 
 // Tests of the round that must be drawn again: accepted tests that lost a
@@ -2455,9 +2533,7 @@ __device__ inline bool consumed_before(const PW &W, int p, uint32_t t, int rid) 
 // tests of the pair can change the first list's draws (a member they
 // removed is drawn again).  Such tests change no list until an earlier
 // event fills their entry, so they do not hold back others.
-__global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ inline void pending_test(BoxDesc *boxes, const PW &W, int n_box, int rid, int i) {
   uint32_t t = W.tid[i];
   const int64_t NP = (int64_t)n_box * W.P_stride;
   const int64_t k = test_pair(W, NP, t);
@@ -2506,12 +2582,16 @@ __global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int n, int rid) {
   W.f_pend[i] = f;
 }
 
+__global__ void k_par_pending(BoxDesc *boxes, PW W, int n_box, int rid) {
+  const int n = W.rc[RC_N];
+  PAR_FOR(i, n) pending_test(boxes, W, n_box, rid, i);
+}
+
 // the running minimum over each box's bins of the next round's minima
 // (one workgroup per box), and this round's cleared for the round after
 constexpr int PREFIX_THREADS = 256;
 __global__ void __launch_bounds__(PREFIX_THREADS) k_minpend_prefix(PW W) {
-  using BSU = hipcub::BlockScan<uint32_t, PREFIX_THREADS>;
-  __shared__ typename BSU::TempStorage tmp;
+  __shared__ scan_storage<PREFIX_THREADS, uint32_t> tmp;
   __shared__ uint32_t carry;
   const int tid = threadIdx.x;
   uint32_t *m = W.minpend_next + (int64_t)blockIdx.x * W.K_stride;
@@ -2520,8 +2600,8 @@ __global__ void __launch_bounds__(PREFIX_THREADS) k_minpend_prefix(PW W) {
     const int q = base + tid;
     uint32_t v = q < W.K_stride ? m[q] : UNCLAIMED;
     if (base > 0 && tid == 0) v = min(v, carry);
-    uint32_t r;
-    BSU(tmp).InclusiveScan(v, r, hipcub::Min());
+    uint32_t tot;
+    const uint32_t r = block_scan_incl<PREFIX_THREADS>(v, OpMin(), tmp, &tot);
     if (q < W.K_stride) {
       m[q] = r;
       c[q] = UNCLAIMED;
@@ -2531,10 +2611,47 @@ __global__ void __launch_bounds__(PREFIX_THREADS) k_minpend_prefix(PW W) {
   }
 }
 
-__global__ void k_par_gather_pending(PW W, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (W.f_pend[i]) W.pend[W.s_pend[i]] = W.tid[i];
+__global__ void k_par_gather_pending(PW W) {
+  const int n = W.rc[RC_N];
+  PAR_FOR(i, n) {
+    if (W.f_pend[i]) W.pend[W.s_pend[i]] = W.tid[i];
+  }
+}
+
+// The tests of a batch's first round: all of them.
+__global__ void k_batch_begin(PW W, int n_batch) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    W.rc[RC_N] = n_batch;
+    W.rc[RC_E] = 0;
+    W.rc[RC_TAIL] = 0;
+    for (int q = 0; q < 4; ++q) W.rstat[q] = 0ull;
+  }
+}
+
+// The end of a device-wide round that ran: its holes join the step's count,
+// and its pending tests go to another device-wide round while that settles
+// most of the batch (at most half of it still pending, and not the last
+// round), else to the tail.  A round without events leaves nothing
+// pending: with no winner no test's inputs changed, and a test on an empty
+// arrival entry stays thinned, as no particle arrived.
+__global__ void k_round_end(PW W, int n_batch, int round) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int n = W.rc[RC_N];
+  if (n == 0) return;  // the round did not run
+  const int E = W.rc[RC_E];
+  const int p = E > 0 ? W.s_pend[n] : 0;
+  *W.hole_top += (unsigned long long)W.s_hole[E];
+  W.pstat[0] += 1ull;
+  W.rstat[round == 0 ? 0 : 2] = (unsigned long long)E;
+  if (p > 0 && round < PAR_ROUNDS - 1 && p <= n_batch / 2) {
+    W.rc[RC_N] = p;
+    W.rstat[1] = (unsigned long long)p;
+  } else {
+    W.rc[RC_N] = 0;
+    W.rc[RC_TAIL] = p;
+    W.rstat[3] = (unsigned long long)p;
+    W.pstat[1] += (unsigned long long)p;
+  }
 }
 
 // first index of the sorted list v[0..n) holding a value >= x
@@ -2593,7 +2710,7 @@ __device__ inline void tail_barrier() {
 // large-weight particle a bin pair draws over and over, which the
 // weighting keeps), and a later test that changes it waits for them.
 __global__ void __launch_bounds__(TAIL_THREADS)
-k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pend, int type,
+k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, const uint32_t *pend, int type,
            uint64_t seed, uint32_t step, int precision, double band, double del_t,
            uint32_t acc_base, int want_terms, int rid0, unsigned long long *rounds_out,
            uint32_t batch_t0) {
@@ -2602,16 +2719,15 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
   const int tid = threadIdx.x;
   BoxDesc *B = &boxes[box];
   const int64_t pb = (int64_t)box * W.P_stride;
+  const int n = W.rc[RC_TAIL];
+  if (n == 0) return;
   const int lo = lower_bound_u32(pend, n, W.pair_exc[pb]);
   const int hi = lower_bound_u32(pend, n, W.pair_exc[pb + W.P_stride]);
   if (lo >= hi) return;
-  using BS = hipcub::BlockScan<int, TAIL_THREADS>;
-  using BS2 = hipcub::BlockScan<long long, TAIL_THREADS>;
-  using BSU = hipcub::BlockScan<uint32_t, TAIL_THREADS>;
-  __shared__ typename BS::TempStorage scan_tmp;
-  __shared__ typename BSU::TempStorage scan_tmp3;
+  __shared__ scan_storage<TAIL_THREADS, int> scan_tmp;
+  __shared__ scan_storage<TAIL_THREADS, uint32_t> scan_tmp3;
   __shared__ uint32_t s_carry;
-  __shared__ typename BS2::TempStorage scan_tmp2;
+  __shared__ scan_storage<TAIL_THREADS, long long> scan_tmp2;
   __shared__ uint32_t w_t[TAIL_THREADS];
   __shared__ int w_k[TAIL_THREADS], w_p1[TAIL_THREADS], w_p2[TAIL_THREADS];
   __shared__ int w_tns[TAIL_THREADS], w_fl[TAIL_THREADS];
@@ -2738,9 +2854,8 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
     long long ai = ((long long)c_id << 32) | (long long)c_app;
     long long ih = ((long long)c_hole << 32) | (long long)c_info;
     long long r_ai, r_ih, t_ai, t_ih;
-    BS2(scan_tmp2).ExclusiveSum(ai, r_ai, t_ai);
-    __syncthreads();
-    BS2(scan_tmp2).ExclusiveSum(ih, r_ih, t_ih);
+    r_ai = block_sum_excl<TAIL_THREADS>(ai, scan_tmp2, &t_ai);
+    r_ih = block_sum_excl<TAIL_THREADS>(ih, scan_tmp2, &t_ih);
     const int r_app = (int)(r_ai & 0xFFFFFFFFll), r_id = (int)(r_ai >> 32);
     const int r_info = (int)(r_ih & 0xFFFFFFFFll), r_hole = (int)(r_ih >> 32);
     int tot;
@@ -2836,8 +2951,8 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
       const int q = base + tid;
       uint32_t v = q < nbin ? s_pm[q] : UNCLAIMED;
       if (base > 0 && tid == 0) v = min(v, s_carry);
-      uint32_t r;
-      BSU(scan_tmp3).InclusiveScan(v, r, hipcub::Min());
+      uint32_t tot_m;
+      const uint32_t r = block_scan_incl<TAIL_THREADS>(v, OpMin(), scan_tmp3, &tot_m);
       if (q < nbin) s_pm[q] = r;
       if (tid == TAIL_THREADS - 1) s_carry = r;
       __syncthreads();
@@ -2857,8 +2972,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
       }
     }
     int r_keep;
-    BS(scan_tmp).ExclusiveSum(keep, r_keep, tot);
-    __syncthreads();
+    r_keep = block_sum_excl<TAIL_THREADS>(keep, scan_tmp, &tot);
     if (keep) {
       w_t[r_keep] = t;
       w_k[r_keep] = (int)(k - pb);
@@ -2893,21 +3007,20 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, int n, const uint32_t *pen
 }
 
 // the pair of every pending test, found in parallel before the tail
-__global__ void k_pend_pairs(PW W, int64_t NP, const uint32_t *pend, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) W.pend_k[i] = (int)test_pair(W, NP, pend[i]);
+__global__ void k_pend_pairs(PW W, int64_t NP, const uint32_t *pend) {
+  const int n = W.rc[RC_TAIL];
+  PAR_FOR(i, n) W.pend_k[i] = (int)test_pair(W, NP, pend[i]);
 }
 
 // the largest bin count among the boxes (their pair space for this step);
 // also clears the two entries after that pair space, which the pair scans
 // read as their last input
 __global__ void __launch_bounds__(256) k_max_nbin(const BoxDesc *boxes, int n_box, uint32_t *out) {
-  using Red = hipcub::BlockReduce<int, 256>;
-  __shared__ typename Red::TempStorage tr;
+  __shared__ scan_storage<256, int> tr;
   int m = 1;
   for (int b = threadIdx.x; b < n_box; b += 256)
     if (!boxes[b].error) m = max(m, boxes[b].n_bin);
-  const int r = Red(tr).Reduce(m, hipcub::Max());
+  const int r = block_reduce<256>(m, OpMax(), tr);
   if (threadIdx.x == 0) out[0] = (uint32_t)r;
 }
 
@@ -3037,6 +3150,8 @@ inline int parallel_init(ParallelWork &W, int n_box, int max_bin) {
   if (hipHostMalloc((void **)&W.h_i32, 64 * sizeof(int)) != hipSuccess) return ERR_HIP;
   if (hipHostMalloc((void **)&W.h_u64, 8 * sizeof(unsigned long long)) != hipSuccess) return ERR_HIP;
   if (hipStreamCreate(&W.st2) != hipSuccess) return ERR_HIP;
+  if (hipMalloc((void **)&W.pstat, 2 * sizeof(unsigned long long)) != hipSuccess) return ERR_HIP;
+  if (hipMemset(W.pstat, 0, 2 * sizeof(unsigned long long)) != hipSuccess) return ERR_HIP;
   if (hipEventCreateWithFlags(&W.ev_fork, hipEventDisableTiming) != hipSuccess) return ERR_HIP;
   if (hipEventCreateWithFlags(&W.ev_join, hipEventDisableTiming) != hipSuccess) return ERR_HIP;
   return 0;
@@ -3090,9 +3205,9 @@ inline void parallel_free_arrays(ParallelWork &W) {
                   d.f_lose, d.f_pend, d.s_pend, d.dead_t, d.dead_r, d.s_app, d.s_id, d.s_info, d.s_hole,
                   d.s_lose, d.box_first,
                   d.holes, d.holes_sorted, d.cmp_lo, d.cmp_hi, d.cmp_head, d.acc_samp,
-                  W.scan_tmp, W.sort_tmp,
+                  d.scan_part, W.sort_tmp, d.rc,
                   d.pair_acc, d.pair_accn, d.acc_exc, d.trec, d.tns, d.ev_rec, d.misc,
-                  d.hole_top, d.tail_rounds, d.pend_k, d.tp_in, d.tpair,
+                  d.hole_top, d.pend_k, d.tp_in, d.tpair,
                   d.acap, d.arr_n, d.arr_head, d.ap_p, d.ap_t, d.ap_dep, d.ap_next, d.ap_top,
                   d.arr_of, d.key0, d.depart_t, d.thin, d.minpend, d.minpend_next};
   for (void *p : ptrs)
@@ -3106,8 +3221,8 @@ inline void parallel_free_arrays(ParallelWork &W) {
   d.P_stride = ps;
   d.tab_classes = tc;
   d.max_bin = mb;
-  W.scan_tmp = W.sort_tmp = nullptr;
-  W.scan_tmp_bytes = W.sort_tmp_bytes = 0;
+  W.sort_tmp = nullptr;
+  W.sort_tmp_bytes = 0;
 }
 
 inline void parallel_free(ParallelWork &W) {
@@ -3121,8 +3236,10 @@ inline void parallel_free(ParallelWork &W) {
   if (W.ev_fork) (void)hipEventDestroy(W.ev_fork);
   if (W.ev_join) (void)hipEventDestroy(W.ev_join);
   if (W.st2) (void)hipStreamDestroy(W.st2);
+  if (W.pstat) (void)hipFree(W.pstat);
   W.ev_fork = W.ev_join = nullptr;
   W.st2 = nullptr;
+  W.pstat = nullptr;
 }
 
 // (re)allocate everything sized by particle capacities and the class count
@@ -3192,7 +3309,7 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(hipMemcpy(d.tile_first, hoff.data(), n_box * sizeof(int64_t), hipMemcpyHostToDevice));
   d.K_hist = d.K_stride;
   PCHECK(pmalloc(&d.hist, H));
-  PCHECK(pmalloc(&d.hist_scan, H));
+  PCHECK(pmalloc(&d.hist_scan, H + 1));
   PCHECK(pmalloc(&d.lstart, (size_t)n_box * d.K_stride));
   PCHECK(pmalloc(&d.lcnt, (size_t)n_box * d.K_stride));
   PCHECK(pmalloc(&d.live, (size_t)n_box * d.K_stride));
@@ -3209,9 +3326,13 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(pmalloc(&d.pair_accn, W.NP + 1));
   PCHECK(pmalloc(&d.acc_exc, W.NP + 1));
   PCHECK(hipMemset(d.pair_accn, 0, sizeof(uint32_t) * (W.NP + 1)));
-  PCHECK(pmalloc(&d.misc, 4));
-  PCHECK(pmalloc(&d.hole_top, 1));
-  PCHECK(pmalloc(&d.tail_rounds, 1));
+  PCHECK(pmalloc(&d.misc, 8));
+  PCHECK(pmalloc(&d.rc, RC_SIZE));
+  PCHECK(hipMemset(d.rc, 0, sizeof(int) * RC_SIZE));
+  d.pstat = W.pstat;
+  PCHECK(pmalloc(&d.hole_top, 8));
+  d.tail_rounds = d.hole_top + 1;
+  d.rstat = d.hole_top + 2;
   const int64_t T = W.T_cap;
   PCHECK(pmalloc(&d.acc, T + 1));
   PCHECK(pmalloc(&d.acc_scan, T + 1));
@@ -3259,17 +3380,10 @@ inline int parallel_layout(ParallelWork &W, const Arrays &A, int n_box,
   PCHECK(hipMemset(d.acc_samp, 0, sizeof(unsigned long long)));
   PCHECK(pmalloc(&d.cmp_hi, n_box));
   PCHECK(pmalloc(&d.cmp_head, n_box));
-  // scan scratch: the largest of the histogram, pair and test scans
-  size_t b1 = 0, b2 = 0, b3 = 0;
-  PCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b1, d.hist, d.hist_scan, (int)std::max<int64_t>(H, 1)));
-  PCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b2, d.pair_n, d.pair_exc, (int)(W.NP + 1)));
-  PCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, b3, d.acc, d.acc_scan, (int)(T + 1)));
-  size_t b4 = 0;
-  PCHECK(hipcub::DeviceScan::InclusiveScan(nullptr, b4, d.tp_in, d.tpair, hipcub::Max(), (int)T));
-  W.scan_tmp_bytes = std::max(std::max(b1, b4), std::max(b2, b3));
-  PCHECK(hipMalloc(&W.scan_tmp, W.scan_tmp_bytes));
-  PCHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, W.sort_tmp_bytes, d.holes, d.holes_sorted,
-                                           (int)W.hole_cap));
+  // the chunk totals of the device-wide scans
+  PCHECK(pmalloc(&d.scan_part, (size_t)SCAN_COLS * SCAN_MAX_BLOCKS));
+  PCHECK(rocprim::radix_sort_keys(nullptr, W.sort_tmp_bytes, d.holes, d.holes_sorted,
+                                  (unsigned)W.hole_cap, 0u, 64u, (hipStream_t)0));
   PCHECK(hipMalloc(&W.sort_tmp, W.sort_tmp_bytes));
   hipLaunchKernelGGL(k_fill_u32, dim3((tc + 255) / 256), dim3(256), 0, 0, d.claim, tc, UNCLAIMED);
   hipLaunchKernelGGL(k_fill_u32, dim3((tc + 255) / 256), dim3(256), 0, 0, d.claim_wr, tc, UNCLAIMED);
@@ -3341,8 +3455,10 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
   // the histogram spans the step's grid: keys are below mb_step * classes
   d.K_hist = mb_step * d.tab_classes;
   PLAUNCH(W, "k_par_hist", k_par_hist, gt, bt, d.K_hist * sizeof(int), st, d_box, d);
-  PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.hist, d.hist_scan,
-                                          (int)((int64_t)d.K_hist * W.n_tiles), st));
+  {
+    const int64_t nh = (int64_t)d.K_hist * W.n_tiles;
+    scan_excl_sum(scan_job(d.hist, d.hist_scan, d.scan_part, nh), nh, st);
+  }
   hipLaunchKernelGGL(k_par_lists, dim3((d.K_stride + 255) / 256, n_box), dim3(256), 0, st, d_box, d,
                      n_box);
   hipLaunchKernelGGL(k_par_acap, dim3((d.K_stride + 255) / 256, n_box), dim3(256), 0, st, d_box, d);
@@ -3354,15 +3470,22 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
   // 4. Poisson test counts per bin pair, laid out in reference order
   PLAUNCH(W, "k_par_pairs", k_par_pairs, dim3((unsigned)((d.P_stride + 255) / 256), n_box),
           dim3(256), 0, st, d_box, A, d, del_t, seed, step);
-  PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.pair_n, d.pair_exc,
-                                          (int)(W.NP + 1), st));
-  PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.pair_accn, d.acc_exc,
-                                          (int)(W.NP + 1), st));
-  PCHECK(hipMemcpyAsync(W.h_u32, d.pair_exc + W.NP, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-  PCHECK(hipMemcpyAsync(W.h_u32 + 1, d.acc_exc + W.NP, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  {
+    // both in one pass; each writes its total after the pair space
+    ScanJob J = scan_job(d.pair_n, d.pair_exc, d.scan_part, W.NP);
+    J.in[1] = d.pair_accn;
+    J.out[1] = d.acc_exc;
+    J.cols = 2;
+    scan_excl_sum(J, W.NP, st);
+  }
+  // the step's test count, its target slots and the first batch's bounds,
+  // in one read
+  hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, d, W.NP, 0u, (uint32_t)W.T_cap,
+                     (uint32_t)ACC_CAP, d.misc);
+  PCHECK(hipMemcpyAsync(W.h_u32 + 4, d.misc, 5 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   PCHECK(hipStreamSynchronize(st));
-  const uint32_t T = W.h_u32[0];
-  const uint32_t T_acc = W.h_u32[1];
+  const uint32_t T = W.h_u32[7];
+  const uint32_t T_acc = W.h_u32[8];
   // the test-to-pair table, when the step is one batch
   d.tpair_ok = 0;
   if (T > 0 && (int64_t)T <= W.T_cap) {
@@ -3370,8 +3493,7 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
                        0u);
     hipLaunchKernelGGL(k_mark_pair_starts, dim3((unsigned)((W.NP + 255) / 256)), dim3(256), 0, st,
                        d, W.NP);
-    PCHECK(hipcub::DeviceScan::InclusiveScan(W.scan_tmp, W.scan_tmp_bytes, d.tp_in, d.tpair,
-                                             hipcub::Max(), (int)T, st));
+    scan_incl_max(scan_job(d.tp_in, d.tpair, d.scan_part, (int64_t)T), (int64_t)T, st);
     d.tpair_ok = 1;
   }
   if (T_acc > 0 && d.r_nsamp == nullptr) {
@@ -3386,15 +3508,23 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
                        d.status, A.total_cap, ST_LIVE);
   }
   d.epoch = W.epoch;
-  // 5. tests, claims and merges, in batches of T_cap tests
+  // 5. tests, claims and merges, in batches of T_cap tests.  The rounds of
+  // a batch are launched without reading their counts back: each kernel
+  // reads them from d.rc and a round that does not run finds a zero count.
+  // Grids are sized from the previous batch's counts, and the kernels stride
+  // over any excess.
   int64_t n_holes = 0;
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, d.hole_top, 0ull);
   for (uint32_t t0 = 0; t0 < T;) {
-    uint32_t t1 = (uint32_t)std::min<int64_t>((int64_t)T, (int64_t)t0 + W.T_cap);
-    uint32_t a0 = 0, a1 = 0;
-    if (T_acc > 0) {
+    uint32_t t1, a0, a1;
+    if (t0 == 0) {
+      t1 = W.h_u32[4];
+      a0 = W.h_u32[5];
+      a1 = W.h_u32[6];
+    } else {
       // the batch ends where its target slots would overflow the records
-      hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, d, W.NP, T, t0,
-                         (uint32_t)W.T_cap, (uint32_t)d.acc_cap, d.misc);
+      hipLaunchKernelGGL(k_batch_end, dim3(1), dim3(64), 0, st, d, W.NP, t0, (uint32_t)W.T_cap,
+                         (uint32_t)d.acc_cap, d.misc);
       PCHECK(hipMemcpyAsync(W.h_u32 + 4, d.misc, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
       PCHECK(hipStreamSynchronize(st));
       t1 = W.h_u32[4];
@@ -3402,11 +3532,9 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
       a1 = W.h_u32[6];
     }
     const int n_acc = (int)(a1 - a0);
-    int n = (int)(t1 - t0);
+    const int n_batch = (int)(t1 - t0);
     const uint32_t batch_t0 = t0;
     t0 = t1;
-    const uint32_t *pend = nullptr;
-    const int n_batch = n;
     if (n_batch > 0)
       hipLaunchKernelGGL(k_fill_u8, dim3((n_batch + 255) / 256), dim3(256), 0, st, d.thin,
                          (int64_t)n_batch, (uint8_t)0);
@@ -3415,36 +3543,21 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
                        dim3(256), 0, st, d.minpend, (int64_t)n_box * d.K_stride, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)(((int64_t)n_box * d.K_stride + 255) / 256)),
                        dim3(256), 0, st, d.minpend_next, (int64_t)n_box * d.K_stride, UNCLAIMED);
-    for (int round = 0; n > 0; ++round) {
-      // more device-wide rounds only while they settle most of the batch:
-      // when over half of it is still pending, chains dominate and the tail
-      // takes them at once
-      if (round >= PAR_ROUNDS || (round >= 1 && n > n_batch / 2)) {
-        // the rest per box, one wavefront each
-        if (n_acc > 0) {
-          hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.rm_top, 1, 0);
-          hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.dup_top, 1, 0);
-        }
-        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, d.hole_top,
-                           (unsigned long long)n_holes);
-        hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, d.tail_rounds, 0ull);
-        hipLaunchKernelGGL(k_pend_pairs, dim3((n + 255) / 256), dim3(256), 0, st, d, W.NP, pend, n);
-        PLAUNCH(W, "k_par_tail", k_par_tail, dim3(n_box), dim3(TAIL_THREADS),
-                2 * d.K_stride * sizeof(int), st, d_box, A, d,
-                n_box, n, pend, type, seed, step, precision, band, del_t, a0,
-                want_terms, (int)W.rid + 1, d.tail_rounds, batch_t0);
-        PCHECK(hipMemcpyAsync(W.h_u64, d.hole_top, sizeof(unsigned long long),
-                              hipMemcpyDeviceToHost, st));
-        PCHECK(hipMemcpyAsync(W.h_u64 + 1, d.tail_rounds, sizeof(unsigned long long),
-                              hipMemcpyDeviceToHost, st));
-        PCHECK(hipStreamSynchronize(st));
-        n_holes = (int64_t)W.h_u64[0];
-        W.n_tail_rounds += (int64_t)W.h_u64[1];
-        W.rid += (int64_t)W.h_u64[1] + 2;  // past every round id the tail used
-        W.n_tail += n;
-        break;
-      }
-      W.n_rounds++;
+    hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(64), 0, st, d, n_batch);
+    const uint32_t *pend = nullptr;
+    // a grid for at most `bound` items, `per` a workgroup, sized from the
+    // last batch's count with a margin (the kernels stride over any excess)
+    auto grid = [](int64_t bound, int64_t last, int per) -> unsigned {
+      int64_t want = last >= 0 ? last + last / 4 + 1024 : bound;
+      if (want > bound) want = bound;
+      return (unsigned)std::max<int64_t>(1, (want + per - 1) / per);
+    };
+    for (int round = 0; round < PAR_ROUNDS; ++round) {
+      // a later round runs only while at most half of the batch is pending
+      const int nb = round == 0 ? n_batch : n_batch / 2;  // its largest test count
+      if (nb <= 0) break;  // a batch of one test has no second round
+      const unsigned g_n = round == 0 ? (unsigned)((nb + 255) / 256) : grid(nb, W.est[1], 256);
+      const unsigned g_e = grid(nb, W.est[round == 0 ? 0 : 2], 256);
       W.rid++;
       if (n_acc > 0) {
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.rm_top, 1, 0);
@@ -3455,69 +3568,83 @@ inline int parallel_step(ParallelWork &W, const Arrays &A, BoxDesc *d_box, int n
       if (n_acc > 0) {
         PCHECK(hipEventRecord(W.ev_fork, st));
         PCHECK(hipStreamWaitEvent(W.st2, W.ev_fork, 0));
-        const int na = pend ? n : n_acc;
         W.tst = W.st2;
-        const int lanes = pend ? 64 : ACC_LANES;
-        PLAUNCH(W, "k_par_acc_tests", k_par_acc_tests, dim3((na + lanes - 1) / lanes), dim3(64), 0, W.st2,
-                d_box, A, d, n_box, type, seed, step, batch_t0, n, pend, del_t, a0, n_acc);
+        const unsigned g_a = pend ? grid(nb, W.est[1], 64) : (unsigned)((n_acc + ACC_LANES - 1) / ACC_LANES);
+        PLAUNCH(W, "k_par_acc_tests", k_par_acc_tests, dim3(g_a), dim3(64), 0, W.st2,
+                d_box, A, d, n_box, type, seed, step, batch_t0, pend, del_t, a0, n_acc);
         W.tst = nullptr;
         PCHECK(hipEventRecord(W.ev_join, W.st2));
       }
-      PLAUNCH(W, "k_par_tests", k_par_tests, dim3((n + 255) / 256), dim3(256), 0, st, d_box, A, d,
-              n_box, type, seed, step, batch_t0, n, pend, precision, band, validation);
+      if (round == 0)
+        PLAUNCH(W, "k_par_tests", k_par_tests<false>, dim3(g_n), dim3(256), 0, st, d_box, A, d,
+                n_box, type, seed, step, batch_t0, pend, precision, band, validation);
+      else
+        PLAUNCH(W, "k_par_tests", k_par_tests<true>, dim3(g_n), dim3(256), 0, st, d_box, A, d,
+                n_box, type, seed, step, batch_t0, pend, precision, band, validation);
       if (n_acc > 0) PCHECK(hipStreamWaitEvent(st, W.ev_join, 0));
-      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.acc + n, 1, 0);
-      PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.acc, d.acc_scan, n + 1, st));
-      PCHECK(hipMemcpyAsync(W.h_i32, d.acc_scan + n, sizeof(int), hipMemcpyDeviceToHost, st));
-      PCHECK(hipStreamSynchronize(st));
-      const int E = W.h_i32[0];
-      if (E == 0) break;
-      hipLaunchKernelGGL(k_par_gather, dim3((n + 255) / 256), dim3(256), 0, st, d, n);
+      scan_excl_sum(scan_job(d.acc, d.acc_scan, d.scan_part, 0, d.rc + RC_N, d.rc + RC_E), nb, st);
+      hipLaunchKernelGGL(k_par_gather, dim3(g_n), dim3(256), 0, st, d);
       hipLaunchKernelGGL(k_fill_int, dim3((n_box + 255) / 256), dim3(256), 0, st, d.box_first,
                          (int64_t)n_box, -1);
-      hipLaunchKernelGGL(k_par_resolve, dim3((E + 255) / 256), dim3(256), 0, st, d_box, A, d, n_box,
-                         E, seed, step);
-      hipLaunchKernelGGL(k_par_box_first, dim3((E + 255) / 256), dim3(256), 0, st, d, E);
-      int *fs[5] = {d.f_app, d.f_id, d.f_info, d.f_hole, d.f_lose};
-      int *ss[5] = {d.s_app, d.s_id, d.s_info, d.s_hole, d.s_lose};
-      for (int q = 0; q < 5; ++q) {
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, fs[q] + E, 1, 0);
-        PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, fs[q], ss[q], E + 1, st));
+      hipLaunchKernelGGL(k_par_resolve, dim3(g_e), dim3(256), 0, st, d_box, A, d, n_box, seed, step);
+      hipLaunchKernelGGL(k_par_box_first, dim3(g_e), dim3(256), 0, st, d);
+      {
+        // the winners' ranks: the five flag arrays in one pass
+        const int *fs[5] = {d.f_app, d.f_id, d.f_info, d.f_hole, d.f_lose};
+        int *ss[5] = {d.s_app, d.s_id, d.s_info, d.s_hole, d.s_lose};
+        ScanJob J = scan_job(fs[0], ss[0], d.scan_part, 0, d.rc + RC_E);
+        for (int q = 1; q < 5; ++q) {
+          J.in[q] = (const uint32_t *)fs[q];
+          J.out[q] = (uint32_t *)ss[q];
+        }
+        J.cols = 5;
+        scan_excl_sum(J, nb, st);
       }
-      hipLaunchKernelGGL(k_par_apply_chains, dim3((E + 3) / 4), dim3(256), 0, st, d_box, A, d, E,
-                         n_holes, (int)W.rid);
-      PLAUNCH(W, "k_par_apply", k_par_apply, dim3((E + 255) / 256), dim3(256), 0, st, d_box, A, d,
-              n_box, E, seed, step, type, want_terms, n_holes, (int)W.rid);
-      hipLaunchKernelGGL(k_par_box_totals, gb, bb, 0, st, d_box, d, n_box, E);
-      hipLaunchKernelGGL(k_par_release, dim3((E + 255) / 256), dim3(256), 0, st, d, E);
-      hipLaunchKernelGGL(k_par_pending, dim3((n + 255) / 256), dim3(256), 0, st, d_box, d, n_box, n,
-                         (int)W.rid);
+      // one wavefront per event, a few events each: most have no long chain
+      hipLaunchKernelGGL(k_par_apply_chains, dim3(std::min(grid(nb, W.est[round == 0 ? 0 : 2], 16), 4096u)),
+                         dim3(256), 0, st, d_box, A, d, (int)W.rid);
+      PLAUNCH(W, "k_par_apply", k_par_apply, dim3(g_e), dim3(256), 0, st, d_box, A, d,
+              n_box, seed, step, type, want_terms, (int)W.rid);
+      hipLaunchKernelGGL(k_par_box_totals, gb, bb, 0, st, d_box, d, n_box);
+      hipLaunchKernelGGL(k_par_release, dim3(g_e), dim3(256), 0, st, d);
+      hipLaunchKernelGGL(k_par_pending, dim3(g_n), dim3(256), 0, st, d_box, d, n_box, (int)W.rid);
       hipLaunchKernelGGL(k_minpend_prefix, dim3(n_box), dim3(PREFIX_THREADS), 0, st, d);
       std::swap(d.minpend, d.minpend_next);
-      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.f_pend + n, 1, 0);
-      PCHECK(hipcub::DeviceScan::ExclusiveSum(W.scan_tmp, W.scan_tmp_bytes, d.f_pend, d.s_pend, n + 1, st));
-      hipLaunchKernelGGL(k_par_gather_pending, dim3((n + 255) / 256), dim3(256), 0, st, d, n);
-      PCHECK(hipMemcpyAsync(W.h_i32 + 1, d.s_pend + n, sizeof(int), hipMemcpyDeviceToHost, st));
-      PCHECK(hipMemcpyAsync(W.h_i32 + 2, d.s_hole + E, sizeof(int), hipMemcpyDeviceToHost, st));
-      PCHECK(hipStreamSynchronize(st));
-      n_holes += W.h_i32[2];
-      n = W.h_i32[1];
+      scan_excl_sum(scan_job(d.f_pend, d.s_pend, d.scan_part, 0, d.rc + RC_N), nb, st);
+      hipLaunchKernelGGL(k_par_gather_pending, dim3(g_n), dim3(256), 0, st, d);
+      hipLaunchKernelGGL(k_round_end, dim3(1), dim3(64), 0, st, d, n_batch, round);
+      // the next round and the tail read the pending tests from d.pend: the
+      // tests kernels copy them into tid before the round writes d.pend
       pend = d.pend;
-      if (n > 0) {
-        // pending tests must not alias the array the next round reads
-        PCHECK(hipMemcpyAsync(d.tid, d.pend, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
-        pend = d.tid;
-        // k_par_tests overwrites tid[i] with the same value it reads
-      }
     }
+    // the rest per box, one workgroup each
+    if (n_acc > 0) {
+      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.rm_top, 1, 0);
+      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, d.dup_top, 1, 0);
+    }
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, st, d.tail_rounds, 0ull);
+    hipLaunchKernelGGL(k_pend_pairs, dim3(grid(n_batch, W.est[3], 256)), dim3(256), 0, st, d,
+                       W.NP, d.pend);
+    PLAUNCH(W, "k_par_tail", k_par_tail, dim3(n_box), dim3(TAIL_THREADS),
+            2 * d.K_stride * sizeof(int), st, d_box, A, d,
+            n_box, d.pend, type, seed, step, precision, band, del_t, a0,
+            want_terms, (int)W.rid + 1, d.tail_rounds, batch_t0);
     if (n_batch > 0)
       hipLaunchKernelGGL(k_thin_counts, dim3((n_batch + 255) / 256), dim3(256), 0, st, d_box, d,
                          W.NP, batch_t0, n_batch);
+    // the holes, the tail's rounds and the batch's round sizes in one read
+    PCHECK(hipMemcpyAsync(W.h_u64, d.hole_top, 6 * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, st));
+    PCHECK(hipStreamSynchronize(st));
+    n_holes = (int64_t)W.h_u64[0];
+    W.n_tail_rounds += (int64_t)W.h_u64[1];
+    for (int q = 0; q < 4; ++q) W.est[q] = (int64_t)W.h_u64[2 + q];
+    W.rid += (int64_t)W.h_u64[1] + 2;  // past every round id the tail used
   }
   // 6. compaction
   if (n_holes > 0) {
-    PCHECK(hipcub::DeviceRadixSort::SortKeys(W.sort_tmp, W.sort_tmp_bytes, d.holes, d.holes_sorted,
-                                             (int)n_holes, 0, 64, st));
+    PCHECK(rocprim::radix_sort_keys(W.sort_tmp, W.sort_tmp_bytes, d.holes, d.holes_sorted,
+                                    (unsigned)n_holes, 0u, 64u, st));
     hipLaunchKernelGGL(k_compact_ranges, gb, bb, 0, st, d_box, d, n_box, (int)n_holes);
     PLAUNCH(W, "k_par_compact", k_compact_move, dim3((unsigned)((n_holes + 255) / 256)), dim3(256),
             0, st, d_box, A, d, (int)n_holes);

@@ -9,7 +9,7 @@
 // tiles of REBAL_TILE particles (grid x = tile, y = box).
 #pragma once
 
-#include <hipcub/hipcub.hpp>
+#include "pmc_scan.h"
 
 #include "pmc_exact.h"
 
@@ -92,10 +92,7 @@ __global__ void __launch_bounds__(REBAL_THREADS)
 k_tile_scan(const BoxDesc *boxes, RebalWork R, int with_kmax) {
   const int box = blockIdx.x, t = threadIdx.x;
   if (!R.plan[box]) return;
-  using Scan = hipcub::BlockScan<int, REBAL_THREADS>;
-  using Red = hipcub::BlockReduce<int, REBAL_THREADS>;
-  __shared__ typename Scan::TempStorage ts;
-  __shared__ typename Red::TempStorage tr;
+  __shared__ scan_storage<REBAL_THREADS, int> ts;
   const int n_tiles = (boxes[box].n_part + REBAL_TILE - 1) / REBAL_TILE;
   const int *a = R.tile_a + (int64_t)box * R.max_tiles;
   int *off = R.tile_off + (int64_t)box * R.max_tiles;
@@ -103,15 +100,14 @@ k_tile_scan(const BoxDesc *boxes, RebalWork R, int with_kmax) {
   for (int c0 = 0; c0 < n_tiles; c0 += REBAL_THREADS) {
     const int k = c0 + t;
     const int v = k < n_tiles ? a[k] : 0;
-    int ex = 0, total = 0;
-    Scan(ts).ExclusiveSum(v, ex, total);
+    int total = 0;
+    const int ex = block_sum_excl<REBAL_THREADS>(v, ts, &total);
     if (k < n_tiles) off[k] = done + ex;
     done += total;
     if (with_kmax && k < n_tiles) kmax = max(kmax, R.tile_kmax[(int64_t)box * R.max_tiles + k]);
-    __syncthreads();
   }
   if (with_kmax) {
-    const int m = Red(tr).Reduce(kmax, hipcub::Max());
+    const int m = block_reduce<REBAL_THREADS>(kmax, OpMax(), ts);
     if (t == 0) R.box_imax[box] = m;
   }
   if (t == 0) R.box_tot[box] = done;
@@ -130,12 +126,11 @@ k_double_count(const BoxDesc *boxes, Arrays A, RebalWork R, int g, int c, int pa
   if (R.plan[box] <= pass || B->error) return;
   const int n = B->n_part, start = blockIdx.x * REBAL_TILE;
   if (start >= n) return;
-  using Red = hipcub::BlockReduce<int, REBAL_THREADS>;
-  __shared__ typename Red::TempStorage tr;
+  __shared__ scan_storage<REBAL_THREADS, int> tr;
   int cnt = 0;
   const int end = min(n, start + REBAL_TILE);
   for (int i = start + t; i < end; i += REBAL_THREADS) cnt += in_gc(A, B->off + i, g, c) ? 1 : 0;
-  const int tot = Red(tr).Sum(cnt);
+  const int tot = block_reduce<REBAL_THREADS>(cnt, OpSum(), tr);
   if (t == 0) R.tile_a[(int64_t)box * R.max_tiles + blockIdx.x] = tot;
 }
 
@@ -146,16 +141,15 @@ k_double_copy(BoxDesc *boxes, Arrays A, RebalWork R, int g, int c, int pass) {
   if (R.plan[box] <= pass || B->error) return;
   const int n = B->n_part, start = blockIdx.x * REBAL_TILE;
   if (start >= n) return;
-  using Scan = hipcub::BlockScan<int, REBAL_THREADS>;
-  __shared__ typename Scan::TempStorage ts;
+  __shared__ scan_storage<REBAL_THREADS, int> ts;
   const int64_t base = B->off, cap = B->cap;
   const int64_t next0 = B->next_id, stride = B->id_stride, offset = B->id_offset;
   int done = R.tile_off[(int64_t)box * R.max_tiles + blockIdx.x];
   for (int k = 0; k < REBAL_ITEMS; ++k) {
     const int i = start + k * REBAL_THREADS + t;
     const int flag = (i < n && in_gc(A, base + i, g, c)) ? 1 : 0;
-    int rank = 0, total = 0;
-    Scan(ts).ExclusiveSum(flag, rank, total);
+    int total = 0;
+    const int rank = block_sum_excl<REBAL_THREADS>(flag, ts, &total);
     if (flag) {
       const int64_t dst = (int64_t)n + done + rank;
       if (dst < cap) {
@@ -164,7 +158,6 @@ k_double_copy(BoxDesc *boxes, Arrays A, RebalWork R, int g, int c, int pass) {
       }
     }
     done += total;
-    __syncthreads();
   }
 }
 
@@ -339,8 +332,7 @@ k_remove_mark(const BoxDesc *boxes, Arrays A, RebalWork R, Removal X, uint64_t s
   if (!R.plan[box] || B->error) return;
   const int n = B->n_part, start = blockIdx.x * REBAL_TILE;
   if (start >= n) return;
-  using Red = hipcub::BlockReduce<int, REBAL_THREADS>;
-  __shared__ typename Red::TempStorage tr;
+  __shared__ scan_storage<REBAL_THREADS, int> tr;
   int cnt = 0, kmax = -1;
   const int end = min(n, start + REBAL_TILE);
   for (int i = start + t; i < end; i += REBAL_THREADS) {
@@ -349,9 +341,8 @@ k_remove_mark(const BoxDesc *boxes, Arrays A, RebalWork R, Removal X, uint64_t s
     cnt += r ? 1 : 0;
     if (!r) kmax = max(kmax, i);
   }
-  const int tot = Red(tr).Sum(cnt);
-  __syncthreads();
-  const int km = Red(tr).Reduce(kmax, hipcub::Max());
+  const int tot = block_reduce<REBAL_THREADS>(cnt, OpSum(), tr);
+  const int km = block_reduce<REBAL_THREADS>(kmax, OpMax(), tr);
   if (t == 0) {
     R.tile_a[(int64_t)box * R.max_tiles + blockIdx.x] = tot;
     R.tile_kmax[(int64_t)box * R.max_tiles + blockIdx.x] = km;
@@ -367,16 +358,15 @@ k_remove_prefix(BoxDesc *boxes, Arrays A, RebalWork R, int action) {
   if (!R.plan[box] || B->error) return;
   const int n = B->n_part, start = blockIdx.x * REBAL_TILE;
   if (start >= n) return;
-  using Scan = hipcub::BlockScan<int, REBAL_THREADS>;
-  __shared__ typename Scan::TempStorage ts;
+  __shared__ scan_storage<REBAL_THREADS, int> ts;
   const int64_t base = B->off;
   const int n_rm = R.box_tot[box];
   int done = R.tile_off[(int64_t)box * R.max_tiles + blockIdx.x];
   for (int k = 0; k < REBAL_ITEMS; ++k) {
     const int i = start + k * REBAL_THREADS + t;
     const int flag = i < n ? R.rm[base + i] : 0;
-    int incl = 0, total = 0;
-    Scan(ts).InclusiveSum(flag, incl, total);
+    int total = 0;
+    const int incl = block_scan_incl<REBAL_THREADS>(flag, OpSum(), ts, &total);
     if (i < n) {
       R.rp[base + i] = done + incl;
       if (flag) {
@@ -389,7 +379,6 @@ k_remove_prefix(BoxDesc *boxes, Arrays A, RebalWork R, int action) {
       }
     }
     done += total;
-    __syncthreads();
   }
 }
 
