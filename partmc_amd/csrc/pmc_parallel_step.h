@@ -2055,21 +2055,22 @@ __device__ inline void put_info(const Arrays &A, BoxDesc *B, Alloc &al, int64_t 
 }
 
 // Both parents' components, read before any is written (dst may be either
-// parent's slot) and all in flight at once: a load inside the selection
-// branches below would cost one memory round trip per component.
+// parent's slot) and all in flight at once.  The loads are unconditional
+// (entries past a parent's count reread its first component, a valid
+// address): a load under `if (c < n)` becomes a branch, and the compiler
+// waits for every load in flight at each one, one memory round trip per
+// component.
 __device__ inline void load_comps2(const Arrays &A, int64_t p1, int64_t p2, int n1, int n2,
                                    int *s1, double *t1, int *s2, double *t2) {
   const int64_t tc = A.total_cap;
 #pragma unroll
   for (int c = 0; c < MAX_COMP; ++c) {
-    if (c < n1) {
-      s1[c] = A.csrc[(int64_t)c * tc + p1];
-      t1[c] = A.ctime[(int64_t)c * tc + p1];
-    }
-    if (c < n2) {
-      s2[c] = A.csrc[(int64_t)c * tc + p2];
-      t2[c] = A.ctime[(int64_t)c * tc + p2];
-    }
+    const int64_t o1 = (int64_t)(c < n1 ? c : 0) * tc + p1;
+    const int64_t o2 = (int64_t)(c < n2 ? c : 0) * tc + p2;
+    s1[c] = A.csrc[o1];
+    t1[c] = A.ctime[o1];
+    s2[c] = A.csrc[o2];
+    t2[c] = A.ctime[o2];
   }
 }
 
