@@ -742,11 +742,27 @@ __device__ inline bool drawable(const PW &W, int p, int key) {
 // that left after the pair began is drawn again among the present ones.
 
 // the first change of slot p in this step: remember its step-start key
-__device__ inline void slot_touch(const PW &W, int64_t p) {
-  if (st_get(W, p) == ST_LIVE) {
-    W.key0[p] = W.key[p];
+// The fields of slot p that the list bookkeeping reads, loaded together
+// (one memory round trip instead of a chain of them); a slot first touched
+// this step records its step-start list and no departure yet.
+struct SlotView {
+  int key, key0, e;
+  uint32_t dep;
+};
+__device__ inline SlotView slot_touch(const PW &W, int64_t p) {
+  SlotView v;
+  const uint8_t stv = W.status[p];
+  v.key = W.key[p];
+  v.key0 = W.key0[p];
+  v.dep = W.depart_t[p];
+  v.e = W.arr_of[p];
+  if ((int)(stv >> 2) != W.epoch || (stv & 3) == ST_LIVE) {
+    v.key0 = v.key;
+    v.dep = UNCLAIMED;
+    W.key0[p] = v.key;
     W.depart_t[p] = UNCLAIMED;
   }
+  return v;
 }
 
 // An arrival takes the next entry of its list's array, or past A_MAX of
@@ -796,18 +812,21 @@ __device__ inline void arrivals_each(const PW &W, int64_t L, F f) {
     for (int e = W.arr_head[L]; e >= 0; e = W.ap_next[e]) f((int64_t)e);
 }
 
-// the particle in slot p leaves the list it is in, at sequence index t
-__device__ inline void slot_leave(const PW &W, int64_t p, uint32_t t) {
-  slot_touch(W, p);
-  const int cur = W.key[p];
-  if (cur < 0) return;
-  if (cur == W.key0[p] && W.depart_t[p] == UNCLAIMED) {
+// the particle in slot p (as v shows it) leaves the list it is in, at
+// sequence index t
+__device__ inline void slot_leave_view(const PW &W, int64_t p, uint32_t t, const SlotView &v) {
+  if (v.key < 0) return;
+  if (v.key == v.key0 && v.dep == UNCLAIMED) {
     W.depart_t[p] = t;  // the slot's step-start member
     return;
   }
-  const int e = W.arr_of[p];  // an arrival of list cur
+  const int e = v.e;  // an arrival of list v.key
   if (e >= 0 && e < W.ap_ovf + W.ap_cap && W.ap_p[e] == (int)p && W.ap_dep[e] == UNCLAIMED)
     W.ap_dep[e] = t;
+}
+
+__device__ inline void slot_leave(const PW &W, int64_t p, uint32_t t) {
+  slot_leave_view(W, p, t, slot_touch(W, p));
 }
 
 // slot p receives a particle of list new_key in place of the one it held
@@ -815,10 +834,10 @@ __device__ inline void slot_leave(const PW &W, int64_t p, uint32_t t) {
 // leaves its list at t_leave, the new one joins at t_join
 __device__ inline void slot_write(const PW &W, BoxDesc *B, int box, int64_t p, uint32_t t_leave,
                                   uint32_t t_join, int new_key) {
-  slot_touch(W, p);
-  if (W.key[p] == new_key) return;  // changed in place, still in its list
-  slot_leave(W, p, t_leave);
-  if (new_key != W.key0[p]) record_arrival(W, B, box, new_key, p, t_join);
+  const SlotView v = slot_touch(W, p);
+  if (v.key == new_key) return;  // changed in place, still in its list
+  slot_leave_view(W, p, t_leave, v);
+  if (new_key != v.key0) record_arrival(W, B, box, new_key, p, t_join);
 }
 
 // an appended slot receives a particle of list new_key at sequence index t
