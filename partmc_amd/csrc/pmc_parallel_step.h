@@ -866,16 +866,17 @@ __device__ inline uint32_t mix32(uint32_t h) {
 // a test on the same list) as the one whose slot hashes lowest under u.  So
 // each arrival present at tau is drawn at the rate of a member.  Returns
 // the pool entry, or -1.
+constexpr int ACHUNK = 8;  // arrival entries loaded together
+
 __device__ inline int64_t arrival_draw(const PW &W, int64_t L, int j, uint32_t tau, uint32_t u,
                                        int exclude) {
   int n_before = 0;
   int64_t best = -1;
   uint32_t best_h = 0;
   int best_p = 0;
-  arrivals_each(W, L, [&](int64_t e) {
-    if (W.ap_t[e] >= tau) return;
+  auto consider = [&](int64_t e, uint32_t te, int pe) {
+    if (te >= tau) return;
     ++n_before;
-    const int pe = W.ap_p[e];
     if (pe == exclude) return;
     const uint32_t h = mix32(u ^ ((uint32_t)pe * 0x9E3779B9u));
     if (best < 0 || h < best_h || (h == best_h && pe < best_p)) {
@@ -883,7 +884,27 @@ __device__ inline int64_t arrival_draw(const PW &W, int64_t L, int j, uint32_t t
       best_h = h;
       best_p = pe;
     }
-  });
+  };
+  // the array part a chunk at a time, its loads issued before any is used
+  // (index clamped to the last entry), entries in order as before
+  const int n = W.arr_n[L];
+  const int na = n < A_MAX ? n : A_MAX;
+  const int64_t e0 = L * A_MAX;
+  for (int i0 = 0; i0 < na; i0 += ACHUNK) {
+    uint32_t te[ACHUNK];
+    int pe[ACHUNK];
+#pragma unroll
+    for (int q = 0; q < ACHUNK; ++q) {
+      const int64_t e = e0 + (i0 + q < na ? i0 + q : na - 1);
+      te[q] = W.ap_t[e];
+      pe[q] = W.ap_p[e];
+    }
+#pragma unroll
+    for (int q = 0; q < ACHUNK; ++q)
+      if (i0 + q < na) consider(e0 + i0 + q, te[q], pe[q]);
+  }
+  if (n > A_MAX)
+    for (int e = W.arr_head[L]; e >= 0; e = W.ap_next[e]) consider((int64_t)e, W.ap_t[e], W.ap_p[e]);
   return j <= n_before ? best : -1;
 }
 
@@ -1194,7 +1215,7 @@ __device__ inline int perset_decide(BoxDesc *B, const Arrays &A, const PW &W, in
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += gridDim.x * blockDim.x)
 
 // test i of the round: draw, evaluate, accept, claim
-__device__ inline void par_test(BoxDesc *boxes, const Arrays &A, const PW &W, int n_box, int type,
+__device__ __forceinline__ void par_test(BoxDesc *boxes, const Arrays &A, const PW &W, int n_box, int type,
                                 uint64_t seed, uint32_t step, uint32_t t0, const uint32_t *pend,
                                 int precision, double band, int validation, int i) {
   const uint32_t t = pend ? pend[i] : t0 + (uint32_t)i;
