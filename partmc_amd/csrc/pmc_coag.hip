@@ -234,12 +234,10 @@ static int layout(pmc_gpu *h, const std::vector<int64_t> &new_cap) {
       for (int s = 0; s < h->n_spec; ++s)
         HIPCHECK(hipMemcpy(A.vol + s * total + d, old.vol + s * old.total_cap + o,
                            n * sizeof(double), hipMemcpyDeviceToDevice));
-      for (int c = 0; c < MAX_COMP; ++c) {
-        HIPCHECK(hipMemcpy(A.csrc + c * total + d, old.csrc + c * old.total_cap + o,
-                           n * sizeof(int), hipMemcpyDeviceToDevice));
-        HIPCHECK(hipMemcpy(A.ctime + c * total + d, old.ctime + c * old.total_cap + o,
-                           n * sizeof(double), hipMemcpyDeviceToDevice));
-      }
+      HIPCHECK(hipMemcpy(A.csrc + d * MAX_COMP, old.csrc + o * MAX_COMP,
+                         n * MAX_COMP * sizeof(int), hipMemcpyDeviceToDevice));
+      HIPCHECK(hipMemcpy(A.ctime + d * MAX_COMP, old.ctime + o * MAX_COMP,
+                         n * MAX_COMP * sizeof(double), hipMemcpyDeviceToDevice));
 #define MOVE(F, T)                                                               \
   HIPCHECK(hipMemcpy(A.F + d, old.F + o, n * sizeof(T), hipMemcpyDeviceToDevice))
       MOVE(vtot, double);
@@ -535,8 +533,8 @@ __global__ void k_default_components(Arrays A, int64_t base, int n, int set_comp
   if (set_comp) {
     A.ncomp[p] = 1;
     for (int c = 0; c < MAX_COMP; ++c) {
-      A.csrc[(int64_t)c * A.total_cap + p] = 0;
-      A.ctime[(int64_t)c * A.total_cap + p] = 0.0;
+      A.csrc[comp_at(p, c)] = 0;
+      A.ctime[comp_at(p, c)] = 0.0;
     }
   }
   if (set_ct) {
@@ -579,18 +577,11 @@ static int put_particles(pmc_gpu *h, const BoxDesc &hb, int64_t first, int n_par
     }
     if (n_comp) {
       HIPCHECK(hipMemcpy(A.ncomp + base, n_comp, n_part * sizeof(int), hipMemcpyHostToDevice));
-      std::vector<int> ci(n_part);
-      std::vector<double> cd(n_part);
-      for (int c = 0; c < MAX_COMP; ++c) {
-        for (int i = 0; i < n_part; ++i) {
-          ci[i] = comp_source[(size_t)i * MAX_COMP + c];
-          cd[i] = comp_time[(size_t)i * MAX_COMP + c];
-        }
-        HIPCHECK(hipMemcpy(A.csrc + c * tc + base, ci.data(), n_part * sizeof(int),
-                           hipMemcpyHostToDevice));
-        HIPCHECK(hipMemcpy(A.ctime + c * tc + base, cd.data(), n_part * sizeof(double),
-                           hipMemcpyHostToDevice));
-      }
+      // the caller's [n_part][MAX_COMP] arrays are the device layout
+      HIPCHECK(hipMemcpy(A.csrc + comp_at(base, 0), comp_source,
+                         (size_t)n_part * MAX_COMP * sizeof(int), hipMemcpyHostToDevice));
+      HIPCHECK(hipMemcpy(A.ctime + comp_at(base, 0), comp_time,
+                         (size_t)n_part * MAX_COMP * sizeof(double), hipMemcpyHostToDevice));
     }
     if (least && greatest) {
       HIPCHECK(hipMemcpy(A.least + base, least, n_part * sizeof(double), hipMemcpyHostToDevice));
@@ -728,18 +719,20 @@ int pmc_gpu_download(pmc_gpu *h, int box, int max_part, int *n_part, double *vol
   if (comp_source || comp_time) {
     // the device keeps only the first n_comp entries of a particle; the
     // rest read as zero
-    std::vector<int> ci(n), nco(n);
-    std::vector<double> cd(n);
+    std::vector<int> nco(n);
     HIPCHECK(hipMemcpy(nco.data(), A.ncomp + base, n * sizeof(int), hipMemcpyDeviceToHost));
-    for (int c = 0; c < MAX_COMP; ++c) {
-      if (comp_source) {
-        HIPCHECK(hipMemcpy(ci.data(), A.csrc + c * tc + base, n * sizeof(int), hipMemcpyDeviceToHost));
-        for (int i = 0; i < n; ++i) comp_source[(size_t)i * MAX_COMP + c] = c < nco[i] ? ci[i] : 0;
-      }
-      if (comp_time) {
-        HIPCHECK(hipMemcpy(cd.data(), A.ctime + c * tc + base, n * sizeof(double), hipMemcpyDeviceToHost));
-        for (int i = 0; i < n; ++i) comp_time[(size_t)i * MAX_COMP + c] = c < nco[i] ? cd[i] : 0.0;
-      }
+    // the device layout is the caller's [n][MAX_COMP]
+    if (comp_source) {
+      HIPCHECK(hipMemcpy(comp_source, A.csrc + comp_at(base, 0), (size_t)n * MAX_COMP * sizeof(int),
+                         hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < n; ++i)
+        for (int c = nco[i] > 0 ? nco[i] : 0; c < MAX_COMP; ++c) comp_source[(size_t)i * MAX_COMP + c] = 0;
+    }
+    if (comp_time) {
+      HIPCHECK(hipMemcpy(comp_time, A.ctime + comp_at(base, 0), (size_t)n * MAX_COMP * sizeof(double),
+                         hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < n; ++i)
+        for (int c = nco[i] > 0 ? nco[i] : 0; c < MAX_COMP; ++c) comp_time[(size_t)i * MAX_COMP + c] = 0.0;
     }
   }
   if (least) HIPCHECK(hipMemcpy(least, A.least + base, n * sizeof(double), hipMemcpyDeviceToHost));

@@ -314,7 +314,7 @@ __host__ __device__ inline double powi(double x, int n) {
 }
 
 // sphere_vol2rad / sphere_rad2vol, src/util.F90:240-271
-__device__ inline double vol2rad(double v) {
+__device__ __forceinline__ double vol2rad(double v) {
   return pow(3.0 * v / 4.0 / C_PI, 1.0 / 3.0);
 }
 __device__ inline double rad2vol(double r) {
@@ -391,7 +391,7 @@ __device__ inline double weight_nc(int type, double mag, double e, double r) {
 
 // aero_weight_array_num_conc_at_radius, src/aero_weight_array.F90:232-252
 // (1-based class)
-__device__ inline double nc_at_radius(const Weights &w, int klass, double r) {
+__device__ __forceinline__ double nc_at_radius(const Weights &w, int klass, double r) {
   double inv_sum = 0.0;
   for (int g = 0; g < w.n_group; ++g) {
     int idx = g + w.n_group * (klass - 1);
@@ -401,7 +401,7 @@ __device__ inline double nc_at_radius(const Weights &w, int klass, double r) {
 }
 
 // coag_num_conc_factor, src/coag_kernel.F90:346-377
-__device__ inline double coag_num_conc_factor(const Weights &w, double i_r,
+__device__ __forceinline__ double coag_num_conc_factor(const Weights &w, double i_r,
                                               double j_r, int ic, int jc,
                                               int ijc) {
   double ij_r = vol2rad(rad2vol(i_r) + rad2vol(j_r));

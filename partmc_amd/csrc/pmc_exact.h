@@ -125,15 +125,15 @@ __device__ inline void copy_particle(const Arrays &A, int64_t from, int64_t to) 
 #pragma unroll
     for (int j = 0; j < 10; ++j) {
       if (c0 + j < nco) {
-        cs[j] = A.csrc[(int64_t)(c0 + j) * tc + from];
-        ct[j] = A.ctime[(int64_t)(c0 + j) * tc + from];
+        cs[j] = A.csrc[comp_at(from, c0 + j)];
+        ct[j] = A.ctime[comp_at(from, c0 + j)];
       }
     }
 #pragma unroll
     for (int j = 0; j < 10; ++j) {
       if (c0 + j < nco) {
-        A.csrc[(int64_t)(c0 + j) * tc + to] = cs[j];
-        A.ctime[(int64_t)(c0 + j) * tc + to] = ct[j];
+        A.csrc[comp_at(to, c0 + j)] = cs[j];
+        A.ctime[comp_at(to, c0 + j)] = ct[j];
       }
     }
   }
@@ -532,8 +532,8 @@ struct ExactBox {
     r.nprim = A->nprim[p];
     r.ncomp = A->ncomp[p];
     for (int c = 0; c < r.ncomp; ++c) {
-      r.csrc[c] = A->csrc[(int64_t)c * A->total_cap + p];
-      r.ctime[c] = A->ctime[(int64_t)c * A->total_cap + p];
+      r.csrc[c] = A->csrc[comp_at(p, c)];
+      r.ctime[c] = A->ctime[comp_at(p, c)];
     }
     r.least = A->least[p];
     r.greatest = A->greatest[p];
@@ -552,8 +552,8 @@ struct ExactBox {
     A->nprim[p] = r.nprim;
     A->ncomp[p] = r.ncomp;
     for (int c = 0; c < r.ncomp; ++c) {
-      A->csrc[(int64_t)c * A->total_cap + p] = r.csrc[c];
-      A->ctime[(int64_t)c * A->total_cap + p] = r.ctime[c];
+      A->csrc[comp_at(p, c)] = r.csrc[c];
+      A->ctime[comp_at(p, c)] = r.ctime[c];
     }
     A->least[p] = r.least;
     A->greatest[p] = r.greatest;

@@ -537,8 +537,8 @@ __device__ inline void load_rec_head(const Arrays &A, int64_t p, Rec &r) {
 
 __device__ inline void load_rec_comps(const Arrays &A, int64_t p, Rec &r) {
   for (int c = 0; c < r.ncomp; ++c) {
-    r.csrc[c] = A.csrc[(int64_t)c * A.total_cap + p];
-    r.ctime[c] = A.ctime[(int64_t)c * A.total_cap + p];
+    r.csrc[c] = A.csrc[comp_at(p, c)];
+    r.ctime[c] = A.ctime[comp_at(p, c)];
   }
 }
 
@@ -1947,8 +1947,8 @@ __device__ inline void write_new_particle(const Arrays &A, const PW &W, BoxDesc 
   A.nprim[slot] = r.nprim;
   A.ncomp[slot] = r.ncomp;
   for (int c = 0; c < r.ncomp; ++c) {
-    A.csrc[(int64_t)c * A.total_cap + slot] = r.csrc[c];
-    A.ctime[(int64_t)c * A.total_cap + slot] = r.ctime[c];
+    A.csrc[comp_at(slot, c)] = r.csrc[c];
+    A.ctime[comp_at(slot, c)] = r.ctime[c];
   }
   A.least[slot] = r.least;
   A.greatest[slot] = r.greatest;
@@ -2096,21 +2096,19 @@ __device__ inline void put_info(const Arrays &A, BoxDesc *B, Alloc &al, int64_t 
 
 // Both parents' components, read before any is written (dst may be either
 // parent's slot) and all in flight at once.  The loads are unconditional
-// (entries past a parent's count reread its first component, a valid
-// address): a load under `if (c < n)` becomes a branch, and the compiler
-// waits for every load in flight at each one, one memory round trip per
-// component.
+// (every one of a particle's MAX_COMP entries, which sit together): a load
+// under `if (c < n)` becomes a branch, and the compiler waits for every
+// load in flight at each one, one memory round trip per component.
 __device__ inline void load_comps2(const Arrays &A, int64_t p1, int64_t p2, int n1, int n2,
                                    int *s1, double *t1, int *s2, double *t2) {
-  const int64_t tc = A.total_cap;
+  (void)n1;
+  (void)n2;
 #pragma unroll
   for (int c = 0; c < MAX_COMP; ++c) {
-    const int64_t o1 = (int64_t)(c < n1 ? c : 0) * tc + p1;
-    const int64_t o2 = (int64_t)(c < n2 ? c : 0) * tc + p2;
-    s1[c] = A.csrc[o1];
-    t1[c] = A.ctime[o1];
-    s2[c] = A.csrc[o2];
-    t2[c] = A.ctime[o2];
+    s1[c] = A.csrc[comp_at(p1, c)];
+    t1[c] = A.ctime[comp_at(p1, c)];
+    s2[c] = A.csrc[comp_at(p2, c)];
+    t2[c] = A.ctime[comp_at(p2, c)];
   }
 }
 
@@ -2122,7 +2120,6 @@ __device__ inline void load_comps2(const Arrays &A, int64_t p1, int64_t p2, int 
 // so the arrays stay in registers.
 __device__ inline void resample_components(const Arrays &A, Stream &s, int64_t p1, int64_t p2,
                                            int64_t dst, int n1, int n2, int np1, int np2) {
-  const int64_t tc = A.total_cap;
   int s1[MAX_COMP], s2[MAX_COMP];
   double t1[MAX_COMP], t2[MAX_COMP];
   load_comps2(A, p1, p2, n1, n2, s1, t1, s2, t2);
@@ -2136,8 +2133,8 @@ __device__ inline void resample_components(const Arrays &A, Stream &s, int64_t p
       if (c < n1 && m < n1_new) {
         const double r = s.uniform();
         if (!((double)(n1 - c) * r >= (double)(n1_new - m))) {
-          A.csrc[(int64_t)m * tc + dst] = s1[c];
-          A.ctime[(int64_t)m * tc + dst] = t1[c];
+          A.csrc[comp_at(dst, m)] = s1[c];
+          A.ctime[comp_at(dst, m)] = t1[c];
           m++;
         }
       }
@@ -2151,8 +2148,8 @@ __device__ inline void resample_components(const Arrays &A, Stream &s, int64_t p
       if (c < n2 && m < n2_new) {
         const double r = s.uniform();
         if (!((double)(n2 - c) * r >= (double)(n2_new - m))) {
-          A.csrc[(int64_t)(n1_new + m) * tc + dst] = s2[c];
-          A.ctime[(int64_t)(n1_new + m) * tc + dst] = t2[c];
+          A.csrc[comp_at(dst, n1_new + m)] = s2[c];
+          A.ctime[comp_at(dst, n1_new + m)] = t2[c];
           m++;
         }
       }
@@ -2198,12 +2195,12 @@ __device__ inline int direct_merge(const Arrays &A, const PW &W, BoxDesc *B, int
 #pragma unroll
     for (int c = 0; c < MAX_COMP; ++c) {
       if (c < n1) {
-        A.csrc[(int64_t)c * tc + dst] = s1[c];
-        A.ctime[(int64_t)c * tc + dst] = t1[c];
+        A.csrc[comp_at(dst, c)] = s1[c];
+        A.ctime[comp_at(dst, c)] = t1[c];
       }
       if (c < n2) {
-        A.csrc[(int64_t)(n1 + c) * tc + dst] = s2[c];
-        A.ctime[(int64_t)(n1 + c) * tc + dst] = t2[c];
+        A.csrc[comp_at(dst, n1 + c)] = s2[c];
+        A.ctime[comp_at(dst, n1 + c)] = t2[c];
       }
     }
   }

@@ -2,7 +2,9 @@
 //
 // All boxes share one structure-of-arrays particle store.  Box b owns the
 // particle slots [off_b, off_b + cap_b); species volumes are species-major
-// (vol[s * total_cap + p]) so per-species sweeps are coalesced.  Per-box
+// (vol[s * total_cap + p]) so per-species sweeps are coalesced; a
+// particle's components are particle-major (csrc[p * MAX_COMP + c]), since
+// they are only read and written a whole particle at a time.  Per-box
 // tables (bin grid, k bounds, pair tables, (bin, class) lists) live in
 // flat arrays with per-box strides fixed at handle creation.
 #pragma once
@@ -61,8 +63,8 @@ struct Arrays {
   double *vtot;              // [total_cap] cached total volume
   int64_t *id;
   int *group, *klass, *nprim, *ncomp;
-  int *csrc;                 // [MAX_COMP][total_cap]
-  double *ctime;             // [MAX_COMP][total_cap]
+  int *csrc;                 // [total_cap][MAX_COMP]: a particle's components are contiguous
+  double *ctime;             // [total_cap][MAX_COMP]
   double *least, *greatest;
   // exact-mode sorted structure (integer_rmap2 size_class)
   int *fwd_bin, *fwd_class, *fwd_idx;
@@ -79,6 +81,12 @@ struct Arrays {
   int64_t *info_id, *info_other;
   int *info_action;
 };
+
+// component c (0-based) of slot p: a particle's MAX_COMP components sit
+// together, so a merge reads and writes a few lines instead of one a
+// component
+__host__ __device__ inline int64_t comp_at(int64_t p, int c) { return p * MAX_COMP + c; }
+
 
 constexpr int SORT_THREADS = 256;
 
