@@ -15,17 +15,41 @@
 namespace pmc {
 
 // ------------------------------------------------------------ helpers
+// Species are read SPEC_CHUNK at a time, the loads of a chunk issued
+// before any is used (index clamped to the last species): a load in the
+// body of a loop with a runtime count costs one memory round trip each.
+// Sums run in species order, as one at a time would.
+constexpr int SPEC_CHUNK = 4;
+
 __device__ inline double particle_volume(const Arrays &A, int64_t p) {
+  const int ns = A.n_spec;
   double v = 0.0;
-  for (int s = 0; s < A.n_spec; ++s) v += A.vol[(int64_t)s * A.total_cap + p];
+  for (int s0 = 0; s0 < ns; s0 += SPEC_CHUNK) {
+    double x[SPEC_CHUNK];
+#pragma unroll
+    for (int u = 0; u < SPEC_CHUNK; ++u)
+      x[u] = A.vol[(int64_t)(s0 + u < ns ? s0 + u : ns - 1) * A.total_cap + p];
+#pragma unroll
+    for (int u = 0; u < SPEC_CHUNK; ++u)
+      if (s0 + u < ns) v += x[u];
+  }
   return v;
 }
 
 __device__ inline double particle_density(const Arrays &A, int64_t p) {
+  const int ns = A.n_spec;
   double m = 0.0, v = 0.0;
-  for (int s = 0; s < A.n_spec; ++s) {
-    double x = A.vol[(int64_t)s * A.total_cap + p];
-    m += x * A.density[s];
+  for (int s0 = 0; s0 < ns; s0 += SPEC_CHUNK) {
+    double x[SPEC_CHUNK], rho[SPEC_CHUNK];
+#pragma unroll
+    for (int u = 0; u < SPEC_CHUNK; ++u) {
+      const int s = s0 + u < ns ? s0 + u : ns - 1;
+      x[u] = A.vol[(int64_t)s * A.total_cap + p];
+      rho[u] = A.density[s];
+    }
+#pragma unroll
+    for (int u = 0; u < SPEC_CHUNK; ++u)
+      if (s0 + u < ns) m += x[u] * rho[u];
   }
   v = particle_volume(A, p);
   return m / v;

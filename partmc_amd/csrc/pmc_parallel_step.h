@@ -1004,7 +1004,7 @@ struct TestOut {
             // arrival entry of the first (second) list
 };
 
-__device__ inline void perset_test(BoxDesc *B, const Arrays &A, const PW &W, int box, int64_t k,
+__device__ __forceinline__ void perset_test(BoxDesc *B, const Arrays &A, const PW &W, int box, int64_t k,
                                    uint32_t t, int type, uint64_t seed, uint32_t step,
                                    int precision, double band, int validation, TestOut &o) {
   o.p1 = o.p2 = -1;
@@ -1133,8 +1133,11 @@ __device__ inline int merge_decide(const Arrays &A, const PW &W, const BoxDesc *
                                    int64_t p1, int64_t p2, int c1, int c2, int cc,
                                    int *new_group) {
   const Weights &w = B->w;
-  double r1 = vol2rad(particle_volume(A, p1));
-  double r2 = vol2rad(particle_volume(A, p2));
+  // the parents' volumes and lists, read together up front
+  const double v1 = particle_volume(A, p1), v2 = particle_volume(A, p2);
+  const int key_p1 = W.key[p1], key_p2 = W.key[p2];
+  double r1 = vol2rad(v1);
+  double r2 = vol2rad(v2);
   double rc = vol2rad(rad2vol(r1) + rad2vol(r2));
   double nc1 = nc_at_radius(w, c1, r1);
   double nc2 = nc_at_radius(w, c2, r2);
@@ -1158,7 +1161,7 @@ __device__ inline int merge_decide(const Arrays &A, const PW &W, const BoxDesc *
     id1_lost = false;
     id2_lost = false;
     if (rm1 && rm2) {
-      if (particle_volume(A, p1) > particle_volume(A, p2)) id2_lost = true;
+      if (v1 > v2) id2_lost = true;
       else id1_lost = true;
     }
     if (rm1 && !id1_lost) o |= O_ID1;
@@ -1171,7 +1174,7 @@ __device__ inline int merge_decide(const Arrays &A, const PW &W, const BoxDesc *
     const int nb = B->n_bin;
     const int b_new = bin_find_log(nb, B->log_lo, B->log_hi, rc);
     const int k_new = (b_new >= 1 && b_new <= nb) ? par_key(b_new, cc, nb) : -1;
-    if (rm1 && rm2) o |= (W.key[p2] == k_new && W.key[p1] != k_new) ? O_SLOT2 : O_SLOT1;
+    if (rm1 && rm2) o |= (key_p2 == k_new && key_p1 != k_new) ? O_SLOT2 : O_SLOT1;
     else if (rm1) o |= O_SLOT1;
     else if (rm2) o |= O_SLOT2;
     else o |= O_APPEND;
@@ -2173,13 +2176,25 @@ __device__ inline int direct_merge(const Arrays &A, const PW &W, BoxDesc *B, int
   const double least = fmin(A.least[p1], A.least[p2]);
   const double greatest = fmax(A.greatest[p1], A.greatest[p2]);
   // dst may be a parent's slot: each species' sum depends only on its own
-  // column, so it is written as it is formed
+  // row, so it is written once formed; a chunk's loads go out together
   double v = 0.0, m = 0.0;
-  for (int s = 0; s < A.n_spec; ++s) {
-    const double x = A.vol[(int64_t)s * tc + p1] + A.vol[(int64_t)s * tc + p2];
-    A.vol[(int64_t)s * tc + dst] = x;
-    v += x;
-    m += x * A.density[s];
+  const int ns = A.n_spec;
+  for (int s0 = 0; s0 < ns; s0 += SPEC_CHUNK) {
+    double x[SPEC_CHUNK], rho[SPEC_CHUNK];
+#pragma unroll
+    for (int u = 0; u < SPEC_CHUNK; ++u) {
+      const int s = s0 + u < ns ? s0 + u : ns - 1;
+      x[u] = A.vol[(int64_t)s * tc + p1] + A.vol[(int64_t)s * tc + p2];
+      rho[u] = A.density[s];
+    }
+#pragma unroll
+    for (int u = 0; u < SPEC_CHUNK; ++u) {
+      if (s0 + u < ns) {
+        A.vol[(int64_t)(s0 + u) * tc + dst] = x[u];
+        v += x[u];
+        m += x[u] * rho[u];
+      }
+    }
   }
   // components: the first parent's, then the second's, all loaded before
   // any is written (dst may be a parent's slot), so the loads go out
