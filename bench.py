@@ -286,8 +286,16 @@ def measured_traffic(workload, kernel):
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        k = json.load(f)["kernels"].get(kernel)
-    return None if k is None else k["bytes"]
+        ks = json.load(f)["kernels"]
+    k = ks.get(kernel)
+    if k is not None:
+        return k["bytes"]
+    # a templated kernel appears once per instantiation ("void k<false>"):
+    # the launches of all of them, averaged
+    parts = [v for name, v in ks.items()
+             if name.split("<")[0].replace("void ", "").strip() == kernel and "<" in name]
+    n = sum(v["dispatches"] for v in parts)
+    return sum(v["bytes"] * v["dispatches"] for v in parts) / n if n else None
 
 
 def main():
