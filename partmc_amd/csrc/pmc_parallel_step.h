@@ -1006,7 +1006,8 @@ struct TestOut {
 
 __device__ __forceinline__ void perset_test(BoxDesc *B, const Arrays &A, const PW &W, int box, int64_t k,
                                    uint32_t t, int type, uint64_t seed, uint32_t step,
-                                   int precision, double band, int validation, TestOut &o) {
+                                   int precision, double band, int validation, TestOut &o,
+                                        const Weights *wl = nullptr) {
   o.p1 = o.p2 = -1;
   o.accept = false;
   o.warn = false;
@@ -1015,7 +1016,8 @@ __device__ __forceinline__ void perset_test(BoxDesc *B, const Arrays &A, const P
   decode_pair(k % W.P_stride, W.tab_classes, W.max_bin, &c1, &c2, &b1, &b2);
   const uint32_t j = t - W.pair_exc[k];
   const uint32_t tau = W.pair_exc[k];  // where the pair begins in the sequence
-  const int nb = B->n_bin, nc = B->w.n_class;
+  const Weights &w = wl ? *wl : B->w;
+  const int nb = B->n_bin, nc = w.n_class;
   const int64_t ti = table_index(A, box, nb, nc, b1, b2, c1, c2);
   const int cc = A.cc[ti];
   const double accept_factor = 1.0 / A.kw[ti];
@@ -1071,7 +1073,6 @@ __device__ __forceinline__ void perset_test(BoxDesc *B, const Arrays &A, const P
   }
   o.p1 = p1;
   o.p2 = p2;
-  const Weights &w = B->w;
   const Env env = B->env;
   double u = ts.uniform();
   double pr;
@@ -1129,10 +1130,12 @@ __device__ inline MergeCtx merge_ctx(BoxDesc *boxes, const PW &W, int n_box, uin
 }
 
 // coagulate_weighting (src/coagulation.F90:810-922): decisions only
+// wl: the box's weights staged in LDS by the caller, or null to read them
+// from the box
 __device__ inline int merge_decide(const Arrays &A, const PW &W, const BoxDesc *B, Stream &s,
                                    int64_t p1, int64_t p2, int c1, int c2, int cc,
-                                   int *new_group) {
-  const Weights &w = B->w;
+                                   int *new_group, const Weights *wl = nullptr) {
+  const Weights &w = wl ? *wl : B->w;
   // the parents' volumes and lists, read together up front
   const double v1 = particle_volume(A, p1), v2 = particle_volume(A, p2);
   const int key_p1 = W.key[p1], key_p2 = W.key[p2];
@@ -2787,6 +2790,10 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, const uint32_t *pend, int 
   __shared__ int s_tot[4];
   __shared__ long long s_hole_base;
   __shared__ int s_err;
+  // the box's weights, read by every test and merge decision of the tail
+  __shared__ Weights s_w;
+  for (int q = tid; q < (int)(sizeof(Weights) / sizeof(int)); q += TAIL_THREADS)
+    ((int *)&s_w)[q] = ((const int *)&B->w)[q];
   extern __shared__ int s_live[];  // live-count changes of the box's lists
   // then, per bin, the lowest index of an unsettled test of the window whose
   // results land in that bin or a lower one (a running minimum)
@@ -2839,7 +2846,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, const uint32_t *pend, int 
         p2 = -1;
       } else {
         TestOut o;
-        perset_test(B, A, W, box, k, t, type, seed, step, precision, band, 0, o);
+        perset_test(B, A, W, box, k, t, type, seed, step, precision, band, 0, o, &s_w);
         p1 = o.p1;
         p2 = o.p2;
         W.thin[t - batch_t0] = o.thin ? 1 : 0;
@@ -2861,7 +2868,7 @@ k_par_tail(BoxDesc *boxes, Arrays A, PW W, int n_box, const uint32_t *pend, int 
       const int cc = A.cc[table_index(A, box, nb, nc, b1, b2, c1, c2)];
       ms.open(seed, box, step, pair_site(nb, nc, b1, b2, c1, c2), t - W.pair_exc[k], RNG_MERGE);
       int g;
-      o_win = merge_decide(A, W, B, ms, p1, p2, c1, c2, cc, &g) | (g << 16);
+      o_win = merge_decide(A, W, B, ms, p1, p2, c1, c2, cc, &g, &s_w) | (g << 16);
     }
     // 2. claims
     if (effect) {
